@@ -1,0 +1,179 @@
+"""ctypes wrapper around oracle/liboracle.so -- the CPU restatement of the hot path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg, always as the checker, never as the thing measured or shipped.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built; run `make -C oracle`")
+        L = C.CDLL(path)
+        L.oracle_new.restype = C.c_void_p
+        L.oracle_new.argtypes = [C.c_int] * 5
+        L.oracle_set_threads.argtypes = [C.c_void_p, C.c_int]
+        L.oracle_set_params.argtypes = [C.c_void_p, _dp, _ip]
+        L.oracle_set_mesh.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.c_void_p, _ip]
+        L.oracle_set_open.argtypes = [C.c_void_p, C.c_int, _ip, _ip]
+        L.oracle_set_land.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip]
+        L.oracle_set_tide.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_set_flow.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_prep.argtypes = [C.c_void_p]
+        L.oracle_prep.restype = C.c_int
+        L.oracle_step.argtypes = [C.c_void_p, C.c_int]
+        L.oracle_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.oracle_rhs.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.oracle_modal2nodal.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.oracle_get_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _ip]
+        L.oracle_set_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _ip]
+        L.oracle_get_rhs.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
+        L.oracle_clear_rhs.argtypes = [C.c_void_p]
+        L.oracle_get_nodecode.argtypes = [C.c_void_p, _ip]
+        L.oracle_num_edges.argtypes = [C.c_void_p, _ip]
+        L.oracle_num_edges.restype = C.c_int
+        L.oracle_get_edges.argtypes = [C.c_void_p, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _dp]
+        L.oracle_get_tables.argtypes = [C.c_void_p] + [_dp] * 12
+        L.oracle_mass.argtypes = [C.c_void_p]
+        L.oracle_mass.restype = C.c_double
+        L.oracle_free.argtypes = [C.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Oracle:
+    """The reference's serial time step, restated in C, driven from a dgswem_amd.fortio.Case."""
+
+    def __init__(self, case, threads: int = 1):
+        L = lib()
+        p = case.params
+        self.case = case
+        self.ne, self.np_, self.dofh = case.ne, case.np_, case.dofh
+        self.nrk = case.dg.rk_stage
+        self.h = L.oracle_new(case.np_, case.ne, case.dg.ph, case.dg.rk_stage, case.dg.rk_order)
+        L.oracle_set_threads(self.h, threads)
+        d = _f([p.g, p.dtdp, p.statim, p.dramp, p.h0, p.cf, p.hbreak, p.ftheta, p.fgamma, p.sfea0])
+        i = _i([p.ics, p.nolibf, p.nolifa, p.nolica, p.nolicat, p.nramp, case.dg.slopeflag,
+                case.dg.rainfall])
+        L.oracle_set_params(self.h, d, i)
+        man = _f(case.manning) if case.manning is not None else None
+        self._keep = [man]
+        L.oracle_set_mesh(self.h, _f(case.x), _f(case.y), _f(case.dp), _f(case.sfac),
+                          _f(case.corif), man.ctypes.data if man is not None else None,
+                          _i(case.nm.reshape(-1)))
+        ol = _i([len(s) for s in case.open_segs]) if case.open_segs else _i([0])
+        on = _i(np.concatenate(case.open_segs)) if case.open_segs else _i([0])
+        L.oracle_set_open(self.h, len(case.open_segs), ol, on)
+        lt = _i([t for t, _ in case.land_segs]) if case.land_segs else _i([0])
+        ll = _i([len(s) for _, s in case.land_segs]) if case.land_segs else _i([0])
+        ln = _i(np.concatenate([s for _, s in case.land_segs])) if case.land_segs else _i([0])
+        L.oracle_set_land(self.h, len(case.land_segs), lt, ll, ln)
+        nb = p.nbfr
+        z1 = _f(np.zeros(1))
+        L.oracle_set_tide(self.h, nb, _f(p.amig) if nb else z1, _f(p.ff) if nb else z1,
+                          _f(p.face) if nb else z1, _f(p.per) if nb else z1,
+                          _f(p.emo.reshape(-1)) if nb else z1, _f(p.efa.reshape(-1)) if nb else z1)
+        nf = p.nffr
+        if nf:
+            L.oracle_set_flow(self.h, nf, p.qnam.shape[1], _f(p.famig), _f(p.fff), _f(p.fface),
+                              _f(p.fper), _f(p.qnam.reshape(-1)), _f(p.qnph.reshape(-1)))
+        else:
+            L.oracle_set_flow(self.h, 0, 0, z1, z1, z1, z1, z1, z1)
+        rc = L.oracle_prep(self.h)
+        if rc != 0:
+            raise RuntimeError("oracle_prep failed")
+        self.it = 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_free(self.h)
+            self.h = None
+
+    def step(self, n: int = 1):
+        lib().oracle_run(self.h, self.it + 1, n)
+        self.it += n
+
+    def state(self):
+        ze = np.empty(self.ne * self.dofh); qx = np.empty_like(ze); qy = np.empty_like(ze)
+        wd = np.empty(self.ne, dtype=np.int32)
+        lib().oracle_get_state(self.h, ze, qx, qy, wd)
+        s = (self.ne, self.dofh)
+        return ze.reshape(s), qx.reshape(s), qy.reshape(s), wd
+
+    def set_state(self, ze, qx, qy, wdflg):
+        lib().oracle_set_state(self.h, _f(ze).reshape(-1), _f(qx).reshape(-1), _f(qy).reshape(-1),
+                               _i(wdflg))
+
+    def rhs(self, it: int, irk: int = 0):
+        """RHS of stage irk at step it for the current stage-1 state (only valid for irk=0)."""
+        L = lib()
+        L.oracle_clear_rhs(self.h)
+        L.oracle_rhs(self.h, it, irk)
+        n = self.ne * self.dofh
+        a, b, c = np.empty(n), np.empty(n), np.empty(n)
+        L.oracle_get_rhs(self.h, irk, a, b, c)
+        L.oracle_clear_rhs(self.h)
+        s = (self.ne, self.dofh)
+        return a.reshape(s), b.reshape(s), c.reshape(s)
+
+    def nodal(self):
+        e, u, v = np.empty(self.np_), np.empty(self.np_), np.empty(self.np_)
+        lib().oracle_modal2nodal(self.h, e, u, v)
+        nc = np.empty(self.np_, dtype=np.int32)
+        lib().oracle_get_nodecode(self.h, nc)
+        return e, u, v, nc
+
+    def edges(self):
+        L = lib()
+        cnt = np.zeros(5, dtype=np.int32)
+        L.oracle_num_edges(self.h, cnt)
+        ned, ni, nl, no, nf = [int(c) for c in cnt]
+        out = dict(nedno=np.empty((ned, 2), np.int32), nedel=np.empty((ned, 2), np.int32),
+                   nedsd=np.empty((ned, 2), np.int32), niedn=np.empty(max(ni, 1), np.int32),
+                   nledn=np.empty(max(nl, 1), np.int32), needn=np.empty(max(no, 1), np.int32),
+                   nfedn=np.empty(max(nf, 1), np.int32), xlen=np.empty(ned), cosnx=np.empty(ned),
+                   sinnx=np.empty(ned))
+        L.oracle_get_edges(self.h, out["nedno"], out["nedel"], out["nedsd"], out["niedn"],
+                           out["nledn"], out["needn"], out["nfedn"], out["xlen"], out["cosnx"],
+                           out["sinnx"])
+        out["niedn"] = out["niedn"][:ni]; out["nledn"] = out["nledn"][:nl]
+        out["needn"] = out["needn"][:no]; out["nfedn"] = out["nfedn"][:nf]
+        return out
+
+    def tables(self):
+        d, ng, na, nrk = self.dofh, self.case.dg.ph + 2, (3 if self.case.dg.ph == 1 else 6), self.nrk
+        t = dict(phi_area=np.empty(na * d), phi_edge=np.empty(3 * ng * d), phi_corner=np.empty(3 * d),
+                 m_inv=np.empty(d), xegp=np.empty(ng), wegp=np.empty(ng), xagp=np.empty(na),
+                 yagp=np.empty(na), wagp=np.empty(na), atvd=np.empty(nrk * nrk),
+                 btvd=np.empty(nrk * nrk), dtvd=np.empty(nrk))
+        lib().oracle_get_tables(self.h, *t.values())
+        t["phi_area"] = t["phi_area"].reshape(na, d)
+        t["phi_edge"] = t["phi_edge"].reshape(3, ng, d)
+        t["phi_corner"] = t["phi_corner"].reshape(3, d)
+        t["atvd"] = t["atvd"].reshape(nrk, nrk); t["btvd"] = t["btvd"].reshape(nrk, nrk)
+        return t
+
+    def mass(self) -> float:
+        return lib().oracle_mass(self.h)
