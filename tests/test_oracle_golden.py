@@ -1,0 +1,39 @@
+"""The oracle against the reference: FP64 states from dgswem_serial and the reference's fort.63.
+
+The golden states were written by tests/golden/make_golden.py from dgswem_serial built from the
+reference sources (DG hot start, 17 significant digits, so the comparison is exact in FP64).
+"""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_STATES, golden, load_case
+from dgswem_amd import fortio
+from oracle.oracle import Oracle
+
+
+@pytest.mark.parametrize("name,steps", GOLDEN_STATES)
+def test_state_matches_reference(name, steps):
+    case = load_case(name)
+    o = Oracle(case, threads=4)
+    o.step(steps)
+    ze, qx, qy, wd = o.state()
+    g = golden(f"{name}_step{steps}")
+    # identical operation order and no FMA on both sides: the states agree bit for bit
+    np.testing.assert_array_equal(wd, g["wdflg"])
+    for a, b in ((ze, g["ze"]), (qx, g["qx"]), (qy, g["qy"])):
+        np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("name", ["quarter_annular", "wetdry", "mass_conservation"])
+def test_last_fort63_snapshot(name):
+    """The reference's own regression test (tests/test_run.py, tests/util.py:30-53), at 8 digits."""
+    case = load_case(name)
+    ref = golden(f"{name}_fort63_last")
+    o = Oracle(case, threads=4)
+    o.step(int(ref["it"]))
+    eta, _, _, nc = o.nodal()
+    if case.params.nolifa == 2:
+        eta = np.where(nc == 0, -99999.0, eta)
+    printed = np.array([float(fortio._fmt_e16_8e3(v)) for v in eta])
+    np.testing.assert_array_equal(printed == -99999.0, ref["eta"] == -99999.0)
+    np.testing.assert_array_equal(printed, ref["eta"])
