@@ -1,0 +1,126 @@
+// Device-side data structures and the numerical flux of the DG hydro step (gfx950, FP64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace dg {
+
+constexpr int KMAXDOF = 6;
+constexpr int KMAXAGP = 6;
+constexpr int KMAXEGP = 4;
+constexpr int KMAXRK = 8;
+
+// Per-slot side descriptor packed 7 bits per slot into one int32 per element:
+//   bits 0-1 local edge L (edge L runs from vertex L+1 to L+2, mod 3)
+//   bit  2   role: 0 = this element is NEDEL(1) ("in"), 1 = NEDEL(2) ("ex")
+//   bits 3-4 kind: 0 internal, 1 elevation-specified (ocean), 2 no-normal-flow (land), 3 flux
+//   bits 5-6 neighbour's local edge (internal edges)
+// Slots are stored in the reference's accumulation order: boundary edges first (ocean, land,
+// flux, each in its list order), then internal edges by global edge number.
+enum SideKind { SIDE_INTERNAL = 0, SIDE_OCEAN = 1, SIDE_LAND = 2, SIDE_FLOW = 3 };
+
+__host__ __device__ inline int side_bits(int info, int slot) { return (info >> (7 * slot)) & 127; }
+
+// Reference-element tables in constant memory (filled from RefTables at init).
+struct DevTables {
+    double phi_area[KMAXAGP][KMAXDOF];
+    double drphi[KMAXAGP][KMAXDOF];
+    double dsphi[KMAXAGP][KMAXDOF];
+    double mw[KMAXAGP][KMAXDOF];
+    double srfac[KMAXAGP][KMAXDOF];
+    double phi_edge[3][KMAXEGP][KMAXDOF];
+    double edgeq[3][KMAXEGP][KMAXDOF];
+    double phi_corner[3][KMAXDOF];
+    double m_inv[KMAXDOF];
+    double wegp[KMAXEGP];
+    double xegp[KMAXEGP];
+};
+
+// Physics switches and constants (src/dgswem.F:131-170 derivations).
+struct Phys {
+    double g, nleq, leq, nleqg, fg_l;
+    double cf, hbreak, ftheta, fgamma, dtdph, h0;
+    double cori;       // uniform Coriolis (used when cori_el == nullptr)
+    double fric;       // uniform FRIC_EL (used without Manning)
+    int ifnlfa, iflinbf, ifhybf, load_manning, rainfall;
+};
+
+// Element-major structure-of-arrays mesh data on the device.
+struct ElemData {
+    int ne;
+    const double* __restrict__ xv;      // [3][ne] vertex x
+    const double* __restrict__ yv;      // [3][ne] vertex y
+    const double* __restrict__ dpv;     // [3][ne] vertex depth DP
+    const double* __restrict__ nbar;    // [ne] mean nodal Manning n, or nullptr
+    const double* __restrict__ cori_el; // [ne] or nullptr
+    const double* __restrict__ sfacq;   // [nagp][ne] spherical factor at area points, or nullptr
+    const double* __restrict__ sfaced;  // [3][negp][ne] spherical factor at edge points, or nullptr
+    const int* __restrict__ sinfo;      // [ne] packed side descriptors
+    const int* __restrict__ snbr;       // [3][ne] neighbour element / boundary-list index per slot
+    const int* __restrict__ nm;         // [3][ne] 0-based vertex nodes
+};
+
+// Boundary forcing.  Per-stage scalars come from host-built tables indexed by the stage counter.
+struct Forcing {
+    int nbfr, nffr;
+    const double* __restrict__ emo;     // [needs][nbfr][2]
+    const double* __restrict__ efa;     // [needs][nbfr][2]
+    const double* __restrict__ qnam;    // [nfeds][nffr][2]
+    const double* __restrict__ qnph;    // [nfeds][nffr][2]
+    const double* __restrict__ argj;    // [nstage][nbfr]
+    const double* __restrict__ rff;     // [nstage][nbfr]
+    const double* __restrict__ fargj;   // [nstage][nffr]
+    const double* __restrict__ frff;    // [nstage][nffr]
+};
+
+// Local Lax-Friedrichs flux (src/numerical_flux.F:750-1022), operation order kept.
+__device__ __forceinline__ void llf_flux(const Phys& P, double ze_in, double ze_ex, double qx_in,
+                                         double qx_ex, double qy_in, double qy_ex, double hb_in,
+                                         double hb_ex, double sf_in, double sf_ex, double nx,
+                                         double ny, double G, double& f0, double& f1, double& f2) {
+    const double NLEQ = P.nleq, LEQ = P.leq, FG_L = P.fg_l;
+    const double NLEQG = G * NLEQ;
+    const double j1 = ze_ex - ze_in, j2 = qx_ex - qx_in, j3 = qy_ex - qy_in;
+    const double ht_in = ze_in * NLEQ + hb_in;
+    double fnl = NLEQ + LEQ * hb_in;
+    const double fx1_in = qx_in * fnl * sf_in, fy1_in = qy_in * fnl;
+    double fu = NLEQ * qx_in, fv = NLEQ * qy_in, fg = NLEQG * ze_in;
+    double fh = 1.0 / (NLEQ * ht_in + LEQ);
+    const double u_in = qx_in * fh, v_in = qy_in * fh;
+    double huu = fu * u_in, hvv = fv * v_in, huv = fu * v_in;
+    double gh2 = fg * (0.5 * ze_in + hb_in) + FG_L * ze_in;
+    const double fx2_in = (huu + gh2) * sf_in, fy2_in = huv;
+    const double fx3_in = huv * sf_in, fy3_in = hvv + gh2;
+    const double ht_ex = ze_ex * NLEQ + hb_ex;
+    fnl = NLEQ + LEQ * hb_ex;
+    const double fx1_ex = qx_ex * fnl * sf_ex, fy1_ex = qy_ex * fnl;
+    fu = NLEQ * qx_ex; fv = NLEQ * qy_ex; fg = NLEQG * ze_ex;
+    fh = 1.0 / (NLEQ * ht_ex + LEQ);
+    const double u_ex = qx_ex * fh, v_ex = qy_ex * fh;
+    huu = fu * u_ex; hvv = fv * v_ex; huv = fu * v_ex;
+    gh2 = fg * (0.5 * ze_ex + hb_ex) + FG_L * ze_ex;
+    const double fx2_ex = (huu + gh2) * sf_ex, fy2_ex = huv;
+    const double fx3_ex = huv * sf_ex, fy3_ex = hvv + gh2;
+    const double a1 = 0.5 * ((fx1_in + fx1_ex) * nx + (fy1_in + fy1_ex) * ny);
+    const double a2 = 0.5 * ((fx2_in + fx2_ex) * nx + (fy2_in + fy2_ex) * ny);
+    const double a3 = 0.5 * ((fx3_in + fx3_ex) * nx + (fy3_in + fy3_ex) * ny);
+    const double un_in = (u_in * nx + v_in * ny) * NLEQ;
+    double nxs = nx * sf_in;
+    const double c_in = sqrt(G * ht_in * (ny * ny + nxs * nxs));
+    const double un_ex = (u_ex * nx + v_ex * ny) * NLEQ;
+    nxs = nx * sf_ex;
+    const double c_ex = sqrt(G * ht_ex * (ny * ny + nxs * nxs));
+    double em = fabs(un_in + c_in);
+    em = fmax(em, fabs(un_in));
+    em = fmax(em, fabs(un_in - c_in));
+    em = fmax(em, fabs(un_ex + c_ex));
+    em = fmax(em, fabs(un_ex));
+    em = fmax(em, fabs(un_ex - c_ex));
+    const double nx2 = NLEQ + LEQ * nx * nx, ny2 = NLEQ + LEQ * ny * ny, nxy = nx * ny;
+    f0 = a1 - 0.5 * em * (j1);
+    f1 = a2 - 0.5 * em * (j2 * nx2 + LEQ * j3 * nxy);
+    f2 = a3 - 0.5 * em * (j3 * ny2 + LEQ * j2 * nxy);
+}
+
+}  // namespace dg

@@ -1,0 +1,675 @@
+// HIP kernels of the DG hydro step for gfx950 (MI355X), FP64.
+//
+// One time step = NRK stages; each stage is
+//   stage_kernel     one thread per element: fluxes on its three edges (ocean / land / flux /
+//                    internal, LLF), volume quadrature, SSP-RK update of the element's modes, and
+//                    (when the slope limiter is off) the element's wet/dry correction.  Internal
+//                    edge fluxes are evaluated by both neighbours with the same operands in the same
+//                    order, so both see bit-identical values and no scatter or atomics are needed.
+//   node_minmax      one thread per node: min/max of the updated cell means over incident elements
+//   limit_wetdry     one thread per element: SLOPELIMITER5 on modes 2-3, then WETDRY
+// plus modal2nodal (one thread per node) on output steps.
+//
+// Arithmetic follows the reference statement by statement (listed per kernel) and is compiled with
+// -ffp-contract=off, so each product and sum is rounded as in dgswem_serial.
+#include "dg_device.hpp"
+
+namespace dg {
+
+__constant__ DevTables c_tab1;   // p = 1
+__constant__ DevTables c_tab2;   // p = 2
+
+template <int P> __device__ __forceinline__ const DevTables& tab();
+template <> __device__ __forceinline__ const DevTables& tab<1>() { return c_tab1; }
+template <> __device__ __forceinline__ const DevTables& tab<2>() { return c_tab2; }
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of element
+// tiles so that a tile's neighbours (close in the Hilbert order) sit in the same XCD's L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
+struct StageArgs {
+    ElemData ed;
+    Phys ph;
+    Forcing fo;
+    const double* __restrict__ uin;      // U(IRK) [3][dof][ne]
+    double* __restrict__ uout;           // U(IRK+1)
+    const uint8_t* __restrict__ wd_in;   // WDFLG during the stage
+    uint8_t* __restrict__ wd_out;        // WDFLG after WETDRY (fused variant)
+    int nprev;                           // stages I < IRK entering the update
+    const double* uprev[KMAXRK];
+    double aprev[KMAXRK];                // ATVD(IRK,I), 0 = term absent
+    const double* rprev[KMAXRK];
+    double bprev[KMAXRK];                // BTVD(IRK,I)*DT, 0 = term absent
+    double acur, bcur;                   // ATVD(IRK,IRK), BTVD(IRK,IRK)*DT
+    double* __restrict__ rout;           // RHS(IRK) kept for later stages, or nullptr
+    int stage;                           // row of the forcing tables
+    double rain;                         // rain source rate (rainfall = 1), else 0
+};
+
+// ------------------------------------------------------------------------------------------
+// WETDRY for one element (src/wetdry.F:58-548, p=1 vertex logic; modes >= 4 zeroed).
+// Updates ze/qx/qy in place and returns the new WDFLG.
+// ------------------------------------------------------------------------------------------
+template <int P>
+__device__ __forceinline__ int wetdry_element(const DevTables& T, double (&U)[3][(P + 1) * (P + 2) / 2],
+                                              const double dpn[3], const double hb[3], double dpe_min,
+                                              int wdflg, double H0) {
+    constexpr int DOF = (P + 1) * (P + 2) / 2;
+    const double ZERO = 1.0e-6, C13 = 1.0 / 3.0, C16 = 1.0 / 6.0;
+    int ndry = 0, ht_max_i = 0;
+    double ht_max = ZERO;
+    double ze_node[3], ht_node[3], ze_hat[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        double z = U[0][0];
+#pragma unroll
+        for (int k = 1; k < 3; k++) z = z + T.phi_corner[i][k] * U[0][k];
+        ze_node[i] = z;
+        ht_node[i] = dpn[i] + z;
+        if (ht_node[i] > ht_max) { ht_max_i = i; ht_max = ht_node[i]; }
+        if (ht_node[i] <= (H0 + ZERO)) ndry++;
+    }
+    const double dp_avg = C13 * (dpn[0] + dpn[1] + dpn[2]);
+    int check = 0;
+    if (ndry == 0) {
+        if (wdflg == 1) return 1;
+        ze_hat[0] = ze_node[0]; ze_hat[1] = ze_node[1]; ze_hat[2] = ze_node[2];
+        check = 1;
+    } else if (ndry == 3) {
+        const double ht_avg = U[0][0] + dp_avg;
+#pragma unroll
+        for (int k = 0; k < DOF; k++) { U[0][k] = 0.0; U[1][k] = 0.0; U[2][k] = 0.0; }
+        U[0][0] = ht_avg - dp_avg;
+        if (ht_avg < ZERO) U[0][0] = ZERO - dp_avg;
+        U[0][1] = -hb[1]; U[0][2] = -hb[2];
+        return 0;
+    } else {
+        const double ht_avg = U[0][0] + dp_avg;
+        if (ht_avg <= (H0 + ZERO)) {
+#pragma unroll
+            for (int k = 0; k < DOF; k++) { U[0][k] = 0.0; U[1][k] = 0.0; U[2][k] = 0.0; }
+            U[0][0] = ht_avg - dp_avg;
+            if (ht_avg < ZERO) U[0][0] = ZERO - dp_avg;
+            U[0][1] = -hb[1]; U[0][2] = -hb[2];
+            ze_hat[0] = ze_hat[1] = ze_hat[2] = U[0][0];
+        } else {
+            // order the vertices by depth (the reference's exchange sort)
+            int l2h[3] = {0, 1, 2};
+#pragma unroll
+            for (int k1 = 0; k1 < 3; k1++)
+#pragma unroll
+                for (int k2 = k1; k2 < 3; k2++)
+                    if (ht_node[l2h[k1]] > ht_node[l2h[k2]]) { const int t = l2h[k1]; l2h[k1] = l2h[k2]; l2h[k2] = t; }
+            double dh[3];
+            dh[l2h[0]] = fmax(H0, ht_node[l2h[0]]) - ht_node[l2h[0]];
+            dh[l2h[1]] = fmax(H0, ht_node[l2h[1]] - 0.5 * dh[l2h[0]]) - ht_node[l2h[1]];
+            dh[l2h[2]] = -dh[l2h[0]] - dh[l2h[1]];
+            int nd = 0;
+            double qxs = 0, qys = 0;
+            ht_max = 0;
+            double ht_hat[3], qxn[3], qyn[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                ht_hat[i] = ht_node[i] + dh[i];
+                double a = U[1][0], b = U[2][0];
+                if (ht_hat[i] > ht_max) { ht_max_i = i; ht_max = ht_hat[i]; }
+#pragma unroll
+                for (int k = 1; k < 3; k++) { a = a + T.phi_corner[i][k] * U[1][k]; b = b + T.phi_corner[i][k] * U[2][k]; }
+                qxn[i] = a; qyn[i] = b;
+                if (ht_hat[i] <= (H0 + 1.0e-10)) { nd++; qxs = qxs + a; qys = qys + b; }
+            }
+            if (nd == 3) {
+#pragma unroll
+                for (int k = 0; k < DOF; k++) { U[1][k] = 0.0; U[2][k] = 0.0; }
+            } else {
+                qxs = qxs / (3.0 - (double)nd);
+                qys = qys / (3.0 - (double)nd);
+                double qxh[3], qyh[3];
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    if (ht_hat[i] <= (H0 + 1.0e-10)) { qxh[i] = 0.0; qyh[i] = 0.0; }
+                    else { qxh[i] = qxn[i] + qxs; qyh[i] = qyn[i] + qys; }
+                }
+                U[1][0] = C13 * (qxh[0] + qxh[1] + qxh[2]);
+                U[1][1] = -(C16 * (qxh[0] + qxh[1])) + C13 * qxh[2];
+                U[1][2] = -(0.5 * qxh[0]) + 0.5 * qxh[1];
+                U[2][0] = C13 * (qyh[0] + qyh[1] + qyh[2]);
+                U[2][1] = -(C16 * (qyh[0] + qyh[1])) + C13 * qyh[2];
+                U[2][2] = -(0.5 * qyh[0]) + 0.5 * qyh[1];
+            }
+#pragma unroll
+            for (int i = 0; i < 3; i++) ze_hat[i] = ht_hat[i] - dpn[i];
+            U[0][1] = -(C16 * (ze_hat[0] + ze_hat[1])) + C13 * ze_hat[2];
+            U[0][2] = -(0.5 * ze_hat[0]) + 0.5 * ze_hat[1];
+        }
+#pragma unroll
+        for (int k = 3; k < DOF; k++) { U[0][k] = 0.0; U[1][k] = 0.0; U[2][k] = 0.0; }
+        check = 1;
+    }
+    if (check) {
+        if (ze_hat[ht_max_i] > (-dpe_min + H0)) return 1;
+#pragma unroll
+        for (int k = 0; k < DOF; k++) { U[1][k] = 0.0; U[2][k] = 0.0; }
+        return 0;
+    }
+    return wdflg;
+}
+
+// ------------------------------------------------------------------------------------------
+// Stage kernel.  Reference statements restated:
+//   OCEAN_EDGE_HYDRO   src/ocean_edge_hydro.F:19-355
+//   LAND_EDGE_HYDRO    src/land_edge_hydro.F:26-311
+//   FLOW_EDGE_HYDRO    src/flow_edge_hydro.F:20-333
+//   INTERNAL_EDGE_HYDRO + add_internal_flux  src/internal_edge_hydro.F:37-383
+//   RHS_DG_HYDRO       src/rhs_dg_hydro.F:18-473
+//   SSP-RK update      src/DG_hydro_timestep.F:238-290
+//   element geometry   src/read_input.F:1408, src/prep_DG.F:959-1163, src/calc_normal.F:28-49
+// ------------------------------------------------------------------------------------------
+template <int P, bool SPH, bool WDFUSED>
+__global__ void __launch_bounds__(256) stage_kernel(StageArgs a) {
+    constexpr int DOF = (P + 1) * (P + 2) / 2;
+    constexpr int NAGP = (P == 1) ? 3 : 6;
+    constexpr int NEGP = P + 2;
+    const DevTables& T = tab<P>();
+    const ElemData& E = a.ed;
+    const Phys& ph = a.ph;
+    const int ne = E.ne;
+    const int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+
+    double U[3][DOF];
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+#pragma unroll
+        for (int k = 0; k < DOF; k++) U[f][k] = a.uin[(size_t)(f * DOF + k) * ne + e];
+    const int wd = a.wd_in[e];
+    double X[3], Y[3], D[3];
+#pragma unroll
+    for (int v = 0; v < 3; v++) {
+        X[v] = E.xv[(size_t)v * ne + e]; Y[v] = E.yv[(size_t)v * ne + e]; D[v] = E.dpv[(size_t)v * ne + e];
+    }
+    const double AREAS = (X[0] - X[2]) * (Y[1] - Y[2]) + (X[2] - X[1]) * (Y[0] - Y[2]);
+    const double c13 = 1.0 / 3.0, c16 = 1.0 / 6.0;
+    const double hb[3] = {c13 * (D[0] + D[1] + D[2]), -(c16 * (D[0] + D[1])) + c13 * D[2],
+                          -(0.5 * D[0]) + 0.5 * D[1]};
+
+    double R[3][DOF];    // boundary-edge part of the RHS (accumulated first)
+    double S[3][DOF];    // internal-edge slot sum
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+#pragma unroll
+        for (int k = 0; k < DOF; k++) { R[f][k] = 0.0; S[f][k] = 0.0; }
+
+    const int info = E.sinfo[e];
+#pragma unroll 1
+    for (int slot = 0; slot < 3; slot++) {
+        const int bits = side_bits(info, slot);
+        const int L = bits & 3, role = (bits >> 2) & 1, kind = (bits >> 3) & 3, nled = (bits >> 5) & 3;
+        const int nb = E.snbr[(size_t)slot * ne + e];
+        // edge geometry in the reference's node order (the order of NEDEL(1)'s local edge)
+        int va = (L + 1) % 3, vb = (L + 2) % 3;
+        if (role) { const int t = va; va = vb; vb = t; }
+        const double dy = Y[vb] - Y[va], dx = X[vb] - X[va];
+        const double xlen = sqrt(dy * dy + dx * dx);
+        const double nx = dy / xlen, ny = -dx / xlen;
+        if (kind == SIDE_INTERNAL) {
+            const int wdn = a.wd_in[nb];
+            if (wd == 0 && wdn == 0) continue;
+            double V[3][DOF];
+#pragma unroll
+            for (int f = 0; f < 3; f++)
+#pragma unroll
+                for (int k = 0; k < DOF; k++) V[f][k] = a.uin[(size_t)(f * DOF + k) * ne + nb];
+            const double d0 = E.dpv[nb], d1 = E.dpv[(size_t)ne + nb], d2 = E.dpv[2 * (size_t)ne + nb];
+            const double hbn[3] = {c13 * (d0 + d1 + d2), -(c16 * (d0 + d1)) + c13 * d2, -(0.5 * d0) + 0.5 * d1};
+            const int led_in = role ? nled : L, led_ex = role ? L : nled;
+            const int wd_in_el = role ? wdn : wd, wd_ex_el = role ? wd : wdn;
+            const double edfac = xlen / AREAS;
+            const double tx = -ny, ty = nx;
+            double acc[3][DOF];
+#pragma unroll
+            for (int f = 0; f < 3; f++)
+#pragma unroll
+                for (int k = 0; k < DOF; k++) acc[f][k] = 0.0;
+#pragma unroll
+            for (int I = 0; I < NEGP; I++) {
+                const int gi = I, gx = NEGP - 1 - I;
+                double ze_in = 0.0, qx_in = 0.0, qy_in = 0.0, ze_ex = 0.0, qx_ex = 0.0, qy_ex = 0.0;
+                double hb_in = 0.0, hb_ex = 0.0;
+#pragma unroll
+                for (int k = 0; k < DOF; k++) {
+                    const double pi = T.phi_edge[led_in][gi][k], px = T.phi_edge[led_ex][gx][k];
+                    const double zi = role ? V[0][k] : U[0][k], xi = role ? V[1][k] : U[1][k], yi = role ? V[2][k] : U[2][k];
+                    const double zx = role ? U[0][k] : V[0][k], xx = role ? U[1][k] : V[1][k], yx = role ? U[2][k] : V[2][k];
+                    ze_in = ze_in + zi * pi; qx_in = qx_in + xi * pi; qy_in = qy_in + yi * pi;
+                    ze_ex = ze_ex + zx * px; qx_ex = qx_ex + xx * px; qy_ex = qy_ex + yx * px;
+                }
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    hb_in = hb_in + (role ? hbn[k] : hb[k]) * T.phi_edge[led_in][gi][k];
+                    hb_ex = hb_ex + (role ? hb[k] : hbn[k]) * T.phi_edge[led_ex][gx][k];
+                }
+                double sf_in = 1.0, sf_ex = 1.0;
+                if (SPH) {
+                    const int el_in = role ? nb : e, el_ex = role ? e : nb;
+                    sf_in = E.sfaced[((size_t)led_in * NEGP + gi) * ne + el_in];
+                    sf_ex = E.sfaced[((size_t)led_ex * NEGP + gx) * ne + el_ex];
+                }
+                double f0, f1, f2;
+                llf_flux(ph, ze_in, ze_ex, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_ex, sf_in, sf_ex, nx, ny, ph.g, f0, f1, f2);
+                double fo = f0, gin = f1, hin = f2, gex = f1, hex = f2;
+                if (fabs(f0) > 1.0e-8) {
+                    if (wd_in_el == 0) {
+                        if (f0 > 0.0) {
+                            // flux out of the dry element: reflect the wet side's state
+                            const double qn_ext = qx_ex * nx + qy_ex * ny, qt_ext = qx_ex * tx + qy_ex * ty;
+                            const double qn_int = -qn_ext, qt_int = qt_ext;
+                            const double den = (nx * ty - ny * tx);
+                            const double mqx = (ty * qn_int - ny * qt_int) / den;
+                            const double mqy = (-tx * qn_int + nx * qt_int) / den;
+                            llf_flux(ph, ze_ex, ze_ex, mqx, qx_ex, mqy, qy_ex, hb_ex, hb_ex, sf_ex, sf_ex, nx, ny, ph.g, f0, f1, f2);
+                            fo = f0; gex = f1; hex = f2; gin = 0.0; hin = 0.0;
+                        } else if (!role) {
+                            llf_flux(ph, ze_in, ze_ex, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_ex, sf_in, sf_ex, nx, ny, 0.0, f0, f1, f2);
+                            gin = f1; hin = f2;
+                        }
+                    } else if (wd_ex_el == 0) {
+                        if (f0 < 0.0) {
+                            const double qn_int = qx_in * nx + qy_in * ny, qt_int = qx_in * tx + qy_in * ty;
+                            const double qn_ext = -qn_int, qt_ext = qt_int;
+                            const double den = (nx * ty - ny * tx);
+                            const double mqx = (ty * qn_ext - ny * qt_ext) / den;
+                            const double mqy = (-tx * qn_ext + nx * qt_ext) / den;
+                            llf_flux(ph, ze_in, ze_in, qx_in, mqx, qy_in, mqy, hb_in, hb_in, sf_in, sf_in, nx, ny, ph.g, f0, f1, f2);
+                            fo = f0; gin = f1; hin = f2; gex = 0.0; hex = 0.0;
+                        } else if (role) {
+                            llf_flux(ph, ze_in, ze_ex, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_ex, sf_in, sf_ex, nx, ny, 0.0, f0, f1, f2);
+                            gex = f1; hex = f2;
+                        }
+                    }
+                }
+                const int mygp = role ? gx : gi;
+#pragma unroll
+                for (int k = 0; k < DOF; k++) {
+                    const double w = edfac * T.edgeq[L][mygp][k];
+                    if (!role) {
+                        acc[0][k] = acc[0][k] - w * fo; acc[1][k] = acc[1][k] - w * gin; acc[2][k] = acc[2][k] - w * hin;
+                    } else {
+                        acc[0][k] = acc[0][k] + w * fo; acc[1][k] = acc[1][k] + w * gex; acc[2][k] = acc[2][k] + w * hex;
+                    }
+                }
+            }
+#pragma unroll
+            for (int f = 0; f < 3; f++)
+#pragma unroll
+                for (int k = 0; k < DOF; k++) S[f][k] = S[f][k] + acc[f][k];
+        } else {
+            if (wd == 0) continue;
+            const double tx = -ny, ty = nx;
+#pragma unroll
+            for (int I = 0; I < NEGP; I++) {
+                double ze_in = 0.0, qx_in = 0.0, qy_in = 0.0, hb_in = 0.0;
+#pragma unroll
+                for (int k = 0; k < DOF; k++) {
+                    const double pv = T.phi_edge[L][I][k];
+                    ze_in = ze_in + U[0][k] * pv; qx_in = qx_in + U[1][k] * pv; qy_in = qy_in + U[2][k] * pv;
+                }
+#pragma unroll
+                for (int k = 0; k < 3; k++) hb_in = hb_in + hb[k] * T.phi_edge[L][I][k];
+                const double sf = SPH ? E.sfaced[((size_t)L * NEGP + I) * ne + e] : 1.0;
+                double f0, f1, f2;
+                if (kind == SIDE_LAND) {
+                    const double qn_int = qx_in * nx + qy_in * ny, qt_int = qx_in * tx + qy_in * ty;
+                    const double qn_ext = -qn_int, qt_ext = qt_int;
+                    const double den = 1.0 / (nx * ty - ny * tx);
+                    const double qx_ex = (ty * qn_ext - ny * qt_ext) * den;
+                    const double qy_ex = (-tx * qn_ext + nx * qt_ext) * den;
+                    llf_flux(ph, ze_in, ze_in, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_in, sf, sf, nx, ny, ph.g, f0, f1, f2);
+                } else if (kind == SIDE_OCEAN) {
+                    double ze_ex = 0.0;
+                    const double xe = T.xegp[I];
+                    for (int jj = 0; jj < a.fo.nbfr; jj++) {
+                        const double argj = a.fo.argj[(size_t)a.stage * a.fo.nbfr + jj];
+                        const double rff = a.fo.rff[(size_t)a.stage * a.fo.nbfr + jj];
+                        const size_t o = ((size_t)nb * a.fo.nbfr + jj) * 2;
+                        const double z1 = a.fo.emo[o] * rff * cos(argj - a.fo.efa[o]);
+                        const double z2 = a.fo.emo[o + 1] * rff * cos(argj - a.fo.efa[o + 1]);
+                        ze_ex = ze_ex + (0.5 * (z1 + z2) + 0.5 * (z2 - z1) * xe);
+                    }
+                    if ((ze_ex * ph.ifnlfa + hb_in) <= 0) ze_ex = fabs(hb_in) + ph.h0;
+                    llf_flux(ph, ze_in, ze_ex, qx_in, qx_in, qy_in, qy_in, hb_in, hb_in, sf, sf, nx, ny, ph.g, f0, f1, f2);
+                } else {
+                    double qn_ext = 0.0, qx_ex = 0.0, qy_ex = 0.0;
+                    const double xe = T.xegp[I];
+                    for (int jj = 0; jj < a.fo.nffr; jj++) {
+                        const double argj = a.fo.fargj[(size_t)a.stage * a.fo.nffr + jj];
+                        const double rff = a.fo.frff[(size_t)a.stage * a.fo.nffr + jj];
+                        const size_t o = ((size_t)nb * a.fo.nffr + jj) * 2;
+                        const double q1 = a.fo.qnam[o], q2 = a.fo.qnam[o + 1];
+                        const double p1 = a.fo.qnph[o], p2 = a.fo.qnph[o + 1];
+                        const double qnam_gp = 0.5 * (q1 + q2) + 0.5 * (q2 - q1) * xe;
+                        const double qnph_gp = 0.5 * (p1 + p2) + 0.5 * (p2 - p1) * xe;
+                        qn_ext = qn_ext + qnam_gp * rff * cos(argj - qnph_gp);
+                        const double qt_ext = 0.0;
+                        qx_ex = -(ty * qn_ext - ny * qt_ext) / (nx * ty - ny * tx);
+                        qy_ex = -(-tx * qn_ext + nx * qt_ext) / (nx * ty - ny * tx);
+                    }
+                    llf_flux(ph, ze_in, ze_in, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_in, sf, sf, nx, ny, ph.g, f0, f1, f2);
+                }
+#pragma unroll
+                for (int k = 0; k < DOF; k++) {
+                    const double w = 2.0 * T.m_inv[k] / AREAS * xlen * T.phi_edge[L][I][k] * T.wegp[I];
+                    R[0][k] = R[0][k] - w * f0; R[1][k] = R[1][k] - w * f1; R[2][k] = R[2][k] - w * f2;
+                }
+            }
+        }
+    }
+    // add_internal_flux: RHS = RHS(boundary) + slot sum
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+#pragma unroll
+        for (int k = 0; k < DOF; k++) R[f][k] = R[f][k] + S[f][k];
+
+    // ---- volume quadrature (RHS_DG_HYDRO)
+    if (a.rain > 0.0) {
+#pragma unroll
+        for (int I = 0; I < NAGP; I++)
+#pragma unroll
+            for (int k = 0; k < DOF; k++) R[0][k] = R[0][k] + T.srfac[I][k] * a.rain;
+    }
+    if (wd != 0) {
+        double AREA = (X[0] - X[2]) * (Y[1] - Y[2]) + (X[2] - X[1]) * (Y[0] - Y[2]);
+        AREA = AREA * 0.5;
+        const double drdx = 1.0 / AREA * (Y[2] - Y[0]);
+        const double dsdx = 1.0 / AREA * (Y[0] - Y[1]);
+        const double drdy = 1.0 / AREA * (X[0] - X[2]);
+        const double dsdy = 1.0 / AREA * (X[1] - X[0]);
+        const double depth_c = hb[0] + U[0][0];
+        const double fh_nl_c = 1.0 / (ph.nleq * depth_c + ph.leq);
+        const double cori = E.cori_el ? E.cori_el[e] : ph.cori;
+        const double nbar = ph.load_manning ? E.nbar[e] : 0.0;
+#pragma unroll
+        for (int I = 0; I < NAGP; I++) {
+            double ze = U[0][0], qx = U[1][0], qy = U[2][0];
+#pragma unroll
+            for (int k = 1; k < DOF; k++) {
+                ze = ze + U[0][k] * T.phi_area[I][k];
+                qx = qx + U[1][k] * T.phi_area[I][k];
+                qy = qy + U[2][k] * T.phi_area[I][k];
+            }
+            double hbq = 0.0, dhbx = 0.0, dhby = 0.0;
+            double dpx[DOF], dpy[DOF];
+#pragma unroll
+            for (int k = 0; k < DOF; k++) {
+                dpx[k] = T.drphi[I][k] * drdx + T.dsphi[I][k] * dsdx;
+                dpy[k] = T.drphi[I][k] * drdy + T.dsphi[I][k] * dsdy;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                hbq = hbq + hb[k] * T.phi_area[I][k];
+                dhbx = dhbx + hb[k] * dpx[k];
+                dhby = dhby + hb[k] * dpy[k];
+            }
+            const double sfq = SPH ? E.sfacq[(size_t)I * ne + e] : 1.0;
+            const double depth = ze + hbq;
+            const double f1 = ph.nleq + ph.leq * hbq;
+            const double fx = qx * f1 * sfq, fy = qy * f1;
+            const double fu = ph.nleq * qx, fv = ph.nleq * qy;
+            const double fg = ph.nleqg * ze * wd;
+            const double fh = 1.0 / (ph.nleq * depth + ph.leq);
+            const double u = qx * fh, v = qy * fh;
+            const double huu = fu * u, hvv = fv * v, huv = fu * v;
+            const double gh2 = fg * (0.5 * ze + hbq) + ph.fg_l * ze;
+            const double gx = (huu + gh2) * sfq, gy = huv;
+            const double hx = huv * sfq, hy = hvv + gh2;
+            double fric = ph.fric;
+            if (ph.load_manning) {
+                fric = ph.g * (nbar * nbar) / pow(depth, 1.0 / 3.0);
+                if (fric < ph.cf) fric = ph.cf;
+            }
+            double tau = fric;
+            if (ph.iflinbf == 0) {
+                const double umag = sqrt(u * u + v * v);
+                tau = tau * umag * fh;
+                if (ph.ifhybf == 1) tau = tau * pow(1.0 + pow(ph.hbreak * fh_nl_c, ph.ftheta), ph.fgamma / ph.ftheta);
+                const double cap = 0.9 * ph.dtdph;
+                if (cap < tau) tau = cap;
+            }
+            const double sx = -(tau * qx) + fg * dhbx * sfq + cori * qy;
+            const double sy = -(tau * qy) + fg * dhby - cori * qx;
+            R[1][0] = R[1][0] + T.srfac[I][0] * sx;
+            R[2][0] = R[2][0] + T.srfac[I][0] * sy;
+#pragma unroll
+            for (int k = 1; k < DOF; k++) {
+                const double xf = T.mw[I][k] * dpx[k], yf = T.mw[I][k] * dpy[k], sr = T.srfac[I][k];
+                R[0][k] = R[0][k] + xf * fx + yf * fy;
+                R[1][k] = R[1][k] + xf * gx + yf * gy + sr * sx;
+                R[2][k] = R[2][k] + xf * hx + yf * hy + sr * sy;
+            }
+        }
+    }
+    // ---- SSP-RK update U(IRK+1) = sum_I ATVD(IRK,I) U(I) + BTVD(IRK,I) DT RHS(I)
+    if (a.rout) {
+#pragma unroll
+        for (int f = 0; f < 3; f++)
+#pragma unroll
+            for (int k = 0; k < DOF; k++) a.rout[(size_t)(f * DOF + k) * ne + e] = R[f][k];
+    }
+    double W[3][DOF];
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+#pragma unroll
+        for (int k = 0; k < DOF; k++) W[f][k] = 0.0;
+    for (int i = 0; i < a.nprev; i++) {
+        const double ai = a.aprev[i], bi = a.bprev[i];
+        if (ai != 0.0) {
+            const double* up = a.uprev[i];
+#pragma unroll
+            for (int f = 0; f < 3; f++)
+#pragma unroll
+                for (int k = 0; k < DOF; k++) W[f][k] = W[f][k] + ai * up[(size_t)(f * DOF + k) * ne + e];
+        }
+        if (bi != 0.0) {
+            const double* rp = a.rprev[i];
+#pragma unroll
+            for (int f = 0; f < 3; f++)
+#pragma unroll
+                for (int k = 0; k < DOF; k++) W[f][k] = W[f][k] + bi * rp[(size_t)(f * DOF + k) * ne + e];
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+#pragma unroll
+        for (int k = 0; k < DOF; k++) W[f][k] = W[f][k] + a.acur * U[f][k] + a.bcur * R[f][k];
+
+    if (WDFUSED) {
+        const double dpe_min = fmin(fmin(D[0], D[1]), D[2]);
+        const int nw = wetdry_element<P>(T, W, D, hb, dpe_min, wd, ph.h0);
+        a.wd_out[e] = (uint8_t)nw;
+    }
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+#pragma unroll
+        for (int k = 0; k < DOF; k++) a.uout[(size_t)(f * DOF + k) * ne + e] = W[f][k];
+}
+
+// ------------------------------------------------------------------------------------------
+// SLOPELIMITER5 part 1 (src/slopelimiter.F:3290-3404): node min/max of the updated cell means.
+// nminmax layout [6][np]: zmin zmax qxmin qxmax qymin qymax.
+// ------------------------------------------------------------------------------------------
+template <int P>
+__global__ void __launch_bounds__(256) node_minmax_kernel(int np, int ne, const int* __restrict__ off,
+                                                          const int* __restrict__ n2e,
+                                                          const double* __restrict__ u,
+                                                          double* __restrict__ mm) {
+    constexpr int DOF = (P + 1) * (P + 2) / 2;
+    const int n = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (n >= np) return;
+    double zmin = 99999., zmax = -99999., xmin = 99999., xmax = -99999., ymin = 99999., ymax = -99999.;
+    for (int j = off[n]; j < off[n + 1]; j++) {
+        const int e = n2e[j] >> 2;
+        const double z = u[e], x = u[(size_t)DOF * ne + e], y = u[(size_t)2 * DOF * ne + e];
+        zmin = fmin(zmin, z); xmin = fmin(xmin, x); ymin = fmin(ymin, y);
+        zmax = fmax(zmax, z); xmax = fmax(xmax, x); ymax = fmax(ymax, y);
+    }
+    mm[n] = zmin; mm[(size_t)np + n] = zmax; mm[2 * (size_t)np + n] = xmin;
+    mm[3 * (size_t)np + n] = xmax; mm[4 * (size_t)np + n] = ymin; mm[5 * (size_t)np + n] = ymax;
+}
+
+// ------------------------------------------------------------------------------------------
+// SLOPELIMITER5 part 2 (src/slopelimiter.F:3434-3676) then WETDRY (src/wetdry.F), per element.
+// ------------------------------------------------------------------------------------------
+template <int P, bool LIMIT, bool WETDRY>
+__global__ void __launch_bounds__(256) limit_wetdry_kernel(int np, ElemData E, double* __restrict__ u,
+                                                           const double* __restrict__ mm,
+                                                           const uint8_t* __restrict__ wd_in,
+                                                           uint8_t* __restrict__ wd_out, double h0) {
+    constexpr int DOF = (P + 1) * (P + 2) / 2;
+    const DevTables& T = tab<P>();
+    const int ne = E.ne;
+    const int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    double U[3][DOF];
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+#pragma unroll
+        for (int k = 0; k < DOF; k++) U[f][k] = u[(size_t)(f * DOF + k) * ne + e];
+    if (LIMIT) {
+        const int n0 = E.nm[e], n1 = E.nm[(size_t)ne + e], n2 = E.nm[2 * (size_t)ne + e];
+#pragma unroll
+        for (int iv = 0; iv < 3; iv++) {
+            const double* mn = mm + (size_t)(2 * iv) * np;
+            const double* mx = mm + (size_t)(2 * iv + 1) * np;
+            const double vmin[3] = {mn[n0], mn[n1], mn[n2]}, vmax[3] = {mx[n0], mx[n1], mx[n2]};
+            const double c0 = U[iv][0];
+            double v[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                double t = c0;
+#pragma unroll
+                for (int kk = 1; kk < 3; kk++) t = t + T.phi_corner[c][kk] * U[iv][kk];
+                v[c] = fmax(fmin(t, vmax[c]), vmin[c]);
+            }
+#pragma unroll
+            for (int ll = 0; ll < 3; ll++) {
+                const double sumloc = (v[0] + v[1] + v[2]) / 3.0;
+                double sumdif = (sumloc - c0) * 3.0;
+                const double signdif = copysign(1.0, sumdif);
+                double dif[3];
+#pragma unroll
+                for (int c = 0; c < 3; c++) dif[c] = (v[c] - c0) * signdif;
+                int kdp = (dif[0] > 0) + (dif[1] > 0) + (dif[2] > 0);
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const double div = fmax(1.0, (double)kdp);
+                    double redfac;
+                    if (dif[k] > 0) { redfac = sumdif * signdif / div; kdp = kdp - 1; }
+                    else redfac = 0;
+                    const double redmax = (signdif > 0) ? v[k] - vmin[k] : vmax[k] - v[k];
+                    redfac = fmin(redfac, redmax);
+                    sumdif = sumdif - redfac * signdif;
+                    v[k] = v[k] - redfac * signdif;
+                }
+            }
+            U[iv][1] = -(1.0 / 6.0 * (v[0] + v[1])) + 1.0 / 3.0 * v[2];
+            U[iv][2] = -(0.5 * v[0]) + 0.5 * v[1];
+        }
+    }
+    if (WETDRY) {
+        double D[3];
+#pragma unroll
+        for (int v = 0; v < 3; v++) D[v] = E.dpv[(size_t)v * ne + e];
+        const double c13 = 1.0 / 3.0, c16 = 1.0 / 6.0;
+        const double hb[3] = {c13 * (D[0] + D[1] + D[2]), -(c16 * (D[0] + D[1])) + c13 * D[2], -(0.5 * D[0]) + 0.5 * D[1]};
+        const double dpe_min = fmin(fmin(D[0], D[1]), D[2]);
+        wd_out[e] = (uint8_t)wetdry_element<P>(T, U, D, hb, dpe_min, wd_in[e], h0);
+    }
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+#pragma unroll
+        for (int k = (WETDRY ? 0 : 1); k < DOF; k++) u[(size_t)(f * DOF + k) * ne + e] = U[f][k];
+}
+
+// ------------------------------------------------------------------------------------------
+// modal2nodal (src/write_results.F:14-174) + NODECODE from the wet flags.
+// ------------------------------------------------------------------------------------------
+template <int P>
+__global__ void __launch_bounds__(256) modal2nodal_kernel(int np, ElemData E, const int* __restrict__ off,
+                                                          const int* __restrict__ n2e,
+                                                          const double* __restrict__ u,
+                                                          const uint8_t* __restrict__ wd, double nleq,
+                                                          double leq, double* __restrict__ out,
+                                                          int* __restrict__ nodecode) {
+    constexpr int DOF = (P + 1) * (P + 2) / 2;
+    const DevTables& T = tab<P>();
+    const int ne = E.ne;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= np) return;
+    double asum = 0;
+    int nc = 0;
+    for (int j = off[n]; j < off[n + 1]; j++) {
+        const int e = n2e[j] >> 2;
+        if (wd[e] == 0) continue;
+        nc = 1;
+        const double X0 = E.xv[e], X1 = E.xv[(size_t)ne + e], X2 = E.xv[2 * (size_t)ne + e];
+        const double Y0 = E.yv[e], Y1 = E.yv[(size_t)ne + e], Y2 = E.yv[2 * (size_t)ne + e];
+        const double areas = (X0 - X2) * (Y1 - Y2) + (X2 - X1) * (Y0 - Y2);
+        asum = asum + 0.5 * areas;
+    }
+    double e2 = 0.0, u2 = 0.0, v2 = 0.0;
+    for (int j = off[n]; j < off[n + 1]; j++) {
+        const int e = n2e[j] >> 2, c = n2e[j] & 3;
+        if (wd[e] == 0) continue;
+        const double X0 = E.xv[e], X1 = E.xv[(size_t)ne + e], X2 = E.xv[2 * (size_t)ne + e];
+        const double Y0 = E.yv[e], Y1 = E.yv[(size_t)ne + e], Y2 = E.yv[2 * (size_t)ne + e];
+        const double areas = (X0 - X2) * (Y1 - Y2) + (X2 - X1) * (Y0 - Y2);
+        const double D0 = E.dpv[e], D1 = E.dpv[(size_t)ne + e], D2 = E.dpv[2 * (size_t)ne + e];
+        const double hb[3] = {1.0 / 3.0 * (D0 + D1 + D2), -(1.0 / 6.0 * (D0 + D1)) + 1.0 / 3.0 * D2, -(0.5 * D0) + 0.5 * D1};
+        double zd = u[e], xd = u[(size_t)DOF * ne + e], yd = u[(size_t)2 * DOF * ne + e], hd = hb[0];
+#pragma unroll
+        for (int kk = 1; kk < DOF; kk++) {
+            const double pc = T.phi_corner[c][kk];
+            zd = zd + pc * u[(size_t)kk * ne + e];
+            xd = xd + pc * u[(size_t)(DOF + kk) * ne + e];
+            yd = yd + pc * u[(size_t)(2 * DOF + kk) * ne + e];
+            hd = hd + pc * (kk < 3 ? hb[kk] : 0.0);
+        }
+        const double area = 0.50 * areas / asum;
+        const double depth = zd + hd;
+        const double fh = 1.0 / (nleq * depth + leq);
+        e2 = e2 + area * zd;
+        u2 = u2 + area * xd * fh;
+        v2 = v2 + area * yd * fh;
+    }
+    out[n] = e2; out[(size_t)np + n] = u2; out[2 * (size_t)np + n] = v2;
+    nodecode[n] = nc;
+}
+
+// Water volume per element: 0.5*AREAS*(ze_1 + hb_1), reduced per block.
+template <int P>
+__global__ void __launch_bounds__(256) mass_kernel(ElemData E, const double* __restrict__ u,
+                                                   double* __restrict__ partial) {
+    __shared__ double sh[256];
+    const int ne = E.ne;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (e < ne) {
+        const double X0 = E.xv[e], X1 = E.xv[(size_t)ne + e], X2 = E.xv[2 * (size_t)ne + e];
+        const double Y0 = E.yv[e], Y1 = E.yv[(size_t)ne + e], Y2 = E.yv[2 * (size_t)ne + e];
+        const double D0 = E.dpv[e], D1 = E.dpv[(size_t)ne + e], D2 = E.dpv[2 * (size_t)ne + e];
+        const double areas = (X0 - X2) * (Y1 - Y2) + (X2 - X1) * (Y0 - Y2);
+        m = 0.5 * areas * (u[e] + 1.0 / 3.0 * (D0 + D1 + D2));
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+}  // namespace dg

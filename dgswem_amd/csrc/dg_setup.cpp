@@ -1,0 +1,295 @@
+// Host-side set-up: reference-element tables, SSP-RK coefficients, edge tables.
+//
+// The numbers produced here must equal the reference's to the last bit for p=1, because the
+// step's arithmetic is compared with dgswem_serial bit for bit.  Each formula therefore keeps the
+// reference's operation order:
+//   Dubiner basis      src/orthogonal_basis_v1.F:114-239 (modes ordered by total degree, the
+//                      p=2 modes expanded from the same three-term recurrence)
+//   area rules         src/quadrature_v1.F:245-275 (degree 2 and 4 symmetric triangle rules)
+//   Lobatto rule       src/quad_rules_general.F:185-249 (GLL nodes; weights 2/(n(n-1)P_{n-1}^2))
+//   mass inverse       src/prep_DG.F:712-721
+//   RK_TIME            src/prep_DG.F:1551-2078
+//   CREATE_EDGE_DATA   src/create_edge_data.F:31-635
+#include "dg_setup.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <unordered_map>
+
+namespace dg {
+
+namespace {
+
+// Basis value and reference-coordinate derivatives of the p<=2 Dubiner modes at (x, y).
+// Mode order: 1, phi(0,1), phi(1,0), phi(0,2), phi(1,1), phi(2,0).
+void dubiner(int p, double x, double y, double* v, double* dr, double* ds) {
+    // degree 0 and 1 (PHI(0,0), PHI(1,0), PHI(0,1) of the recurrence)
+    const double p10 = (1.0 + 2.0 * x + y) / 2.0;
+    const double d10r = 1.0, d10s = 1.0 / 2.0;
+    const double f01 = (1.0 + 2.0 * 0 + (3.0 + 2.0 * 0) * y) / 2.0;   // (1+3y)/2
+    const double p01 = 1.0 * f01;
+    const double d01r = 0.0 * f01;
+    const double d01s = 0.0 * f01 + 1.0 * (3.0 + 2.0 * 0) / 2.0;
+    double val[6] = {1.0, p01, p10, 0, 0, 0};
+    double vr[6] = {0.0, d01r, d10r, 0, 0, 0};
+    double vs[6] = {0.0, d01s, d10s, 0, 0, 0};
+    if (p >= 2) {
+        // PHI(2,0): (3/2) PHI10 PHI10 - (1/2) ((1-y)/2)^2
+        const double a = (2.0 * 1 + 1.0) / (1 + 1.0), b = 1 / (1 + 1.0);
+        const double om = (1.0 - y) / 2.0, om2 = om * om;
+        const double p20 = a * p10 * p10 - b * om2 * 1.0;
+        const double d20r = a * (d10r * p10 + p10 * d10r) - b * om2 * 0.0;
+        const double d20s = a * (d10s * p10 + p10 * d10s) - b * ((y - 1) / 2 * 1.0 + om2 * 0.0);
+        // PHI(1,1) = PHI(1,0) (1 + 2 + 5y)/2
+        const double g = (1.0 + 2.0 * 1 + (3.0 + 2.0 * 1) * y) / 2.0;
+        const double p11 = p10 * g;
+        const double d11r = d10r * g;
+        const double d11s = d10s * (1.0 + 2.0 * 1 + (3.0 + 2.0 * 1) * y) / 2.0 + p10 * (3.0 + 2.0 * 1) / 2.0;
+        // PHI(0,2) = (AQ y + BQ) PHI(0,1) - CQ PHI(0,0), q=1, p=0
+        const double AQ = (2.0 * 1 + 2.0 + 0.0) * (2.0 * 1 + 3.0 + 0.0) / (2.0 * 1 + 2.0) / (1 + 2.0 + 0.0);
+        const double t = (2.0 * 0 + 1.0);
+        const double BQ = t * t * (2.0 * 1 + 2.0 + 0.0) / (2.0 * 1 + 2.0) / (2.0 * 1 + 0.0 + 1.0) / (1 + 2.0 + 0.0);
+        const double CQ = (1 + 0.0 + 1.0) * 1 * (2.0 * 1 + 3.0 + 0.0) / (1 + 1.0) / (1 + 2.0 + 0.0) / (2.0 * 1 + 0.0 + 1.0);
+        const double p02 = (AQ * y + BQ) * p01 - CQ * 1.0;
+        const double d02r = (AQ * y + BQ) * d01r - CQ * 0.0;
+        const double d02s = (AQ * y + BQ) * d01s + AQ * p01 - CQ * 0.0;
+        val[3] = p02; val[4] = p11; val[5] = p20;
+        vr[3] = d02r; vr[4] = d11r; vr[5] = d20r;
+        vs[3] = d02s; vs[4] = d11s; vs[5] = d20s;
+    }
+    const int dof = (p + 1) * (p + 2) / 2;
+    for (int k = 0; k < dof; k++) {
+        // the reference zeroes |value| < 1e-15 (src/orthogonal_basis_v1.F:229-231)
+        v[k] = std::fabs(val[k]) < 1.0e-15 ? 0.0 : val[k];
+        dr[k] = std::fabs(vr[k]) < 1.0e-15 ? 0.0 : vr[k];
+        ds[k] = std::fabs(vs[k]) < 1.0e-15 ? 0.0 : vs[k];
+    }
+}
+
+// Legendre P_n(x) by the three-term recurrence.
+double legendre(int n, double x) {
+    double pm = 1.0, pc = x;
+    if (n == 0) return pm;
+    for (int k = 1; k < n; k++) {
+        const double kk = k;
+        const double pn = ((2.0 * kk + 1.0) * x * pc - kk * pm) / (kk + 1.0);
+        pm = pc; pc = pn;
+    }
+    return pc;
+}
+
+}  // namespace
+
+bool build_ref_tables(int p, RefTables& t, std::string& err) {
+    if (p < 1 || p > 2) { err = "polynomial order must be 1 or 2"; return false; }
+    t = RefTables{};
+    t.p = p;
+    t.dof = (p + 1) * (p + 2) / 2;
+    t.negp = p + 2;                      // NEGP = ceil((2p+3)/2)
+    // --- area rule of degree 2p: orbits of multiplicity 3 at barycentric (a, a, 1-2a)
+    double A[2], W[2];
+    int norb;
+    if (p == 1) { norb = 1; A[0] = 0.1666666666666666666666666666666666; W[0] = 0.3333333333333333333333333333333333; }
+    else {
+        norb = 2;
+        A[0] = 0.4459484909159648863183292538830519; A[1] = 0.0915762135097707434595714634022015;
+        W[0] = 0.2233815896780114656950070084331228; W[1] = 0.1099517436553218676383263249002105;
+    }
+    t.nagp = 3 * norb;
+    const double v1x = -1.0, v1y = -1.0, v2x = 1.0, v2y = -1.0, v3x = -1.0, v3y = 1.0, area = 2.0;
+    for (int o = 0; o < norb; o++) {
+        const double a = A[o], b = a, c = 1.0 - (a + b);
+        const double bc[3][3] = {{a, b, c}, {b, c, a}, {c, a, b}};
+        for (int r = 0; r < 3; r++) {
+            const int q = 3 * o + r;
+            t.xagp[q] = v1x * bc[r][0] + v2x * bc[r][1] + v3x * bc[r][2];
+            t.yagp[q] = v1y * bc[r][0] + v2y * bc[r][1] + v3y * bc[r][2];
+            t.wagp[q] = area * W[o];
+        }
+    }
+    // --- Gauss-Lobatto-Legendre rule on [-1,1]
+    const int n = t.negp;
+    if (n == 3) { t.xegp[0] = -1.0; t.xegp[1] = 0.0; t.xegp[2] = 1.0; }
+    else { const double r = 1.0 / std::sqrt(5.0); t.xegp[0] = -1.0; t.xegp[1] = -r; t.xegp[2] = r; t.xegp[3] = 1.0; }
+    {
+        const double m = n - 2;
+        for (int q = 0; q < n; q++) {
+            if (q == 0 || q == n - 1) t.wegp[q] = 2.0 / ((m + 1.0) * (m + 2.0));
+            else {
+                const double pl = legendre(n - 1, t.xegp[q]);
+                t.wegp[q] = 2.0 / ((m + 1.0) * (m + 2.0) * pl * pl);
+            }
+        }
+    }
+    // --- basis tables
+    double v[MAXDOF], dr[MAXDOF], ds[MAXDOF];
+    for (int q = 0; q < t.nagp; q++) {
+        dubiner(p, t.xagp[q], t.yagp[q], v, dr, ds);
+        for (int k = 0; k < t.dof; k++) { t.phi_area[q][k] = v[k]; t.drphi[q][k] = dr[k]; t.dsphi[q][k] = ds[k]; }
+    }
+    const double cx[3] = {-1.0, 1.0, -1.0}, cy[3] = {-1.0, -1.0, 1.0};
+    for (int c = 0; c < 3; c++) {
+        dubiner(p, cx[c], cy[c], v, dr, ds);
+        for (int k = 0; k < t.dof; k++) t.phi_corner[c][k] = v[k];
+    }
+    // local edge l runs from vertex l+1 to vertex l+2 (mod 3): points parametrised by XEGP
+    for (int l = 0; l < 3; l++)
+        for (int q = 0; q < n; q++) {
+            const double s = t.xegp[q];
+            const double px = (l == 0) ? -s : (l == 1 ? -1.0 : s);
+            const double py = (l == 0) ? s : (l == 1 ? -s : -1.0);
+            dubiner(p, px, py, v, dr, ds);
+            for (int k = 0; k < t.dof; k++) t.phi_edge[l][q][k] = v[k];
+        }
+    {
+        int m = 0;
+        for (int j = 0; j <= p; j++)
+            for (int i = 0; i <= j; i++) {
+                const int jj = j - i;
+                t.m_inv[m++] = ((2.0 * i + 1.0) * (2.0 * jj + 2.0 * i + 2.0) / 4.0);
+            }
+    }
+    for (int l = 0; l < 3; l++)
+        for (int q = 0; q < n; q++)
+            for (int k = 0; k < t.dof; k++)
+                t.edgeq[l][q][k] = 2.0 * t.m_inv[k] * t.phi_edge[l][q][k] * t.wegp[q];
+    for (int q = 0; q < t.nagp; q++)
+        for (int k = 0; k < t.dof; k++) {
+            t.mw[q][k] = t.m_inv[k] * t.wagp[q];
+            t.srfac[q][k] = t.m_inv[k] * t.wagp[q] * t.phi_area[q][k];
+        }
+    return true;
+}
+
+bool build_rk(int s, int r, RKScheme& k, std::string& err) {
+    k = RKScheme{};
+    if (s < 1 || s > MAXRK) { err = "unsupported number of RK stages"; return false; }
+    k.nrk = s;
+    auto A = [&](int i, int j, double v) { k.a[i - 1][j - 1] = v; };
+    auto B = [&](int i, int j, double v) { k.b[i - 1][j - 1] = v; };
+    if (s == 1 && r == 1) { A(1, 1, 1.0); B(1, 1, 1.0); }
+    else if (r == 2 && s == 3) {
+        A(1,1,1.0); A(2,1,0.087353119859156); A(3,1,0.344956917166841); A(2,2,0.912646880140844);
+        A(3,3,0.655043082833159); B(1,1,0.528005024856522); B(3,1,0.022826837460491);
+        B(2,2,0.481882138633993); B(3,3,0.345866039233415);
+    } else if (r == 2 && s == 4) {
+        A(1,1,1.0); A(2,1,0.394806441339829); A(3,1,0.002797307087390); A(4,1,0.252860909354373);
+        A(2,2,0.605193558660171); A(3,3,0.997202692912610); A(4,4,0.747139090645627);
+        B(1,1,0.406584463657504); B(3,1,0.013637216641451); B(4,1,0.016453567333598);
+        B(2,2,0.246062298456822); B(3,3,0.405447122055692); B(4,4,0.303775146447707);
+    } else if (r == 2 && s == 5) {
+        A(1,1,1.0); A(2,1,0.235593265061659); A(3,1,0.174017972351526); A(4,1,0.235264368870758);
+        A(5,1,0.141720372339803); A(2,2,0.764406734938341); A(4,2,0.000058643383967);
+        A(5,2,0.095374613155521); A(3,3,0.825982027648475); A(5,3,0.000311763705780);
+        A(4,4,0.764676987745275); A(5,5,0.762593250798895);
+        B(1,1,0.324840618151514); B(3,1,0.108822380501601); B(4,1,0.054392262422093);
+        B(5,1,0.000000180291569); B(2,2,0.248310356296551); B(4,2,0.000019049753098);
+        B(5,2,0.030981548293401); B(3,3,0.268312512443371); B(5,3,0.000101273514903);
+        B(4,4,0.248398145385413); B(5,5,0.247721262987686);
+    } else if (r == 2) {
+        // optimal SSP(s,2): Euler steps of dt/(s-1), last stage averaged with the start
+        for (int i = 1; i <= s; i++) {
+            if (i < s) { A(i, i, 1.0); B(i, i, 1.0 / (s - 1)); }
+            else { A(s, 1, 1.0 / s); A(s, s, (s - 1.0) / s); B(s, s, 1.0 / s); }
+        }
+    } else if (s == 3 && r == 3) {
+        A(1,1,1.0); A(2,1,3.0/4.0); A(2,2,1.0/4.0); A(3,1,1.0/3.0); A(3,3,2.0/3.0);
+        B(1,1,1.0); B(2,2,1.0/4.0); B(3,3,2.0/3.0);
+    } else { err = "unsupported RK scheme (rk_stage, rk_order)"; return false; }
+    // stage times: c_k = sum_{l<k} C(k-1,l) with C(i,j) = B(i,j) + sum_l C(l,j) A(i,l+1)
+    double C[MAXRK][MAXRK]{};
+    for (int j = 0; j < s; j++)
+        for (int i = 0; i < s; i++) {
+            double acc = 0.0;
+            for (int l = j + 1; l <= i; l++) acc = acc + C[l - 1][j] * k.a[i][l];   // L = K+1..I-1 (1-based I = i+1)
+            C[i][j] = k.b[i][j] + acc;
+        }
+    k.dtvd[0] = 0.0;
+    for (int st = 1; st < s; st++) {
+        double acc = 0.0;
+        for (int l = 0; l < st; l++) acc = acc + C[st - 1][l];
+        k.dtvd[st] = acc;
+    }
+    return true;
+}
+
+bool create_edge_data(int np, int ne, const int* nm, int nope, const int* nvdll, const int* nbdv,
+                      int nbou, const int* ibtype, const int* nvell, const int* nbvv, EdgeData& ed,
+                      std::string& err) {
+    auto node = [&](int e, int k) { return nm[(size_t)k * ne + e] - 1; };
+    ed = EdgeData{};
+    ed.neled.assign((size_t)ne * 3, -1);
+    ed.nedno.reserve((size_t)ne * 3); ed.nedel.reserve((size_t)ne * 3);
+    std::unordered_map<uint64_t, int> key2edge;
+    key2edge.reserve((size_t)ne * 2);
+    auto key = [](int a, int b) { if (a > b) std::swap(a, b); return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b; };
+    // edges numbered in order of first appearance scanning elements then local edges 1..3;
+    // the first element is EL_IN and fixes the node order (its counter-clockwise order)
+    for (int e = 0; e < ne; e++) {
+        const int n[3] = {node(e, 0), node(e, 1), node(e, 2)};
+        for (int l = 0; l < 3; l++) {
+            const int a = n[(l + 1) % 3], b = n[(l + 2) % 3];
+            if (a < 0 || a >= np || b < 0 || b >= np) { err = "node index out of range"; return false; }
+            auto it = key2edge.find(key(a, b));
+            if (it == key2edge.end()) {
+                const int id = ed.nedges++;
+                key2edge.emplace(key(a, b), id);
+                ed.nedno.push_back(a); ed.nedno.push_back(b);
+                ed.nedel.push_back(e); ed.nedel.push_back(-1);
+                ed.neled[(size_t)e * 3 + l] = id;
+            } else {
+                const int id = it->second;
+                if (ed.nedel[2 * (size_t)id + 1] >= 0) {
+                    err = "possible duplicate element " + std::to_string(e + 1);
+                    return false;
+                }
+                ed.nedel[2 * (size_t)id + 1] = e;
+                ed.neled[(size_t)e * 3 + l] = id;
+            }
+        }
+    }
+    const int ned = ed.nedges;
+    std::vector<int> cls(ned, -1);
+    for (int i = 0; i < ned; i++)
+        if (ed.nedel[2 * (size_t)i] >= 0 && ed.nedel[2 * (size_t)i + 1] >= 0) { ed.niedn.push_back(i); cls[i] = 0; }
+    auto find = [&](int a, int b) -> int {
+        auto it = key2edge.find(key(a, b));
+        return it == key2edge.end() ? -1 : it->second;
+    };
+    size_t off = 0;
+    for (int k = 0; k < nope; k++) {
+        for (int i = 0; i + 1 < nvdll[k]; i++) {
+            const int e = find(nbdv[off + i] - 1, nbdv[off + i + 1] - 1);
+            if (e < 0) { err = "error in processing open ocean boundary conditions"; return false; }
+            ed.needn.push_back(e); cls[e] = 4;
+        }
+        off += nvdll[k];
+    }
+    off = 0;
+    for (int k = 0; k < nbou; k++) {
+        const int t = ibtype[k];
+        const bool land = (t == 0 || t == 10 || t == 20 || t == 1 || t == 11 || t == 21);
+        const bool flow = (t == 2 || t == 12 || t == 22);
+        if (!land && !flow) { err = "boundary type " + std::to_string(t) + " is not supported"; return false; }
+        for (int i = 0; i + 1 < nvell[k]; i++) {
+            const int e = find(nbvv[off + i] - 1, nbvv[off + i + 1] - 1);
+            if (e < 0) continue;                 // the reference reports and skips such pairs
+            cls[e] = 1;
+            (land ? ed.nledn : ed.nfedn).push_back(e);
+        }
+        off += nvell[k];
+    }
+    for (int i = 0; i < ned; i++)
+        if (cls[i] < 0) ed.nledn.push_back(i);   // unclassified boundary edges are land
+    ed.nedsd.assign((size_t)ned * 2, -1);
+    for (int i = 0; i < ned; i++)
+        for (int s = 0; s < 2; s++) {
+            const int e = ed.nedel[2 * (size_t)i + s];
+            if (e < 0) continue;
+            for (int l = 0; l < 3; l++)
+                if (ed.neled[(size_t)e * 3 + l] == i) ed.nedsd[2 * (size_t)i + s] = l;
+        }
+    return true;
+}
+
+}  // namespace dg

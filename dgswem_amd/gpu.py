@@ -1,0 +1,248 @@
+"""Host-side driver of libdgswem_hip.so through its C ABI (include/dgswem_gpu.h).
+
+`GpuSolver(case)` does for the GPU library what the reference's Fortran host does around
+DG_TIMESTEP: it hands over the mesh, the edge tables of CREATE_EDGE_DATA, the boundary forcing
+assembled the way PREP_DG assembles EMO_DG/EFA_DG and QNAM_DG/QNPH_DG, and then advances the state
+with dg_gpu_step.  There is no CPU fallback: without the built library every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .fortio import Case
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdgswem_hip.so")
+_LIB = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+# kernel ids of dg_gpu_kernel_stats
+K_STAGE, K_MINMAX, K_LIMIT, K_M2N = 0, 1, 2, 3
+
+# every symbol include/dgswem_gpu.h declares
+EXPORTED = ["dg_create_edge_data", "dg_gpu_init", "dg_gpu_set_tide", "dg_gpu_set_flow",
+            "dg_gpu_set_state", "dg_gpu_get_state", "dg_gpu_step", "dg_gpu_synchronize",
+            "dg_gpu_modal2nodal", "dg_gpu_mass", "dg_gpu_profile", "dg_gpu_kernel_stats",
+            "dg_gpu_last_error", "dg_gpu_finalize"]
+
+
+class DgParams(C.Structure):
+    _fields_ = [("p", C.c_int), ("rk_stage", C.c_int), ("rk_order", C.c_int), ("ics", C.c_int),
+                ("nolibf", C.c_int), ("nolifa", C.c_int), ("nolica", C.c_int),
+                ("nolicat", C.c_int), ("nramp", C.c_int), ("slopeflag", C.c_int),
+                ("rainfall", C.c_int), ("load_manning", C.c_int), ("g", C.c_double),
+                ("dtdp", C.c_double), ("statim", C.c_double), ("dramp", C.c_double),
+                ("h0", C.c_double), ("cf", C.c_double), ("hbreak", C.c_double),
+                ("ftheta", C.c_double), ("fgamma", C.c_double), ("sfea0", C.c_double)]
+
+
+class GpuError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libdgswem_hip.so and declare its signatures (no device call happens here)."""
+    global _LIB
+    if _LIB is not None and path == LIB_PATH:
+        return _LIB
+    if not os.path.exists(path):
+        raise GpuError(f"{path} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = C.CDLL(path)
+    L.dg_gpu_last_error.restype = C.c_char_p
+    L.dg_create_edge_data.argtypes = [C.c_int, C.c_int, _ip, C.c_int, _ip, _ip, C.c_int, _ip, _ip,
+                                      _ip, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _ip]
+    L.dg_gpu_init.argtypes = [C.POINTER(DgParams), C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp,
+                              C.c_void_p, _ip, C.c_int, _ip, _ip, _ip, C.c_int, _ip, C.c_int, _ip,
+                              C.c_int, _ip, C.c_int, _ip, C.POINTER(C.c_int)]
+    L.dg_gpu_set_tide.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.dg_gpu_set_flow.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.dg_gpu_set_state.argtypes = [C.c_int, _dp, _dp, _dp, C.c_void_p]
+    L.dg_gpu_get_state.argtypes = [C.c_int, _dp, _dp, _dp, _ip]
+    L.dg_gpu_step.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.dg_gpu_synchronize.argtypes = [C.c_int]
+    L.dg_gpu_modal2nodal.argtypes = [C.c_int, _dp, _dp, _dp, _ip]
+    L.dg_gpu_mass.argtypes = [C.c_int, C.POINTER(C.c_double)]
+    L.dg_gpu_profile.argtypes = [C.c_int, C.c_int]
+    L.dg_gpu_kernel_stats.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_longlong), C.c_char_p, C.c_int]
+    L.dg_gpu_finalize.argtypes = [C.c_int]
+    for name in EXPORTED:
+        getattr(L, name).restype = C.c_char_p if name == "dg_gpu_last_error" else C.c_int
+    if path == LIB_PATH:
+        _LIB = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise GpuError(load_library().dg_gpu_last_error().decode())
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def params_of(case: Case) -> DgParams:
+    p = case.params
+    return DgParams(p=case.dg.ph, rk_stage=case.dg.rk_stage, rk_order=case.dg.rk_order, ics=p.ics,
+                    nolibf=p.nolibf, nolifa=p.nolifa, nolica=p.nolica, nolicat=p.nolicat,
+                    nramp=p.nramp, slopeflag=case.dg.slopeflag, rainfall=case.dg.rainfall,
+                    load_manning=1 if case.manning is not None else 0, g=p.g, dtdp=p.dtdp,
+                    statim=p.statim, dramp=p.dramp, h0=p.h0, cf=p.cf, hbreak=p.hbreak,
+                    ftheta=p.ftheta, fgamma=p.fgamma, sfea0=p.sfea0)
+
+
+def create_edge_data(case: Case) -> dict:
+    """CREATE_EDGE_DATA through the library (1-based arrays, like the reference)."""
+    L = load_library()
+    ne, n_nodes = case.ne, case.np_
+    nm_cm = _i(case.nm.T.reshape(-1))            # NM(NE,3) column-major
+    ov = _i([len(s) for s in case.open_segs] or [0])
+    on = _i(np.concatenate(case.open_segs) if case.open_segs else [0])
+    lt = _i([t for t, _ in case.land_segs] or [0])
+    lv = _i([len(s) for _, s in case.land_segs] or [0])
+    ln = _i(np.concatenate([s for _, s in case.land_segs]) if case.land_segs else [0])
+    cap = 3 * ne
+    nedno = np.zeros(2 * cap, np.int32); nedel = np.zeros(2 * cap, np.int32)
+    nedsd = np.zeros(2 * cap, np.int32)
+    niedn = np.zeros(cap, np.int32); nledn = np.zeros(cap, np.int32)
+    needn = np.zeros(cap, np.int32); nfedn = np.zeros(cap, np.int32)
+    counts = np.zeros(5, np.int32)
+    _check(L.dg_create_edge_data(n_nodes, ne, nm_cm, len(case.open_segs), ov, on,
+                                 len(case.land_segs), lt, lv, ln, cap, nedno, nedel, nedsd, niedn,
+                                 nledn, needn, nfedn, counts))
+    ned, ni, nl, no, nf = [int(c) for c in counts]
+    return dict(nedges=ned, nedno=nedno[:2 * ned].reshape(ned, 2), nedel=nedel[:2 * ned].reshape(ned, 2),
+                nedsd=nedsd[:2 * ned].reshape(ned, 2), niedn=niedn[:ni], nledn=nledn[:nl],
+                needn=needn[:no], nfedn=nfedn[:nf])
+
+
+def emo_dg(case: Case) -> tuple:
+    """EMO_DG/EFA_DG(NBFR,NEEDS,2) from per-node EMO/EFA (src/prep_DG.F:183-210), column-major."""
+    p = case.params
+    pairs = []
+    jj = 0
+    for seg in case.open_segs:
+        for k in range(len(seg) - 1):
+            pairs.append((jj, jj + 1)); jj += 1
+        jj += 1
+    needs = len(pairs)
+    emo = np.zeros((p.nbfr, needs, 2)); efa = np.zeros((p.nbfr, needs, 2))
+    for i in range(p.nbfr):
+        for L, (a, b) in enumerate(pairs):
+            emo[i, L] = (p.emo[i, a], p.emo[i, b]); efa[i, L] = (p.efa[i, a], p.efa[i, b])
+    return emo.reshape(-1, order="F"), efa.reshape(-1, order="F")
+
+
+def qnam_dg(case: Case) -> tuple:
+    """QNAM_DG/QNPH_DG(NFFR,NFEDS,2) from per-node QNAM/QNPH (src/prep_DG.F:211-250)."""
+    p = case.params
+    pairs = []
+    jj = 0
+    for t, seg in case.land_segs:
+        if t not in (2, 12, 22):
+            continue
+        for k in range(len(seg) - 1):
+            pairs.append((jj, jj + 1)); jj += 1
+        jj += 1
+    qa = np.zeros((p.nffr, len(pairs), 2)); qp = np.zeros((p.nffr, len(pairs), 2))
+    for i in range(p.nffr):
+        for L, (a, b) in enumerate(pairs):
+            qa[i, L] = (p.qnam[i, a], p.qnam[i, b]); qp[i, L] = (p.qnph[i, a], p.qnph[i, b])
+    return qa.reshape(-1, order="F"), qp.reshape(-1, order="F")
+
+
+class GpuSolver:
+    """One DG-SWEM run on the GPU (one library handle)."""
+
+    def __init__(self, case: Case):
+        L = load_library()
+        self.case = case
+        self.ne, self.np_, self.dofh = case.ne, case.np_, case.dofh
+        ed = create_edge_data(case)
+        self.edges = ed
+        prm = params_of(case)
+        man = _f(case.manning) if case.manning is not None else None
+        h = C.c_int(-1)
+        _check(L.dg_gpu_init(C.byref(prm), case.np_, case.ne, _f(case.x), _f(case.y), _f(case.dp),
+                             _f(case.sfac), _f(case.corif),
+                             man.ctypes.data if man is not None else None,
+                             _i(case.nm.T.reshape(-1)), ed["nedges"], _i(ed["nedno"].reshape(-1)),
+                             _i(ed["nedel"].reshape(-1)), _i(ed["nedsd"].reshape(-1)),
+                             len(ed["niedn"]), _i(ed["niedn"]), len(ed["nledn"]), _i(ed["nledn"]),
+                             len(ed["needn"]), _i(ed["needn"]), len(ed["nfedn"]), _i(ed["nfedn"]),
+                             C.byref(h)))
+        self.h = h.value
+        p = case.params
+        if p.nbfr > 0:
+            emo, efa = emo_dg(case)
+            _check(L.dg_gpu_set_tide(self.h, p.nbfr, _f(p.amig), _f(p.ff), _f(p.face), _f(p.per),
+                                     _f(emo), _f(efa)))
+        if p.nffr > 0:
+            qa, qp = qnam_dg(case)
+            _check(L.dg_gpu_set_flow(self.h, p.nffr, _f(p.famig), _f(p.fff), _f(p.fface),
+                                     _f(p.fper), _f(qa), _f(qp)))
+        self.it = 0
+
+    def close(self):
+        if getattr(self, "h", -1) >= 0:
+            load_library().dg_gpu_finalize(self.h)
+            self.h = -1
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n: int = 1, sync: bool = True) -> None:
+        L = load_library()
+        _check(L.dg_gpu_step(self.h, self.it + 1, n))
+        self.it += n
+        if sync:
+            _check(L.dg_gpu_synchronize(self.h))
+
+    def synchronize(self) -> None:
+        _check(load_library().dg_gpu_synchronize(self.h))
+
+    def state(self):
+        n = self.ne * self.dofh
+        ze, qx, qy = np.empty(n), np.empty(n), np.empty(n)
+        wd = np.empty(self.ne, np.int32)
+        _check(load_library().dg_gpu_get_state(self.h, ze, qx, qy, wd))
+        s = (self.ne, self.dofh)
+        return ze.reshape(s), qx.reshape(s), qy.reshape(s), wd
+
+    def set_state(self, ze, qx, qy, wdflg=None):
+        w = _i(wdflg) if wdflg is not None else None
+        _check(load_library().dg_gpu_set_state(self.h, _f(ze).reshape(-1), _f(qx).reshape(-1),
+                                                _f(qy).reshape(-1),
+                                                w.ctypes.data if w is not None else None))
+
+    def nodal(self):
+        e, u, v = np.empty(self.np_), np.empty(self.np_), np.empty(self.np_)
+        nc = np.empty(self.np_, np.int32)
+        _check(load_library().dg_gpu_modal2nodal(self.h, e, u, v, nc))
+        return e, u, v, nc
+
+    def mass(self) -> float:
+        m = C.c_double()
+        _check(load_library().dg_gpu_mass(self.h, C.byref(m)))
+        return m.value
+
+    def profile(self, enable: bool) -> None:
+        _check(load_library().dg_gpu_profile(self.h, 1 if enable else 0))
+
+    def kernel_stats(self, kid: int):
+        t = C.c_double(); n = C.c_longlong(); name = C.create_string_buffer(64)
+        _check(load_library().dg_gpu_kernel_stats(self.h, kid, C.byref(t), C.byref(n), name, 64))
+        return name.value.decode(), t.value, n.value

@@ -1,0 +1,128 @@
+/*
+ * dgswem_gpu.h -- C ABI of libdgswem_hip.so, the MI355X (gfx950) explicit hydro step of DG-SWEM.
+ *
+ * Every entry point is plain C: scalars by value, arrays as pointers to host memory, 1-based
+ * indices and Fortran column-major layouts exactly as the reference's modules hold them, so that
+ * the Fortran host binds them with iso_c_binding beside src/c_interface.F90 (the reference's only
+ * C-ABI precedent, src/c_interface.F90:3-20 + src/count_edges.cpp:12-40).  All entry points return
+ * 0 on success and a nonzero code on failure; dg_gpu_last_error() gives the message (the Fortran
+ * side STOPs with it, like the reference's fatal-error style).
+ *
+ * Ownership: the library copies every input at the call; it never keeps host pointers.  Device
+ * buffers belong to the handle and are freed by dg_gpu_finalize.  One host thread per handle.
+ */
+#ifndef DGSWEM_GPU_H
+#define DGSWEM_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Run-control scalars after READ_INPUT / PREP_DG (names as in src/global.F, src/dg.F). */
+typedef struct dg_params {
+    int p;            /* PL = PH: polynomial order, 1 or 2 (src/read_fort_dg.F90:455-457)      */
+    int rk_stage;     /* RK_STAGE (NRK)                                                          */
+    int rk_order;     /* RK_ORDER                                                                */
+    int ics;          /* 1 Cartesian, 2 spherical via CPP (SFAC != 1)                             */
+    int nolibf;       /* 0 linear, 1 quadratic/Manning, 2 hybrid bottom friction                 */
+    int nolifa;       /* >=2 switches wetting and drying on                                       */
+    int nolica;       /* advective terms flags (NLEQ/LEQ, src/dgswem.F:150-170)                   */
+    int nolicat;
+    int nramp;        /* 0 or 1: tanh ramp over DRAMP days                                        */
+    int slopeflag;    /* 0 off, otherwise SLOPELIMITER5                                           */
+    int rainfall;     /* 0 off, 1 constant rain rate                                              */
+    int load_manning; /* LoadManningsN: per-node Manning's n is given                            */
+    double g;         /* G                                                                       */
+    double dtdp;      /* DTDP, the time step (s)                                                  */
+    double statim;    /* STATIM (days)                                                            */
+    double dramp;     /* DRAMP (days)                                                             */
+    double h0;        /* H0, minimum depth for wetting and drying                                 */
+    double cf;        /* CF (or TAU for NOLIBF=0)                                                 */
+    double hbreak, ftheta, fgamma; /* hybrid friction parameters                                 */
+    double sfea0;     /* SFEA0 (radians), CPP projection centre latitude                          */
+} dg_params;
+
+/* CREATE_EDGE_DATA + CALC_NORMAL (src/create_edge_data.F:31-635, src/calc_normal.F:13-52):
+ * build the edge tables from NM and the boundary node strings.  Hosts that already hold the
+ * reference's tables (the Fortran driver after PREP_DG) skip this and pass theirs to dg_gpu_init.
+ *   nm          NM(NE,3), 1-based, counter-clockwise (Fortran column-major: nm[k*ne + e])
+ *   nope, nvdll, nbdv      open-boundary segments: NVDLL(k) nodes each, NBDV concatenated
+ *   nbou, ibtype, nvell, nbvv  land/flux segments with their types (0,1,10,11,20,21 land;
+ *                          2,12,22 flux-specified)
+ * Output arrays are caller-allocated with capacity 3*NE edges (ned_cap); counts returned in
+ * counts[5] = {NEDGES, NIEDS, NLEDS, NEEDS, NFEDS}.  nedno/nedel/nedsd are the reference's
+ * (2,MNED) arrays (column-major: [edge][2]); nedel is 0 where an edge has one element. */
+int dg_create_edge_data(int np, int ne, const int* nm,
+                        int nope, const int* nvdll, const int* nbdv,
+                        int nbou, const int* ibtype, const int* nvell, const int* nbvv,
+                        int ned_cap, int* nedno, int* nedel, int* nedsd,
+                        int* niedn, int* nledn, int* needn, int* nfedn, int* counts);
+
+/* Upload mesh, parameters and edge tables; cold-start the state (ZE=QX=QY=0) and set the initial
+ * wet/dry flags the way PREP_DG does (src/prep_DG.F:1214-1300).  Replaces, per step, the
+ * reference's PREP_DG-side tables that DG_HYDRO_TIMESTEP reads (src/dgswem.F:196 call site).
+ *   x, y        projected node coordinates X(NP), Y(NP)
+ *   dp          DP(NP) after READ_INPUT's H0 floor
+ *   sfac        SFAC(NP) (1 for ICS=1)
+ *   corif       CORIF(NP)
+ *   manning     ManningsN(NP), or NULL when not loaded
+ *   edge arrays as produced by dg_create_edge_data / CREATE_EDGE_DATA
+ * *handle receives the new context id. */
+int dg_gpu_init(const dg_params* prm, int np, int ne, const double* x, const double* y,
+                const double* dp, const double* sfac, const double* corif, const double* manning,
+                const int* nm, int nedges, const int* nedno, const int* nedel, const int* nedsd,
+                int nieds, const int* niedn, int nleds, const int* nledn, int needs,
+                const int* needn, int nfeds, const int* nfedn, int* handle);
+
+/* Periodic elevation forcing on ocean edges (replaces the OCEAN_EDGE_HYDRO inputs,
+ * src/ocean_edge_hydro.F:59-90; src/prep_DG.F:183-210 builds EMO_DG/EFA_DG).
+ *   amig, ff, face, per   AMIG(NBFR), FF, FACE (rad), PER
+ *   emo_dg, efa_dg        EMO_DG(NBFR,NEEDS,2), EFA_DG(NBFR,NEEDS,2), column-major */
+int dg_gpu_set_tide(int handle, int nbfr, const double* amig, const double* ff,
+                    const double* face, const double* per, const double* emo_dg,
+                    const double* efa_dg);
+
+/* Periodic normal-flux forcing on flux-specified edges (FLOW_EDGE_HYDRO inputs,
+ * src/flow_edge_hydro.F:45-70; QNAM_DG/QNPH_DG(NFFR,NFEDS,2) from src/prep_DG.F:211-250). */
+int dg_gpu_set_flow(int handle, int nffr, const double* famig, const double* fff,
+                    const double* fface, const double* fper, const double* qnam_dg,
+                    const double* qnph_dg);
+
+/* Replace / read the stage-1 state ZE(DOFH,NE,1), QX, QY (column-major: [elem][mode]) and
+ * WDFLG(NE) -- the arrays DG_HYDRO_TIMESTEP advances (src/dg.F:286-313). */
+int dg_gpu_set_state(int handle, const double* ze, const double* qx, const double* qy,
+                     const int* wdflg);
+int dg_gpu_get_state(int handle, double* ze, double* qx, double* qy, int* wdflg);
+
+/* Advance nsteps time steps it, it+1, ..., it+nsteps-1: each is one DG_HYDRO_TIMESTEP(IT)
+ * (src/DG_hydro_timestep.F:24-744: all RK stages, edge fluxes, volume quadrature, SSP-RK update,
+ * SLOPELIMITER5, WETDRY).  Replaces the call at src/DG_timestep.F:30.  Asynchronous on the
+ * handle's stream; dg_gpu_synchronize waits. */
+int dg_gpu_step(int handle, int it, int nsteps);
+int dg_gpu_synchronize(int handle);
+
+/* modal2nodal (src/write_results.F:14-174): ETA2, UU2, VV2 at the nodes and NODECODE (1 where an
+ * incident element is wet; WETDRY's rule, src/wetdry.F:142-169,525-548). Synchronous. */
+int dg_gpu_modal2nodal(int handle, double* eta2, double* uu2, double* vv2, int* nodecode);
+
+/* Total water volume sum_e A_e*(ze_1 + hb_1) of the stage-1 state (mode 1 is the cell mean). */
+int dg_gpu_mass(int handle, double* mass);
+
+/* Kernel timing with HIP events on the handle's stream.  With enable=1 every kernel launch of
+ * dg_gpu_step is bracketed by events; dg_gpu_kernel_stats returns the summed milliseconds and
+ * launch count of kernel `kernel_id` (0 stage kernel, 1 node min/max, 2 limiter+wet/dry,
+ * 3 modal2nodal) since the last reset.  No counterpart in the reference (it has no timers). */
+int dg_gpu_profile(int handle, int enable);
+int dg_gpu_kernel_stats(int handle, int kernel_id, double* total_ms, long long* launches,
+                        char* name, int name_len);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* dg_gpu_last_error(void);
+
+/* Free the handle's device buffers (src/dgswem.F:429-443 end of run). */
+int dg_gpu_finalize(int handle);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DGSWEM_GPU_H */
