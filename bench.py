@@ -1,0 +1,157 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X DG hydro step: DOF-stage updates/s, FP64, p=1, ~1M elements.
+
+Workload (the plain method at scale): seeded Delaunay basin of ~1M triangles (500k nodes) over
+1000 km x 600 km, Gaussian-random-field bathymetry 5-4000 m with a shelf rising to +2 m onshore,
+5 tidal constituents on the open boundary, per-node Manning n in [0.02, 0.03], SSP(2,2), slope
+limiter and wetting/drying on, dt from CFL 0.3.  A "step" is one full time step (both RK stages
+with edge fluxes, volume quadrature, RK update, limiter and wet/dry) over all elements.
+
+    python bench.py [--gpus 1] [--steps 2000] [--warmup 50] [--p 1] [--elements 1000000]
+
+Prints one JSON line.  `value` = NE * 3 * Np * NRK * steps / wall time of the timed steps, with the
+state resident in HBM.  `roofline` is for the stage kernel: algorithmic bytes per launch
+(300 B per element-stage for p=1, 480 B for p=2) over its mean launch time from HIP events
+recorded on the library's stream during the timed region.  `cpu_baseline` times the CPU oracle
+(a bit-exact restatement of dgswem_serial's step) with OpenMP on this host's cores on a short
+sample of the same mesh.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0            # MI355X HBM3E
+B_PER_ELEM_STAGE = {1: 300.0, 2: 480.0}
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(p: int):
+    """HBM bytes per stage-kernel launch from the committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"pmc_stage_kernel_delaunay1m_p{p}.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return float(json.load(f)["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(case, sample_steps: int, threads: int):
+    from oracle.oracle import Oracle
+    o = Oracle(case, threads=threads)
+    o.step(1)                                          # warm caches / first touch
+    t0 = time.perf_counter()
+    o.step(sample_steps)
+    dt = time.perf_counter() - t0
+    dof = case.ne * 3 * case.dofh * case.dg.rk_stage * sample_steps
+    return dof / dt, dt
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--p", type=int, default=1)
+    ap.add_argument("--elements", type=int, default=1_000_000)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=0, help="oracle sample steps (0 = auto)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the step shards by element partition, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    from dgswem_amd import gpu, mesh
+
+    t0 = time.perf_counter()
+    case = mesh.delaunay_basin(n_elem=args.elements, seed=args.seed, p=args.p,
+                               nsteps=args.steps + args.warmup)
+    log(f"mesh: {case.ne} elements, {case.np_} nodes, dt {case.params.dtdp:.4f} s "
+        f"({time.perf_counter() - t0:.1f} s)")
+    g = gpu.GpuSolver(case)
+    log("library initialised")
+    g.step(args.warmup)
+    g.synchronize()
+    g.profile(True)
+    g.synchronize()
+    t1 = time.perf_counter()
+    g.step(args.steps, sync=False)
+    g.synchronize()
+    wall = time.perf_counter() - t1
+    stats = {k: g.kernel_stats(k) for k in (gpu.K_STAGE, gpu.K_MINMAX, gpu.K_LIMIT)}
+    g.profile(False)
+    ze, qx, qy, wd = g.state()
+    import numpy as np
+    finite = bool(np.isfinite(ze).all() and np.isfinite(qx).all() and np.isfinite(qy).all())
+    ne, dof, nrk = case.ne, case.dofh, case.dg.rk_stage
+    dof_stage = ne * 3 * dof * nrk * args.steps
+    value = dof_stage / wall
+    name, tot_ms, launches = stats[gpu.K_STAGE]
+    avg_s = tot_ms / 1e3 / max(launches, 1)
+    alg_bytes = B_PER_ELEM_STAGE[args.p] * ne
+    achieved = alg_bytes / avg_s / 1e9
+    kernels = {}
+    for k, (nm, ms, cnt) in stats.items():
+        if cnt:
+            kernels[nm] = {"launches": cnt, "avg_us": round(ms * 1e3 / cnt, 2),
+                           "share_of_wall": round(ms / 1e3 / wall, 4)}
+    log(f"timed {args.steps} steps: {wall * 1e3:.1f} ms, stage kernel {avg_s * 1e6:.1f} us/launch, "
+        f"{achieved:.0f} GB/s algorithmic, finite={finite}")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = len(os.sched_getaffinity(0))
+        steps_cpu = args.cpu_steps or max(2, int(30 * 1_000_000 / max(ne, 1)))
+        log(f"cpu baseline: oracle, {threads} threads, {steps_cpu} steps")
+        rate, secs = cpu_baseline(case, steps_cpu, threads)
+        cpu = {"value": rate, "unit": "DOF-stage updates/s", "cores": threads, "kind": "port",
+               "sample": f"{steps_cpu} steps of the same {ne}-element mesh ({secs:.1f} s) with the "
+                         f"OpenMP oracle (bit-exact restatement of dgswem_serial's step)"}
+
+    out = {
+        "metric": "DOF-stage updates/s FP64 p=1 1M elems",
+        "value": value,
+        "unit": "DOF-stage updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"seeded Delaunay tidal basin with shelf, {ne} elements, p={args.p}, "
+                               f"SSP(2,2), slope limiter 5, wetting/drying, 5 constituents",
+                   "elements": ne, "nodes": case.np_, "p": args.p, "rk_stages": nrk,
+                   "dt_s": case.params.dtdp, "limiter": case.dg.slopeflag,
+                   "wetdry": case.params.nolifa >= 2, "seed": args.seed, "state_finite": finite},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(args.p),
+                     "kernel": name, "avg_launch_us": avg_s * 1e6,
+                     "algorithmic_bytes_per_launch": alg_bytes},
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu_baseline"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
+    g.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1128,7 +1128,8 @@ static void wetdry(oracle_t *o) {
     const double ZERO = 1.0e-6, HUGEV = 1.0e6, H0 = o->h0;
     const double C13 = 1.0 / 3.0, C16 = 1.0 / 6.0;
     for (int i = 0; i < o->np; i++) o->nodecode[i] = 0;
-    /* element loop is sequential: NODECODE writes are benign but keep the reference's order */
+    /* elements are independent; NODECODE only ever receives the value 1 (atomic writes) */
+#pragma omp parallel for num_threads(o->nthreads) schedule(static)
     for (int J = 0; J < o->ne; J++) {
         int no[3] = {o->nm[3 * J], o->nm[3 * J + 1], o->nm[3 * J + 2]};
         double *ze = &o->ze[((size_t)s * o->ne + J) * dofh];
@@ -1151,7 +1152,10 @@ static void wetdry(oracle_t *o) {
         if (ndry == 0) {
             if (o->wdflg[J] == 1) {
                 check = 0;
-                o->nodecode[no[0]] = 1; o->nodecode[no[1]] = 1; o->nodecode[no[2]] = 1;
+                for (int c = 0; c < 3; c++) {
+#pragma omp atomic write
+                    o->nodecode[no[c]] = 1;
+                }
             } else {
                 for (int c = 0; c < 3; c++) ze_hat[c] = ze_node[c];
                 check = 1;
@@ -1222,7 +1226,10 @@ static void wetdry(oracle_t *o) {
         if (check) {
             if (ze_hat[ht_max_i] > (-o->dpe_min[J] + H0)) {
                 o->wdflg[J] = 1;
-                o->nodecode[no[0]] = 1; o->nodecode[no[1]] = 1; o->nodecode[no[2]] = 1;
+                for (int c = 0; c < 3; c++) {
+#pragma omp atomic write
+                    o->nodecode[no[c]] = 1;
+                }
             } else {
                 for (int k = 0; k < dofh; k++) { qx[k] = 0.0; qy[k] = 0.0; }
                 o->wdflg[J] = 0;
