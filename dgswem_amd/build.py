@@ -42,10 +42,12 @@ def build_oracle() -> str:
 
 
 def build_reference() -> bool:
-    """oracle/_ref/dgswem_serial from /root/reference (only where the reference is present)."""
+    """oracle/_ref/dgswem_serial from /root/reference, and oracle/_ref/dgswem_gpu (the reference's
+    Fortran host linked with the GPU step), only where the reference sources are present."""
     if not os.path.isdir(os.environ.get("DGSWEM_REFERENCE", "/root/reference")):
         return False
     _run(["bash", os.path.join(ROOT, "oracle", "build_ref.sh")])
+    _run(["bash", os.path.join(ROOT, "oracle", "build_dropin.sh")])
     return True
 
 

@@ -201,8 +201,8 @@ int build_stage_tables(Ctx& c, int it, int n) {
 template <int P>
 int launch_step(Ctx& c, int row0) {
     const int ne = c.ne, np = c.np;
-    const int B = 256;
-    const int ge = (ne + B - 1) / B, gn = (np + B - 1) / B;
+    const int B = 256, BS = StageBlock<P>::N;
+    const int ge = (ne + B - 1) / B, gn = (np + B - 1) / B, gs = (ne + BS - 1) / BS;
     const bool wdon = c.prm.nolifa >= 2;
     const bool lim = c.prm.slopeflag != 0;
     const bool sph = c.prm.ics == 2;
@@ -236,11 +236,11 @@ int launch_step(Ctx& c, int row0) {
         const bool fuse = wdon && !lim;
         int ev = prof_begin(c, K_STAGE);
         if (sph) {
-            if (fuse) hipLaunchKernelGGL((stage_kernel<P, true, true>), dim3(ge), dim3(B), 0, c.stream, a);
-            else hipLaunchKernelGGL((stage_kernel<P, true, false>), dim3(ge), dim3(B), 0, c.stream, a);
+            if (fuse) hipLaunchKernelGGL((stage_kernel<P, true, true>), dim3(gs), dim3(BS), 0, c.stream, a);
+            else hipLaunchKernelGGL((stage_kernel<P, true, false>), dim3(gs), dim3(BS), 0, c.stream, a);
         } else {
-            if (fuse) hipLaunchKernelGGL((stage_kernel<P, false, true>), dim3(ge), dim3(B), 0, c.stream, a);
-            else hipLaunchKernelGGL((stage_kernel<P, false, false>), dim3(ge), dim3(B), 0, c.stream, a);
+            if (fuse) hipLaunchKernelGGL((stage_kernel<P, false, true>), dim3(gs), dim3(BS), 0, c.stream, a);
+            else hipLaunchKernelGGL((stage_kernel<P, false, false>), dim3(gs), dim3(BS), 0, c.stream, a);
         }
         prof_end(c, ev);
         HIPCHK(hipGetLastError());

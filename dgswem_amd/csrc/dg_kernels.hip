@@ -168,16 +168,33 @@ __device__ __forceinline__ int wetdry_element(const DevTables& T, double (&U)[3]
 //   SSP-RK update      src/DG_hydro_timestep.F:238-290
 //   element geometry   src/read_input.F:1408, src/prep_DG.F:959-1163, src/calc_normal.F:28-49
 // ------------------------------------------------------------------------------------------
+template <int P> struct StageBlock { static constexpr int N = (P == 1) ? 256 : 128; };
+
+// Trace of the element's modes at a point with basis values ph[] (sum from zero, mode order).
+template <int DOF>
+__device__ __forceinline__ double trace(const double (&u)[DOF], const double* ph) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < DOF; k++) s = s + u[k] * ph[k];
+    return s;
+}
+
 template <int P, bool SPH, bool WDFUSED>
-__global__ void __launch_bounds__(256) stage_kernel(StageArgs a) {
+__global__ void __launch_bounds__(StageBlock<P>::N) stage_kernel(StageArgs a) {
     constexpr int DOF = (P + 1) * (P + 2) / 2;
     constexpr int NAGP = (P == 1) ? 3 : 6;
     constexpr int NEGP = P + 2;
+    constexpr int NT = StageBlock<P>::N;
+    // per-thread accumulators kept in LDS to bound register use: [0] boundary-edge part of the
+    // RHS (accumulated first, like the reference's boundary edge routines), [1] internal-edge
+    // slot sum (add_internal_flux)
+    __shared__ double acc_lds[2][3 * DOF][NT];
     const DevTables& T = tab<P>();
     const ElemData& E = a.ed;
     const Phys& ph = a.ph;
     const int ne = E.ne;
-    const int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int tid = threadIdx.x;
+    const int e = xcd_tile(blockIdx.x, gridDim.x) * NT + tid;
     if (e >= ne) return;
 
     double U[3][DOF];
@@ -186,22 +203,17 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs a) {
 #pragma unroll
         for (int k = 0; k < DOF; k++) U[f][k] = a.uin[(size_t)(f * DOF + k) * ne + e];
     const int wd = a.wd_in[e];
-    double X[3], Y[3], D[3];
-#pragma unroll
-    for (int v = 0; v < 3; v++) {
-        X[v] = E.xv[(size_t)v * ne + e]; Y[v] = E.yv[(size_t)v * ne + e]; D[v] = E.dpv[(size_t)v * ne + e];
-    }
-    const double AREAS = (X[0] - X[2]) * (Y[1] - Y[2]) + (X[2] - X[1]) * (Y[0] - Y[2]);
     const double c13 = 1.0 / 3.0, c16 = 1.0 / 6.0;
-    const double hb[3] = {c13 * (D[0] + D[1] + D[2]), -(c16 * (D[0] + D[1])) + c13 * D[2],
-                          -(0.5 * D[0]) + 0.5 * D[1]};
-
-    double R[3][DOF];    // boundary-edge part of the RHS (accumulated first)
-    double S[3][DOF];    // internal-edge slot sum
+    double AREAS, hb[3];
+    {
+        const double X0 = E.xv[e], X1 = E.xv[(size_t)ne + e], X2 = E.xv[2 * (size_t)ne + e];
+        const double Y0 = E.yv[e], Y1 = E.yv[(size_t)ne + e], Y2 = E.yv[2 * (size_t)ne + e];
+        const double D0 = E.dpv[e], D1 = E.dpv[(size_t)ne + e], D2 = E.dpv[2 * (size_t)ne + e];
+        AREAS = (X0 - X2) * (Y1 - Y2) + (X2 - X1) * (Y0 - Y2);
+        hb[0] = c13 * (D0 + D1 + D2); hb[1] = -(c16 * (D0 + D1)) + c13 * D2; hb[2] = -(0.5 * D0) + 0.5 * D1;
+    }
 #pragma unroll
-    for (int f = 0; f < 3; f++)
-#pragma unroll
-        for (int k = 0; k < DOF; k++) { R[f][k] = 0.0; S[f][k] = 0.0; }
+    for (int j = 0; j < 3 * DOF; j++) { acc_lds[0][j][tid] = 0.0; acc_lds[1][j][tid] = 0.0; }
 
     const int info = E.sinfo[e];
 #pragma unroll 1
@@ -209,12 +221,14 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs a) {
         const int bits = side_bits(info, slot);
         const int L = bits & 3, role = (bits >> 2) & 1, kind = (bits >> 3) & 3, nled = (bits >> 5) & 3;
         const int nb = E.snbr[(size_t)slot * ne + e];
-        // edge geometry in the reference's node order (the order of NEDEL(1)'s local edge)
+        // edge geometry in the reference's node order (the counter-clockwise order of NEDEL(1))
         int va = (L + 1) % 3, vb = (L + 2) % 3;
         if (role) { const int t = va; va = vb; vb = t; }
-        const double dy = Y[vb] - Y[va], dx = X[vb] - X[va];
+        const double dy = E.yv[(size_t)vb * ne + e] - E.yv[(size_t)va * ne + e];
+        const double dx = E.xv[(size_t)vb * ne + e] - E.xv[(size_t)va * ne + e];
         const double xlen = sqrt(dy * dy + dx * dx);
         const double nx = dy / xlen, ny = -dx / xlen;
+        const double tx = -ny, ty = nx;
         if (kind == SIDE_INTERNAL) {
             const int wdn = a.wd_in[nb];
             if (wd == 0 && wdn == 0) continue;
@@ -223,102 +237,93 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs a) {
             for (int f = 0; f < 3; f++)
 #pragma unroll
                 for (int k = 0; k < DOF; k++) V[f][k] = a.uin[(size_t)(f * DOF + k) * ne + nb];
-            const double d0 = E.dpv[nb], d1 = E.dpv[(size_t)ne + nb], d2 = E.dpv[2 * (size_t)ne + nb];
-            const double hbn[3] = {c13 * (d0 + d1 + d2), -(c16 * (d0 + d1)) + c13 * d2, -(0.5 * d0) + 0.5 * d1};
-            const int led_in = role ? nled : L, led_ex = role ? L : nled;
+            double hbn[3];
+            {
+                const double d0 = E.dpv[nb], d1 = E.dpv[(size_t)ne + nb], d2 = E.dpv[2 * (size_t)ne + nb];
+                hbn[0] = c13 * (d0 + d1 + d2); hbn[1] = -(c16 * (d0 + d1)) + c13 * d2; hbn[2] = -(0.5 * d0) + 0.5 * d1;
+            }
             const int wd_in_el = role ? wdn : wd, wd_ex_el = role ? wd : wdn;
             const double edfac = xlen / AREAS;
-            const double tx = -ny, ty = nx;
             double acc[3][DOF];
 #pragma unroll
             for (int f = 0; f < 3; f++)
 #pragma unroll
                 for (int k = 0; k < DOF; k++) acc[f][k] = 0.0;
-#pragma unroll
+#pragma unroll 1
             for (int I = 0; I < NEGP; I++) {
                 const int gi = I, gx = NEGP - 1 - I;
-                double ze_in = 0.0, qx_in = 0.0, qy_in = 0.0, ze_ex = 0.0, qx_ex = 0.0, qy_ex = 0.0;
-                double hb_in = 0.0, hb_ex = 0.0;
-#pragma unroll
-                for (int k = 0; k < DOF; k++) {
-                    const double pi = T.phi_edge[led_in][gi][k], px = T.phi_edge[led_ex][gx][k];
-                    const double zi = role ? V[0][k] : U[0][k], xi = role ? V[1][k] : U[1][k], yi = role ? V[2][k] : U[2][k];
-                    const double zx = role ? U[0][k] : V[0][k], xx = role ? U[1][k] : V[1][k], yx = role ? U[2][k] : V[2][k];
-                    ze_in = ze_in + zi * pi; qx_in = qx_in + xi * pi; qy_in = qy_in + yi * pi;
-                    ze_ex = ze_ex + zx * px; qx_ex = qx_ex + xx * px; qy_ex = qy_ex + yx * px;
-                }
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    hb_in = hb_in + (role ? hbn[k] : hb[k]) * T.phi_edge[led_in][gi][k];
-                    hb_ex = hb_ex + (role ? hb[k] : hbn[k]) * T.phi_edge[led_ex][gx][k];
-                }
-                double sf_in = 1.0, sf_ex = 1.0;
+                const int gown = role ? gx : gi, gnb = role ? gi : gx;
+                const double* po = T.phi_edge[L][gown];
+                const double* pn = T.phi_edge[nled][gnb];
+                const double zo = trace<DOF>(U[0], po), xo = trace<DOF>(U[1], po), yo = trace<DOF>(U[2], po);
+                const double zn = trace<DOF>(V[0], pn), xn = trace<DOF>(V[1], pn), yn = trace<DOF>(V[2], pn);
+                const double ho = trace<3>(hb, po), hn = trace<3>(hbn, pn);
+                double so = 1.0, sn = 1.0;
                 if (SPH) {
-                    const int el_in = role ? nb : e, el_ex = role ? e : nb;
-                    sf_in = E.sfaced[((size_t)led_in * NEGP + gi) * ne + el_in];
-                    sf_ex = E.sfaced[((size_t)led_ex * NEGP + gx) * ne + el_ex];
+                    so = E.sfaced[((size_t)L * NEGP + gown) * ne + e];
+                    sn = E.sfaced[((size_t)nled * NEGP + gnb) * ne + nb];
                 }
+                const double ze_in = role ? zn : zo, qx_in = role ? xn : xo, qy_in = role ? yn : yo;
+                const double ze_ex = role ? zo : zn, qx_ex = role ? xo : xn, qy_ex = role ? yo : yn;
+                const double hb_in = role ? hn : ho, hb_ex = role ? ho : hn;
+                const double sf_in = role ? sn : so, sf_ex = role ? so : sn;
                 double f0, f1, f2;
                 llf_flux(ph, ze_in, ze_ex, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_ex, sf_in, sf_ex, nx, ny, ph.g, f0, f1, f2);
-                double fo = f0, gin = f1, hin = f2, gex = f1, hex = f2;
+                double fo = f0, gm = role ? f1 : f1, hm = f2;
+                // dry-side treatment (src/internal_edge_hydro.F:212-302): 1 reflect into a dry
+                // "in" element, 2 zero-gravity flux for a dry "in", 3 reflect into a dry "ex",
+                // 4 zero-gravity flux for a dry "ex"; only what this element's side receives
+                int mode = 0;
                 if (fabs(f0) > 1.0e-8) {
-                    if (wd_in_el == 0) {
-                        if (f0 > 0.0) {
-                            // flux out of the dry element: reflect the wet side's state
-                            const double qn_ext = qx_ex * nx + qy_ex * ny, qt_ext = qx_ex * tx + qy_ex * ty;
-                            const double qn_int = -qn_ext, qt_int = qt_ext;
-                            const double den = (nx * ty - ny * tx);
-                            const double mqx = (ty * qn_int - ny * qt_int) / den;
-                            const double mqy = (-tx * qn_int + nx * qt_int) / den;
-                            llf_flux(ph, ze_ex, ze_ex, mqx, qx_ex, mqy, qy_ex, hb_ex, hb_ex, sf_ex, sf_ex, nx, ny, ph.g, f0, f1, f2);
-                            fo = f0; gex = f1; hex = f2; gin = 0.0; hin = 0.0;
-                        } else if (!role) {
-                            llf_flux(ph, ze_in, ze_ex, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_ex, sf_in, sf_ex, nx, ny, 0.0, f0, f1, f2);
-                            gin = f1; hin = f2;
-                        }
-                    } else if (wd_ex_el == 0) {
-                        if (f0 < 0.0) {
-                            const double qn_int = qx_in * nx + qy_in * ny, qt_int = qx_in * tx + qy_in * ty;
-                            const double qn_ext = -qn_int, qt_ext = qt_int;
-                            const double den = (nx * ty - ny * tx);
-                            const double mqx = (ty * qn_ext - ny * qt_ext) / den;
-                            const double mqy = (-tx * qn_ext + nx * qt_ext) / den;
-                            llf_flux(ph, ze_in, ze_in, qx_in, mqx, qy_in, mqy, hb_in, hb_in, sf_in, sf_in, nx, ny, ph.g, f0, f1, f2);
-                            fo = f0; gin = f1; hin = f2; gex = 0.0; hex = 0.0;
-                        } else if (role) {
-                            llf_flux(ph, ze_in, ze_ex, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_ex, sf_in, sf_ex, nx, ny, 0.0, f0, f1, f2);
-                            gex = f1; hex = f2;
-                        }
-                    }
+                    if (wd_in_el == 0) mode = (f0 > 0.0) ? 1 : (role ? 0 : 2);
+                    else if (wd_ex_el == 0) mode = (f0 < 0.0) ? 3 : (role ? 4 : 0);
                 }
-                const int mygp = role ? gx : gi;
+                if (mode) {
+                    double a_zi = ze_in, a_xi = qx_in, a_yi = qy_in, a_hi = hb_in, a_si = sf_in;
+                    double a_ze = ze_ex, a_xe = qx_ex, a_ye = qy_ex, a_he = hb_ex, a_se = sf_ex;
+                    double G = ph.g;
+                    if (mode == 1 || mode == 3) {
+                        const double sx = (mode == 1) ? qx_ex : qx_in, sy = (mode == 1) ? qy_ex : qy_in;
+                        const double qn = sx * nx + sy * ny, qt = sx * tx + sy * ty;
+                        const double qnm = -qn, qtm = qt;
+                        const double den = (nx * ty - ny * tx);
+                        const double mqx = (ty * qnm - ny * qtm) / den;
+                        const double mqy = (-tx * qnm + nx * qtm) / den;
+                        if (mode == 1) { a_zi = ze_ex; a_xi = mqx; a_yi = mqy; a_hi = hb_ex; a_si = sf_ex; }
+                        else { a_ze = ze_in; a_xe = mqx; a_ye = mqy; a_he = hb_in; a_se = sf_in; }
+                    } else {
+                        G = 0.0;
+                    }
+                    double r0, r1, r2;
+                    llf_flux(ph, a_zi, a_ze, a_xi, a_xe, a_yi, a_ye, a_hi, a_he, a_si, a_se, nx, ny, G, r0, r1, r2);
+                    // mode 1: fo, ex momentum from the reflected flux, zero for the dry "in"
+                    // mode 3: fo, in momentum from the reflected flux, zero for the dry "ex"
+                    if (mode == 1) { fo = r0; gm = role ? r1 : 0.0; hm = role ? r2 : 0.0; }
+                    else if (mode == 3) { fo = r0; gm = role ? 0.0 : r1; hm = role ? 0.0 : r2; }
+                    else { gm = r1; hm = r2; }
+                }
+                const double* eq = T.edgeq[L][gown];
 #pragma unroll
                 for (int k = 0; k < DOF; k++) {
-                    const double w = edfac * T.edgeq[L][mygp][k];
+                    const double w = edfac * eq[k];
                     if (!role) {
-                        acc[0][k] = acc[0][k] - w * fo; acc[1][k] = acc[1][k] - w * gin; acc[2][k] = acc[2][k] - w * hin;
+                        acc[0][k] = acc[0][k] - w * fo; acc[1][k] = acc[1][k] - w * gm; acc[2][k] = acc[2][k] - w * hm;
                     } else {
-                        acc[0][k] = acc[0][k] + w * fo; acc[1][k] = acc[1][k] + w * gex; acc[2][k] = acc[2][k] + w * hex;
+                        acc[0][k] = acc[0][k] + w * fo; acc[1][k] = acc[1][k] + w * gm; acc[2][k] = acc[2][k] + w * hm;
                     }
                 }
             }
 #pragma unroll
             for (int f = 0; f < 3; f++)
 #pragma unroll
-                for (int k = 0; k < DOF; k++) S[f][k] = S[f][k] + acc[f][k];
+                for (int k = 0; k < DOF; k++) acc_lds[1][f * DOF + k][tid] = acc_lds[1][f * DOF + k][tid] + acc[f][k];
         } else {
             if (wd == 0) continue;
-            const double tx = -ny, ty = nx;
-#pragma unroll
+#pragma unroll 1
             for (int I = 0; I < NEGP; I++) {
-                double ze_in = 0.0, qx_in = 0.0, qy_in = 0.0, hb_in = 0.0;
-#pragma unroll
-                for (int k = 0; k < DOF; k++) {
-                    const double pv = T.phi_edge[L][I][k];
-                    ze_in = ze_in + U[0][k] * pv; qx_in = qx_in + U[1][k] * pv; qy_in = qy_in + U[2][k] * pv;
-                }
-#pragma unroll
-                for (int k = 0; k < 3; k++) hb_in = hb_in + hb[k] * T.phi_edge[L][I][k];
+                const double* pv = T.phi_edge[L][I];
+                const double ze_in = trace<DOF>(U[0], pv), qx_in = trace<DOF>(U[1], pv), qy_in = trace<DOF>(U[2], pv);
+                const double hb_in = trace<3>(hb, pv);
                 const double sf = SPH ? E.sfaced[((size_t)L * NEGP + I) * ne + e] : 1.0;
                 double f0, f1, f2;
                 if (kind == SIDE_LAND) {
@@ -361,57 +366,62 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs a) {
                 }
 #pragma unroll
                 for (int k = 0; k < DOF; k++) {
-                    const double w = 2.0 * T.m_inv[k] / AREAS * xlen * T.phi_edge[L][I][k] * T.wegp[I];
-                    R[0][k] = R[0][k] - w * f0; R[1][k] = R[1][k] - w * f1; R[2][k] = R[2][k] - w * f2;
+                    const double w = 2.0 * T.m_inv[k] / AREAS * xlen * pv[k] * T.wegp[I];
+                    acc_lds[0][k][tid] = acc_lds[0][k][tid] - w * f0;
+                    acc_lds[0][DOF + k][tid] = acc_lds[0][DOF + k][tid] - w * f1;
+                    acc_lds[0][2 * DOF + k][tid] = acc_lds[0][2 * DOF + k][tid] - w * f2;
                 }
             }
         }
     }
-    // add_internal_flux: RHS = RHS(boundary) + slot sum
+    // add_internal_flux: RHS = RHS(boundary edges) + slot sum
+    double R[3][DOF];
 #pragma unroll
     for (int f = 0; f < 3; f++)
 #pragma unroll
-        for (int k = 0; k < DOF; k++) R[f][k] = R[f][k] + S[f][k];
+        for (int k = 0; k < DOF; k++) R[f][k] = acc_lds[0][f * DOF + k][tid] + acc_lds[1][f * DOF + k][tid];
 
     // ---- volume quadrature (RHS_DG_HYDRO)
     if (a.rain > 0.0) {
-#pragma unroll
+#pragma unroll 1
         for (int I = 0; I < NAGP; I++)
 #pragma unroll
             for (int k = 0; k < DOF; k++) R[0][k] = R[0][k] + T.srfac[I][k] * a.rain;
     }
     if (wd != 0) {
-        double AREA = (X[0] - X[2]) * (Y[1] - Y[2]) + (X[2] - X[1]) * (Y[0] - Y[2]);
-        AREA = AREA * 0.5;
-        const double drdx = 1.0 / AREA * (Y[2] - Y[0]);
-        const double dsdx = 1.0 / AREA * (Y[0] - Y[1]);
-        const double drdy = 1.0 / AREA * (X[0] - X[2]);
-        const double dsdy = 1.0 / AREA * (X[1] - X[0]);
+        double drdx, dsdx, drdy, dsdy;
+        {
+            const double X0 = E.xv[e], X1 = E.xv[(size_t)ne + e], X2 = E.xv[2 * (size_t)ne + e];
+            const double Y0 = E.yv[e], Y1 = E.yv[(size_t)ne + e], Y2 = E.yv[2 * (size_t)ne + e];
+            double AREA = (X0 - X2) * (Y1 - Y2) + (X2 - X1) * (Y0 - Y2);
+            AREA = AREA * 0.5;
+            drdx = 1.0 / AREA * (Y2 - Y0);
+            dsdx = 1.0 / AREA * (Y0 - Y1);
+            drdy = 1.0 / AREA * (X0 - X2);
+            dsdy = 1.0 / AREA * (X1 - X0);
+        }
         const double depth_c = hb[0] + U[0][0];
         const double fh_nl_c = 1.0 / (ph.nleq * depth_c + ph.leq);
         const double cori = E.cori_el ? E.cori_el[e] : ph.cori;
         const double nbar = ph.load_manning ? E.nbar[e] : 0.0;
-#pragma unroll
+#pragma unroll 1
         for (int I = 0; I < NAGP; I++) {
+            const double* pa = T.phi_area[I];
             double ze = U[0][0], qx = U[1][0], qy = U[2][0];
 #pragma unroll
             for (int k = 1; k < DOF; k++) {
-                ze = ze + U[0][k] * T.phi_area[I][k];
-                qx = qx + U[1][k] * T.phi_area[I][k];
-                qy = qy + U[2][k] * T.phi_area[I][k];
+                ze = ze + U[0][k] * pa[k];
+                qx = qx + U[1][k] * pa[k];
+                qy = qy + U[2][k] * pa[k];
             }
             double hbq = 0.0, dhbx = 0.0, dhby = 0.0;
-            double dpx[DOF], dpy[DOF];
-#pragma unroll
-            for (int k = 0; k < DOF; k++) {
-                dpx[k] = T.drphi[I][k] * drdx + T.dsphi[I][k] * dsdx;
-                dpy[k] = T.drphi[I][k] * drdy + T.dsphi[I][k] * dsdy;
-            }
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                hbq = hbq + hb[k] * T.phi_area[I][k];
-                dhbx = dhbx + hb[k] * dpx[k];
-                dhby = dhby + hb[k] * dpy[k];
+                const double dpxk = T.drphi[I][k] * drdx + T.dsphi[I][k] * dsdx;
+                const double dpyk = T.drphi[I][k] * drdy + T.dsphi[I][k] * dsdy;
+                hbq = hbq + hb[k] * pa[k];
+                dhbx = dhbx + hb[k] * dpxk;
+                dhby = dhby + hb[k] * dpyk;
             }
             const double sfq = SPH ? E.sfacq[(size_t)I * ne + e] : 1.0;
             const double depth = ze + hbq;
@@ -444,7 +454,9 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs a) {
             R[2][0] = R[2][0] + T.srfac[I][0] * sy;
 #pragma unroll
             for (int k = 1; k < DOF; k++) {
-                const double xf = T.mw[I][k] * dpx[k], yf = T.mw[I][k] * dpy[k], sr = T.srfac[I][k];
+                const double dpxk = T.drphi[I][k] * drdx + T.dsphi[I][k] * dsdx;
+                const double dpyk = T.drphi[I][k] * drdy + T.dsphi[I][k] * dsdy;
+                const double xf = T.mw[I][k] * dpxk, yf = T.mw[I][k] * dpyk, sr = T.srfac[I][k];
                 R[0][k] = R[0][k] + xf * fx + yf * fy;
                 R[1][k] = R[1][k] + xf * gx + yf * gy + sr * sx;
                 R[2][k] = R[2][k] + xf * hx + yf * hy + sr * sy;
@@ -463,6 +475,7 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs a) {
     for (int f = 0; f < 3; f++)
 #pragma unroll
         for (int k = 0; k < DOF; k++) W[f][k] = 0.0;
+#pragma unroll 1
     for (int i = 0; i < a.nprev; i++) {
         const double ai = a.aprev[i], bi = a.bprev[i];
         if (ai != 0.0) {
@@ -486,6 +499,7 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs a) {
         for (int k = 0; k < DOF; k++) W[f][k] = W[f][k] + a.acur * U[f][k] + a.bcur * R[f][k];
 
     if (WDFUSED) {
+        const double D[3] = {E.dpv[e], E.dpv[(size_t)ne + e], E.dpv[2 * (size_t)ne + e]};
         const double dpe_min = fmin(fmin(D[0], D[1]), D[2]);
         const int nw = wetdry_element<P>(T, W, D, hb, dpe_min, wd, ph.h0);
         a.wd_out[e] = (uint8_t)nw;

@@ -15,7 +15,7 @@ import numpy as np
 from .fortio import Case
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libdgswem_hip.so")
+LIB_PATH = os.environ.get("DGSWEM_HIP_LIB") or os.path.join(_HERE, "libdgswem_hip.so")
 _LIB = None
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")

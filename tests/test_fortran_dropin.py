@@ -1,0 +1,56 @@
+"""The reference's own Fortran host driving the GPU step (oracle/_ref/dgswem_gpu).
+
+oracle/build_dropin.sh links every object of the reference's serial solver except DG_timestep.o
+with dgswem_amd/fortran/dg_timestep_gpu.F90 (DG_TIMESTEP calling the C ABI through
+dgswem_amd/fortran/gpu_interface.F90) and libdgswem_hip.so.  The binary reads the same
+fort.14/15/dg/13 and writes the same fort.63 / DG hot start as dgswem_serial.
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, golden, load_case
+from dgswem_amd import fortio, mesh
+
+pytestmark = pytest.mark.gpu
+BIN = os.path.join(ROOT, "oracle", "_ref", "dgswem_gpu")
+
+
+def run(d):
+    if not os.path.exists(BIN):
+        pytest.skip("oracle/_ref/dgswem_gpu is not built (needs the reference sources)")
+    r = subprocess.run([BIN], cwd=d, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0 and "CPU_TIME" in r.stdout, r.stdout[-3000:] + r.stderr[-2000:]
+
+
+@pytest.mark.parametrize("name", ["quarter_annular", "wetdry", "mass_conservation"])
+def test_dropin_reproduces_reference_fort63(name, tmp_path):
+    d = str(tmp_path)
+    for f in ("fort.14", "fort.15", "fort.dg"):
+        shutil.copy(os.path.join(GOLDEN, name, f), d)
+    if name == "wetdry":
+        case = fortio.barriers_as_land(fortio.read_case(d))
+        fortio.write_fort14(os.path.join(d, "fort.14"), case, "wetdry-land")
+    print(f"dropin {name}", flush=True)
+    run(d)
+    _, _, got = fortio.read_fort63(os.path.join(d, "fort.63"))
+    ref = golden(f"{name}_fort63_last")["eta"]
+    np.testing.assert_array_equal(got[-1] == -99999.0, ref == -99999.0)
+    np.testing.assert_allclose(got[-1], ref, rtol=0, atol=2e-8 * np.abs(ref).max())
+
+
+def test_dropin_hot_start_matches_reference_fp64(tmp_path):
+    """Config-1 basin, 1000 steps: the DG hot start written by the drop-in vs dgswem_serial's."""
+    d = str(tmp_path)
+    case = mesh.structured_basin(nsteps=1000)
+    mesh.write_run_dir(d, case, dghot=1, dghotspool=1000, nspoolge=1000)
+    run(d)
+    hs = fortio.read_dg_hotstart(d, 3, case.ne)
+    ref = golden("c1_step1000")
+    np.testing.assert_array_equal(hs["wdflg"], ref["wdflg"])
+    for k in ("ze", "qx", "qy"):
+        rel = np.linalg.norm(hs[k] - ref[k]) / np.linalg.norm(ref[k])
+        assert rel <= 1e-10, (k, rel)
