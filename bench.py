@@ -68,10 +68,12 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=0, help="oracle sample steps (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--block", type=int, default=0,
+                    help="stage-kernel block size (0 = autotune on the warmed-up state)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the step shards by element partition, see DESIGN.md)", file=sys.stderr)
+              "(one GPU per run; a multi-GPU run would shard the mesh by element partition)", file=sys.stderr)
         return 2
 
     from dgswem_amd import gpu, mesh
@@ -85,6 +87,12 @@ def main() -> int:
     log("library initialised")
     g.step(args.warmup)
     g.synchronize()
+    if args.block:
+        g.set_block(args.block)
+        block = args.block
+    else:
+        block = g.autotune()
+    log(f"stage kernel block size {block} ({'set' if args.block else 'autotuned'})")
     g.profile(True)
     g.synchronize()
     t1 = time.perf_counter()
@@ -138,7 +146,8 @@ def main() -> int:
                                f"SSP(2,2), slope limiter 5, wetting/drying, 5 constituents",
                    "elements": ne, "nodes": case.np_, "p": args.p, "rk_stages": nrk,
                    "dt_s": case.params.dtdp, "limiter": case.dg.slopeflag,
-                   "wetdry": case.params.nolifa >= 2, "seed": args.seed, "state_finite": finite},
+                   "wetdry": case.params.nolifa >= 2, "seed": args.seed, "state_finite": finite,
+                   "block": block},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(args.p),
                      "kernel": name, "avg_launch_us": avg_s * 1e6,

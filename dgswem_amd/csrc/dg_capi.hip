@@ -1,5 +1,11 @@
-// C ABI of libdgswem_hip.so (declared in include/dgswem_gpu.h): device context management, the
-// per-step launch sequence and host<->device layout conversion.
+// C ABI of libdgswem_hip.so (declared in include/dgswem_gpu.h): device contexts, the per-step
+// launch sequence, ensembles of uneven meshes batched into one launch per stage, HIP-graph replay
+// and the block-size autotuner.
+//
+// A context is a batch of members (one member for a single run).  Members are concatenated
+// element-major: element e of member m is global element eoff[m] + e; neighbour, node and
+// boundary-list indices are offset the same way, and a per-element scenario id selects the
+// member's time step, H0, CF and forcing rows.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -7,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -36,11 +43,15 @@ int fail(const std::string& m) { g_err = m; return 1; }
 enum { K_STAGE = 0, K_MINMAX = 1, K_LIMIT = 2, K_M2N = 3, K_COUNT = 4 };
 const char* kKernelNames[K_COUNT] = {"stage_kernel", "node_minmax_kernel", "limit_wetdry_kernel",
                                      "modal2nodal_kernel"};
+constexpr int kTableChunk = 2048;   // steps of forcing rows per table upload
 
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0, cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { if (p) (void)hipFree(p); }
     hipError_t alloc(size_t count) {
         n = count;
@@ -56,40 +67,68 @@ struct DevBuf {
     }
 };
 
-struct Ctx {
+// Host-side data of one scenario (one mesh and its forcing), in local indices.
+struct Member {
     dg_params prm{};
-    int np = 0, ne = 0, dof = 0, nagp = 0, negp = 0;
+    int np = 0, ne = 0;
+    std::vector<double> xv, yv, dpv, nbar, cori, sfacq, sfaced;   // element SoA, [k][ne]
+    double cori_val = 0.0;                                        // uniform Coriolis
+    std::vector<int> sinfo, snbr, nm, n2e_off, n2e;
+    int needs = 0, nfeds = 0;
+    int nbfr = 0, nffr = 0;
+    std::vector<double> amig, ff, face, per, famig, fff, fface, fper;
+    std::vector<double> emo, efa, qnam, qnph;   // [edge][constituent][2]
+    std::vector<double> ze0;                    // [dof][ne] cold-start elevation modes
+    std::vector<uint8_t> wd0;                   // cold-start wet/dry flags
+};
+
+struct Ctx {
+    std::vector<Member> mem;
+    int S = 1, np = 0, ne = 0, dof = 0, nagp = 0, negp = 0;
+    std::vector<int> eoff, noff, ooff, foff;     // member offsets: elements, nodes, ocean, flux edges
+    int nbfr = 0, nffr = 0;                      // constituent counts, maxima over members
+    dg_params prm{};                             // switches shared by all members
     RefTables T;
     RKScheme rk;
     Phys phys{};
     hipStream_t stream = nullptr;
-    // state
+    // state: NRK+1 rotating stage slots, kept RHS of earlier stages, double-buffered wet/dry flags
     DevBuf<double> u[KMAXRK + 1];
-    int ucur = 0;                      // slot holding the stage-1 state
+    int ucur = 0;                                // slot holding the stage-1 state
     DevBuf<double> r[KMAXRK];
     bool keep_r[KMAXRK] = {};
     DevBuf<uint8_t> wd[2];
     int wdcur = 0;
     // mesh
+    DevBuf<uint16_t> sid;
+    DevBuf<Scen> scen;
     DevBuf<double> xv, yv, dpv, nbar, cori_el, sfacq, sfaced;
     DevBuf<int> sinfo, snbr, nm, n2e_off, n2e;
-    DevBuf<double> nmm;
-    DevBuf<double> outbuf;
+    bool has_cori_el = false;
+    DevBuf<double> nmm, outbuf, partial;
     DevBuf<int> nodecode;
-    DevBuf<double> partial;
     // forcing
-    int needs = 0, nfeds = 0, nbfr = 0, nffr = 0;
-    std::vector<double> amig, ff, face, per, famig, fff, fface, fper;
-    DevBuf<double> emo, efa, qnam, qnph;
-    DevBuf<double> argj, rff, fargj, frff;
-    std::vector<double> h_argj, h_rff, h_fargj, h_frff;
+    DevBuf<double> emo, efa, qnam, qnph, argj, rff, fargj, frff;
+    DevBuf<int> step;                            // device step counter within the current tables
+    int rows = 0;                                // table rows per scenario
+    // launch configuration
+    int block = 256;
+    // graphs of one slot-rotation period, keyed by (ucur, wdcur) at capture
+    bool use_graph = false;
+    std::map<int, hipGraphExec_t> graphs;
     // profiling
     bool prof = false;
     double ktime[K_COUNT] = {};
     long long kcount[K_COUNT] = {};
     std::vector<std::pair<hipEvent_t, hipEvent_t>> evpool;
-    std::vector<std::pair<int, int>> pending;   // (kernel id, event pair index)
+    std::vector<std::pair<int, int>> pending;    // (kernel id, event pair index)
+    void drop_graphs() {
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+        graphs.clear();
+    }
     ~Ctx() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        drop_graphs();
         for (auto& e : evpool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -121,9 +160,9 @@ DevTables to_dev(const RefTables& t) {
     return d;
 }
 
-// ---- profiling helpers
-int prof_begin(Ctx& c, int kid) {
-    if (!c.prof) return -1;
+// ---- profiling helpers (no events are recorded into a captured graph)
+int prof_begin(Ctx& c, int kid, bool capturing) {
+    if (!c.prof || capturing) return -1;
     const int idx = (int)c.pending.size();
     if (idx >= (int)c.evpool.size()) {
         hipEvent_t a, b;
@@ -153,132 +192,583 @@ int prof_flush(Ctx& c) {
 ElemData elem_data(Ctx& c) {
     ElemData E{};
     E.ne = c.ne;
+    E.sid = c.S > 1 ? c.sid.p : nullptr;
+    E.scen = c.scen.p;
     E.xv = c.xv.p; E.yv = c.yv.p; E.dpv = c.dpv.p;
-    E.nbar = c.prm.load_manning ? c.nbar.p : nullptr;
-    E.cori_el = c.cori_el.n ? c.cori_el.p : nullptr;
+    E.nbar = c.phys.load_manning ? c.nbar.p : nullptr;
+    E.cori_el = c.has_cori_el ? c.cori_el.p : nullptr;
     E.sfacq = c.prm.ics == 2 ? c.sfacq.p : nullptr;
     E.sfaced = c.prm.ics == 2 ? c.sfaced.p : nullptr;
     E.sinfo = c.sinfo.p; E.snbr = c.snbr.p; E.nm = c.nm.p;
     return E;
 }
 
-// Per-stage boundary forcing scalars for steps it..it+n-1 (host arithmetic of
-// src/DG_hydro_timestep.F:79-100, src/ocean_edge_hydro.F:59-75, src/flow_edge_hydro.F:45-62).
-int build_stage_tables(Ctx& c, int it, int n) {
-    const int ns = n * c.rk.nrk;
-    c.h_argj.assign((size_t)ns * std::max(c.nbfr, 1), 0.0);
-    c.h_rff.assign((size_t)ns * std::max(c.nbfr, 1), 0.0);
-    c.h_fargj.assign((size_t)ns * std::max(c.nffr, 1), 0.0);
-    c.h_frff.assign((size_t)ns * std::max(c.nffr, 1), 0.0);
-    for (int s = 0; s < n; s++) {
-        const int IT = it + s;
-        const double itdt = IT * c.prm.dtdp;
-        const double time_a = itdt + c.prm.statim * 86400.0;
-        for (int irk = 0; irk < c.rk.nrk; irk++) {
-            const int row = s * c.rk.nrk + irk;
-            const double timedg = time_a - c.prm.dtdp + c.rk.dtvd[irk] * c.prm.dtdp;
-            double ramp = 1.0;
-            if (c.prm.nramp == 1) ramp = std::tanh((2.0 * ((IT - 1) + c.rk.dtvd[irk]) * c.prm.dtdp / 86400.0) / c.prm.dramp);
-            for (int jj = 0; jj < c.nbfr; jj++) {
-                const int ncyc = (c.per[jj] == 0.0) ? 0 : (int)(timedg / c.per[jj]);
-                c.h_argj[(size_t)row * c.nbfr + jj] = c.amig[jj] * (timedg - ncyc * c.per[jj]) + c.face[jj];
-                c.h_rff[(size_t)row * c.nbfr + jj] = c.ff[jj] * ramp;
+Forcing forcing(Ctx& c) {
+    Forcing F{};
+    F.nbfr = c.nbfr; F.nffr = c.nffr; F.rows = c.rows; F.nrk = c.rk.nrk;
+    F.step = c.step.p;
+    F.emo = c.emo.p; F.efa = c.efa.p; F.qnam = c.qnam.p; F.qnph = c.qnph.p;
+    F.argj = c.argj.p; F.rff = c.rff.p; F.fargj = c.fargj.p; F.frff = c.frff.p;
+    return F;
+}
+
+// --------------------------------------------------------------------------------------------
+// Member construction on the host: side slots in the reference's accumulation order, element
+// SoA arrays, the node->element CSR and the cold start.
+// --------------------------------------------------------------------------------------------
+int build_member(Member& m, const dg_params* prm, const RefTables& T, int np, int ne, const double* x,
+                 const double* y, const double* dp, const double* corif, const double* manning,
+                 const int* nm, int nedges, const int* nedel, const int* nedsd, int nieds,
+                 const int* niedn, int nleds, const int* nledn, int needs, const int* needn,
+                 int nfeds, const int* nfedn) {
+    m.prm = *prm;
+    m.np = np; m.ne = ne;
+    const int nagp = T.nagp, negp = T.negp, dof = T.dof;
+    std::vector<int> kind_of(nedges, -1), list_pos(nedges, 0);
+    auto mark = [&](int n, const int* lst, int kind) -> int {
+        for (int i = 0; i < n; i++) {
+            const int ed = lst[i] - 1;
+            if (ed < 0 || ed >= nedges) return fail("edge list refers to an edge out of range");
+            kind_of[ed] = kind; list_pos[ed] = i;
+        }
+        return 0;
+    };
+    if (mark(nleds, nledn, SIDE_LAND) || mark(nfeds, nfedn, SIDE_FLOW) || mark(needs, needn, SIDE_OCEAN) ||
+        mark(nieds, niedn, SIDE_INTERNAL))
+        return 1;
+    m.sinfo.assign(ne, 0);
+    m.snbr.assign((size_t)3 * ne, -1);
+    {
+        // per element: its three edges with (kind order, position) sort keys
+        std::vector<std::array<long long, 3>> keys(ne);
+        std::vector<std::array<int, 3>> bitsv(ne), nbv(ne);
+        std::vector<int> cnt(ne, 0);
+        for (int ed = 0; ed < nedges; ed++) {
+            for (int s = 0; s < 2; s++) {
+                const int el = nedel[2 * ed + s] - 1;
+                if (el < 0) continue;
+                const int L = nedsd[2 * ed + s] - 1;
+                if (el >= ne || L < 0 || L > 2) return fail("corrupt edge tables");
+                const int k = kind_of[ed];
+                if (k < 0) return fail("edge " + std::to_string(ed + 1) + " is in no edge list");
+                int bits = L | (s << 2) | (k << 3);
+                int nb = -1;
+                if (k == SIDE_INTERNAL) {
+                    const int other = nedel[2 * ed + (1 - s)] - 1;
+                    const int oled = nedsd[2 * ed + (1 - s)] - 1;
+                    if (other < 0 || other >= ne || oled < 0 || oled > 2)
+                        return fail("internal edge without a valid second element");
+                    bits |= oled << 5;
+                    nb = other;
+                } else {
+                    if (s != 0) return fail("boundary edge on the second side");
+                    nb = list_pos[ed];
+                }
+                // boundary sides first in the order ocean, land, flux and their list order;
+                // internal sides after, by global edge number
+                const int korder = (k == SIDE_OCEAN) ? 0 : (k == SIDE_LAND) ? 1 : (k == SIDE_FLOW) ? 2 : 3;
+                const long long key = (long long)korder * (1LL << 40) + (k == SIDE_INTERNAL ? ed : list_pos[ed]);
+                const int j = cnt[el]++;
+                if (j >= 3) return fail("element with more than three edges");
+                keys[el][j] = key; bitsv[el][j] = bits; nbv[el][j] = nb;
             }
-            for (int jj = 0; jj < c.nffr; jj++) {
-                const int ncyc = (c.fper[jj] == 0.0) ? 0 : (int)(timedg / c.fper[jj]);
-                c.h_fargj[(size_t)row * c.nffr + jj] = c.famig[jj] * (timedg - ncyc * c.fper[jj]) + c.fface[jj];
-                c.h_frff[(size_t)row * c.nffr + jj] = c.fff[jj] * ramp;
+        }
+        for (int e = 0; e < ne; e++) {
+            if (cnt[e] != 3) return fail("element " + std::to_string(e + 1) + " does not have three edges");
+            int ord[3] = {0, 1, 2};
+            std::sort(ord, ord + 3, [&](int a, int b) { return keys[e][a] < keys[e][b]; });
+            int info = 0;
+            for (int j = 0; j < 3; j++) {
+                info |= bitsv[e][ord[j]] << (7 * j);
+                m.snbr[(size_t)j * ne + e] = nbv[e][ord[j]];
+            }
+            m.sinfo[e] = info;
+        }
+    }
+    m.needs = needs; m.nfeds = nfeds;
+    // ---- element SoA arrays (src/prep_DG.F:982-983 nodal means)
+    m.xv.resize((size_t)3 * ne); m.yv.resize((size_t)3 * ne); m.dpv.resize((size_t)3 * ne);
+    m.nm.resize((size_t)3 * ne);
+    const bool man = prm->load_manning && manning;
+    if (man) m.nbar.resize(ne);
+    bool cori_uniform = true;
+    for (int i = 1; i < np; i++) if (corif[i] != corif[0]) { cori_uniform = false; break; }
+    if (!cori_uniform) m.cori.resize(ne);
+    for (int e = 0; e < ne; e++) {
+        int n[3];
+        for (int v = 0; v < 3; v++) {
+            n[v] = nm[(size_t)v * ne + e] - 1;
+            if (n[v] < 0 || n[v] >= np) return fail("NM refers to a node out of range");
+            m.xv[(size_t)v * ne + e] = x[n[v]];
+            m.yv[(size_t)v * ne + e] = y[n[v]];
+            m.dpv[(size_t)v * ne + e] = dp[n[v]];
+            m.nm[(size_t)v * ne + e] = n[v];
+        }
+        const double AREAS = (x[n[0]] - x[n[2]]) * (y[n[1]] - y[n[2]]) + (x[n[2]] - x[n[1]]) * (y[n[0]] - y[n[2]]);
+        if (!(AREAS > 0.0)) return fail("element " + std::to_string(e + 1) + " is not counter-clockwise");
+        if (man) m.nbar[e] = (manning[n[0]] + manning[n[1]] + manning[n[2]]) / 3.0;
+        if (!cori_uniform) m.cori[e] = (corif[n[0]] + corif[n[1]] + corif[n[2]]) / 3.0;
+    }
+    m.cori_val = (corif[0] + corif[0] + corif[0]) / 3.0;
+    // spherical factors at the quadrature points (src/prep_DG.F:1010-1016,1085-1091)
+    if (prm->ics == 2) {
+        const double R = 6378206.4;
+        m.sfacq.resize((size_t)nagp * ne);
+        m.sfaced.resize((size_t)3 * negp * ne);
+        for (int e = 0; e < ne; e++) {
+            const double Y1 = m.yv[e], Y2 = m.yv[(size_t)ne + e], Y3 = m.yv[2 * (size_t)ne + e];
+            const double yd[3] = {1.0 / 3.0 * (Y1 + Y2 + Y3), -(1.0 / 6.0 * (Y1 + Y2)) + 1.0 / 3.0 * Y3,
+                                  -(0.5 * Y1) + 0.5 * Y2};
+            for (int I = 0; I < nagp; I++) {
+                double ye = 0.0;
+                for (int k = 0; k < 3; k++) ye = ye + yd[k] * T.phi_area[I][k];
+                m.sfacq[(size_t)I * ne + e] = std::cos(prm->sfea0) / std::cos(ye / R);
+            }
+            for (int L = 0; L < 3; L++)
+                for (int I = 0; I < negp; I++) {
+                    double ye = 0.0;
+                    for (int k = 0; k < 3; k++) ye = ye + yd[k] * T.phi_edge[L][I][k];
+                    m.sfaced[((size_t)L * negp + I) * ne + e] = std::cos(prm->sfea0) / std::cos(ye / R);
+                }
+        }
+    }
+    // ---- node -> element CSR in element order (entries: element*4 + local vertex)
+    m.n2e_off.assign(np + 1, 0);
+    m.n2e.resize((size_t)3 * ne);
+    for (int e = 0; e < ne; e++) for (int v = 0; v < 3; v++) m.n2e_off[m.nm[(size_t)v * ne + e] + 1]++;
+    for (int i = 0; i < np; i++) m.n2e_off[i + 1] += m.n2e_off[i];
+    {
+        std::vector<int> fill(np, 0);
+        for (int e = 0; e < ne; e++)
+            for (int v = 0; v < 3; v++) {
+                const int n = m.nm[(size_t)v * ne + e];
+                m.n2e[m.n2e_off[n] + fill[n]++] = (e << 2) | v;
+            }
+    }
+    // ---- cold start: ZE=QX=QY=0 except the initial dry check (src/prep_DG.F:1214-1300)
+    m.ze0.assign((size_t)dof * ne, 0.0);
+    m.wd0.assign(ne, 1);
+    if (prm->nolifa >= 2) {
+        const double H0 = prm->h0;
+        for (int e = 0; e < ne; e++) {
+            const double d[3] = {m.dpv[e], m.dpv[(size_t)ne + e], m.dpv[2 * (size_t)ne + e]};
+            double z[3] = {0.0, 0.0, 0.0};
+            for (int v = 0; v < 3; v++) if (d[v] < H0) z[v] = std::max(z[v], H0 - d[v]);
+            if ((z[0] + z[1] + z[2]) / 3.0 != 0.0) {
+                if (!(0.0 > (z[0] + z[1] + z[2]) / 3.0)) {
+                    m.ze0[(size_t)0 * ne + e] = (z[0] + z[1] + z[2]) / 3.0;
+                    m.ze0[(size_t)1 * ne + e] = -(1.0 / 6.0 * (z[0] + z[1])) + 1.0 / 3.0 * z[2];
+                    m.ze0[(size_t)2 * ne + e] = -(0.5 * z[0]) + 0.5 * z[1];
+                }
+                m.wd0[e] = 0;
             }
         }
     }
-    HIPCHK(c.argj.upload(c.h_argj.data(), c.h_argj.size(), c.stream));
-    HIPCHK(c.rff.upload(c.h_rff.data(), c.h_rff.size(), c.stream));
-    HIPCHK(c.fargj.upload(c.h_fargj.data(), c.h_fargj.size(), c.stream));
-    HIPCHK(c.frff.upload(c.h_frff.data(), c.h_frff.size(), c.stream));
     return 0;
 }
 
+// Per-scenario scalars (src/dgswem.F:131-170, src/rhs_dg_hydro.F:46, src/prep_DG.F:982-983).
+Scen scen_of(const Member& m) {
+    Scen s{};
+    s.dtdp = m.prm.dtdp;
+    s.dtdph = 1.0 / m.prm.dtdp;
+    s.h0 = m.prm.h0;
+    s.cf = m.prm.cf;
+    s.fric = (m.prm.cf + m.prm.cf + m.prm.cf) / 3.0;
+    s.cori = m.cori_val;
+    s.rain = m.prm.rainfall == 1 ? 7.0556e-6 : 0.0;
+    return s;
+}
+
+// Boundary forcing coefficients of all members, padded to the largest constituent counts, and
+// the per-stage tables, allocated once per context so that pointers captured in a graph stay valid.
+int upload_forcing(Ctx& c) {
+    c.nbfr = 0; c.nffr = 0;
+    for (auto& m : c.mem) { c.nbfr = std::max(c.nbfr, m.nbfr); c.nffr = std::max(c.nffr, m.nffr); }
+    const int tot_o = c.ooff[c.S], tot_f = c.foff[c.S];
+    std::vector<double> em((size_t)tot_o * std::max(c.nbfr, 1) * 2, 0.0), ef(em.size(), 0.0);
+    std::vector<double> qa((size_t)tot_f * std::max(c.nffr, 1) * 2, 0.0), qp(qa.size(), 0.0);
+    for (int s = 0; s < c.S; s++) {
+        const Member& m = c.mem[s];
+        for (int L = 0; L < m.needs; L++)
+            for (int j = 0; j < m.nbfr; j++)
+                for (int k = 0; k < 2; k++) {
+                    const size_t d = ((size_t)(c.ooff[s] + L) * c.nbfr + j) * 2 + k, o = ((size_t)L * m.nbfr + j) * 2 + k;
+                    em[d] = m.emo[o];
+                    ef[d] = m.efa[o];
+                }
+        for (int L = 0; L < m.nfeds; L++)
+            for (int j = 0; j < m.nffr; j++)
+                for (int k = 0; k < 2; k++) {
+                    const size_t d = ((size_t)(c.foff[s] + L) * c.nffr + j) * 2 + k, o = ((size_t)L * m.nffr + j) * 2 + k;
+                    qa[d] = m.qnam[o];
+                    qp[d] = m.qnph[o];
+                }
+    }
+    c.drop_graphs();
+    HIPCHK(hipStreamSynchronize(c.stream));
+    HIPCHK(c.emo.upload(em.data(), em.size(), c.stream));
+    HIPCHK(c.efa.upload(ef.data(), ef.size(), c.stream));
+    HIPCHK(c.qnam.upload(qa.data(), qa.size(), c.stream));
+    HIPCHK(c.qnph.upload(qp.data(), qp.size(), c.stream));
+    const size_t rows = (size_t)kTableChunk * c.rk.nrk;
+    HIPCHK(c.argj.alloc((size_t)c.S * rows * std::max(c.nbfr, 1)));
+    HIPCHK(c.rff.alloc((size_t)c.S * rows * std::max(c.nbfr, 1)));
+    HIPCHK(c.fargj.alloc((size_t)c.S * rows * std::max(c.nffr, 1)));
+    HIPCHK(c.frff.alloc((size_t)c.S * rows * std::max(c.nffr, 1)));
+    c.rows = (int)rows;
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return 0;
+}
+
+// Concatenate the members and upload everything except the state.
+int upload_mesh(Ctx& c) {
+    const int S = c.S;
+    c.eoff.assign(S + 1, 0); c.noff.assign(S + 1, 0); c.ooff.assign(S + 1, 0); c.foff.assign(S + 1, 0);
+    for (int s = 0; s < S; s++) {
+        c.eoff[s + 1] = c.eoff[s] + c.mem[s].ne;
+        c.noff[s + 1] = c.noff[s] + c.mem[s].np;
+        c.ooff[s + 1] = c.ooff[s] + c.mem[s].needs;
+        c.foff[s + 1] = c.foff[s] + c.mem[s].nfeds;
+    }
+    if ((long long)c.eoff[S] * 4 >= (1LL << 31) || (long long)c.noff[S] >= (1LL << 31))
+        return fail("batch too large for 32-bit element indices");
+    const int ne = c.ne = c.eoff[S], np = c.np = c.noff[S];
+    const int nagp = c.nagp, negp = c.negp;
+    std::vector<double> xv((size_t)3 * ne), yv((size_t)3 * ne), dpv((size_t)3 * ne);
+    std::vector<int> sinfo(ne), snbr((size_t)3 * ne), nm((size_t)3 * ne), off(np + 1, 0), n2e((size_t)3 * ne);
+    std::vector<uint16_t> sid(ne);
+    std::vector<Scen> scen(S);
+    std::vector<double> nbar, cori, sfq, sfe;
+    c.has_cori_el = false;
+    for (auto& m : c.mem) if (!m.cori.empty()) c.has_cori_el = true;
+    if (c.phys.load_manning) nbar.resize(ne);
+    if (c.has_cori_el) cori.resize(ne);
+    if (c.prm.ics == 2) { sfq.resize((size_t)nagp * ne); sfe.resize((size_t)3 * negp * ne); }
+    for (int s = 0; s < S; s++) {
+        const Member& m = c.mem[s];
+        const int e0 = c.eoff[s], n0 = c.noff[s], me = m.ne;
+        scen[s] = scen_of(m);
+        for (int e = 0; e < me; e++) {
+            const int g = e0 + e;
+            sid[g] = (uint16_t)s;
+            sinfo[g] = m.sinfo[e];
+            for (int v = 0; v < 3; v++) {
+                xv[(size_t)v * ne + g] = m.xv[(size_t)v * me + e];
+                yv[(size_t)v * ne + g] = m.yv[(size_t)v * me + e];
+                dpv[(size_t)v * ne + g] = m.dpv[(size_t)v * me + e];
+                nm[(size_t)v * ne + g] = m.nm[(size_t)v * me + e] + n0;
+                const int kind = (side_bits(m.sinfo[e], v) >> 3) & 3;
+                const int nb = m.snbr[(size_t)v * me + e];
+                const int shift = kind == SIDE_INTERNAL ? e0 : kind == SIDE_OCEAN ? c.ooff[s] : kind == SIDE_FLOW ? c.foff[s] : 0;
+                snbr[(size_t)v * ne + g] = nb + shift;
+            }
+            if (c.phys.load_manning) nbar[g] = m.nbar[e];
+            if (c.has_cori_el) cori[g] = m.cori.empty() ? m.cori_val : m.cori[e];
+            if (c.prm.ics == 2) {
+                for (int I = 0; I < nagp; I++) sfq[(size_t)I * ne + g] = m.sfacq[(size_t)I * me + e];
+                for (int j = 0; j < 3 * negp; j++) sfe[(size_t)j * ne + g] = m.sfaced[(size_t)j * me + e];
+            }
+        }
+        for (int i = 0; i < m.np; i++) off[n0 + i + 1] = m.n2e_off[i + 1] - m.n2e_off[i];
+        for (size_t j = 0; j < m.n2e.size(); j++) {
+            const int ent = m.n2e[j];
+            n2e[(size_t)3 * e0 + j] = (((ent >> 2) + e0) << 2) | (ent & 3);
+        }
+    }
+    for (int i = 0; i < np; i++) off[i + 1] += off[i];
+    HIPCHK(c.xv.upload(xv.data(), xv.size(), c.stream));
+    HIPCHK(c.yv.upload(yv.data(), yv.size(), c.stream));
+    HIPCHK(c.dpv.upload(dpv.data(), dpv.size(), c.stream));
+    HIPCHK(c.nm.upload(nm.data(), nm.size(), c.stream));
+    HIPCHK(c.sinfo.upload(sinfo.data(), sinfo.size(), c.stream));
+    HIPCHK(c.snbr.upload(snbr.data(), snbr.size(), c.stream));
+    HIPCHK(c.n2e_off.upload(off.data(), off.size(), c.stream));
+    HIPCHK(c.n2e.upload(n2e.data(), n2e.size(), c.stream));
+    HIPCHK(c.scen.upload(scen.data(), scen.size(), c.stream));
+    if (S > 1) HIPCHK(c.sid.upload(sid.data(), sid.size(), c.stream));
+    if (c.phys.load_manning) HIPCHK(c.nbar.upload(nbar.data(), nbar.size(), c.stream));
+    if (c.has_cori_el) HIPCHK(c.cori_el.upload(cori.data(), cori.size(), c.stream));
+    if (c.prm.ics == 2) {
+        HIPCHK(c.sfacq.upload(sfq.data(), sfq.size(), c.stream));
+        HIPCHK(c.sfaced.upload(sfe.data(), sfe.size(), c.stream));
+    }
+    HIPCHK(c.nmm.alloc((size_t)6 * np));
+    HIPCHK(c.outbuf.alloc((size_t)3 * np));
+    HIPCHK(c.nodecode.alloc(np));
+    HIPCHK(c.partial.alloc((ne + 255) / 256));
+    HIPCHK(c.step.alloc(1));
+    const int nrk = c.rk.nrk, dof = c.dof;
+    for (int i = 0; i <= nrk; i++) {
+        HIPCHK(c.u[i].alloc((size_t)3 * dof * ne));
+        HIPCHK(hipMemsetAsync(c.u[i].p, 0, (size_t)3 * dof * ne * sizeof(double), c.stream));
+    }
+    for (int i = 0; i < nrk; i++) {
+        c.keep_r[i] = false;
+        for (int s = i + 1; s < nrk; s++) if (c.rk.b[s][i] != 0.0) c.keep_r[i] = true;
+        if (c.keep_r[i]) HIPCHK(c.r[i].alloc((size_t)3 * dof * ne));
+    }
+    for (int i = 0; i < 2; i++) HIPCHK(c.wd[i].alloc(ne));
+    // the host vectors go out of scope: wait for the copies
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return upload_forcing(c);
+}
+
+// Reference tables, RK scheme, stream and the physics constants shared by all members
+// (src/dgswem.F:131-170).
+int init_ctx_common(Ctx& c, const dg_params* prm, bool manning) {
+    c.prm = *prm;
+    std::string err;
+    if (!build_ref_tables(prm->p, c.T, err)) return fail(err);
+    if (!build_rk(prm->rk_stage, prm->rk_order, c.rk, err)) return fail(err);
+    c.dof = c.T.dof; c.nagp = c.T.nagp; c.negp = c.T.negp;
+    c.block = prm->p == 1 ? 256 : 128;
+    HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (!g_tables_set[prm->p]) {
+            DevTables d = to_dev(c.T);
+            if (prm->p == 1) HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_tab1), &d, sizeof d));
+            else HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_tab2), &d, sizeof d));
+            g_tables_set[prm->p] = true;
+        }
+    }
+    Phys& P = c.phys;
+    P.g = prm->g;
+    P.nleq = prm->nolicat == 0 ? 0.0 : 1.0;
+    P.leq = prm->nolicat == 0 ? 1.0 : 0.0;
+    P.nleqg = P.nleq * P.g;
+    P.fg_l = P.leq * P.g;
+    P.hbreak = prm->hbreak; P.ftheta = prm->ftheta; P.fgamma = prm->fgamma;
+    P.ifnlfa = prm->nolifa == 0 ? 0 : 1;
+    P.iflinbf = prm->nolibf == 0 ? 1 : 0;
+    P.ifhybf = prm->nolibf == 2 ? 1 : 0;
+    P.load_manning = manning ? 1 : 0;
+    return 0;
+}
+
+// Members' cold-start state into the stage-1 slot.
+int upload_initial_state(Ctx& c) {
+    const int ne = c.ne, dof = c.dof;
+    std::vector<double> u((size_t)3 * dof * ne, 0.0);
+    std::vector<uint8_t> w(ne, 1);
+    for (int s = 0; s < c.S; s++) {
+        const Member& m = c.mem[s];
+        for (int k = 0; k < dof; k++)
+            for (int e = 0; e < m.ne; e++) u[(size_t)k * ne + c.eoff[s] + e] = m.ze0[(size_t)k * m.ne + e];
+        for (int e = 0; e < m.ne; e++) w[c.eoff[s] + e] = m.wd0[e];
+    }
+    c.ucur = 0; c.wdcur = 0;
+    HIPCHK(hipMemcpyAsync(c.u[0].p, u.data(), u.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    HIPCHK(hipMemcpyAsync(c.wd[0].p, w.data(), ne, hipMemcpyHostToDevice, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return 0;
+}
+
+// Per-stage forcing rows for steps it..it+n-1 of every member (host arithmetic of
+// src/DG_hydro_timestep.F:79-100, src/ocean_edge_hydro.F:59-75, src/flow_edge_hydro.F:45-62).
+int build_stage_tables(Ctx& c, int it, int n) {
+    if (c.nbfr == 0 && c.nffr == 0) return 0;
+    if (n > kTableChunk) return fail("internal: table chunk overflow");
+    const int nrk = c.rk.nrk, rows = c.rows;
+    std::vector<double> aj((size_t)c.S * rows * std::max(c.nbfr, 1), 0.0), rf(aj.size(), 0.0);
+    std::vector<double> fa((size_t)c.S * rows * std::max(c.nffr, 1), 0.0), fr(fa.size(), 0.0);
+    for (int sc = 0; sc < c.S; sc++) {
+        const Member& m = c.mem[sc];
+        for (int s = 0; s < n; s++) {
+            const int IT = it + s;
+            const double itdt = IT * m.prm.dtdp;
+            const double time_a = itdt + m.prm.statim * 86400.0;
+            for (int irk = 0; irk < nrk; irk++) {
+                const size_t row = (size_t)sc * rows + (size_t)s * nrk + irk;
+                const double timedg = time_a - m.prm.dtdp + c.rk.dtvd[irk] * m.prm.dtdp;
+                double ramp = 1.0;
+                if (m.prm.nramp == 1)
+                    ramp = std::tanh((2.0 * ((IT - 1) + c.rk.dtvd[irk]) * m.prm.dtdp / 86400.0) / m.prm.dramp);
+                for (int jj = 0; jj < m.nbfr; jj++) {
+                    const int ncyc = (m.per[jj] == 0.0) ? 0 : (int)(timedg / m.per[jj]);
+                    aj[row * c.nbfr + jj] = m.amig[jj] * (timedg - ncyc * m.per[jj]) + m.face[jj];
+                    rf[row * c.nbfr + jj] = m.ff[jj] * ramp;
+                }
+                for (int jj = 0; jj < m.nffr; jj++) {
+                    const int ncyc = (m.fper[jj] == 0.0) ? 0 : (int)(timedg / m.fper[jj]);
+                    fa[row * c.nffr + jj] = m.famig[jj] * (timedg - ncyc * m.fper[jj]) + m.fface[jj];
+                    fr[row * c.nffr + jj] = m.fff[jj] * ramp;
+                }
+            }
+        }
+    }
+    // the previous chunk's launches may still read the tables
+    HIPCHK(hipStreamSynchronize(c.stream));
+    if (c.nbfr) {
+        HIPCHK(hipMemcpyAsync(c.argj.p, aj.data(), aj.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        HIPCHK(hipMemcpyAsync(c.rff.p, rf.data(), rf.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    }
+    if (c.nffr) {
+        HIPCHK(hipMemcpyAsync(c.fargj.p, fa.data(), fa.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        HIPCHK(hipMemcpyAsync(c.frff.p, fr.data(), fr.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    }
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return 0;
+}
+
+template <int P, int NT>
+void launch_stage(Ctx& c, const StageArgs& a, bool sph, bool fuse) {
+    const int grid = (c.ne + NT - 1) / NT;
+    if (sph) {
+        if (fuse) hipLaunchKernelGGL((stage_kernel<P, NT, true, true>), dim3(grid), dim3(NT), 0, c.stream, a);
+        else hipLaunchKernelGGL((stage_kernel<P, NT, true, false>), dim3(grid), dim3(NT), 0, c.stream, a);
+    } else {
+        if (fuse) hipLaunchKernelGGL((stage_kernel<P, NT, false, true>), dim3(grid), dim3(NT), 0, c.stream, a);
+        else hipLaunchKernelGGL((stage_kernel<P, NT, false, false>), dim3(grid), dim3(NT), 0, c.stream, a);
+    }
+}
+
+// Block sizes: p=1 64/128/256 threads; p=2 64/128 (its LDS accumulators double per thread).
 template <int P>
-int launch_step(Ctx& c, int row0) {
+void launch_stage_nt(Ctx& c, const StageArgs& a, bool sph, bool fuse, int nt) {
+    if (nt == 64) {
+        launch_stage<P, 64>(c, a, sph, fuse);
+    } else if constexpr (P == 1) {
+        if (nt == 128) launch_stage<P, 128>(c, a, sph, fuse);
+        else launch_stage<P, 256>(c, a, sph, fuse);
+    } else {
+        launch_stage<P, 128>(c, a, sph, fuse);
+    }
+}
+
+StageArgs stage_args(Ctx& c, int irk, const int slot[]) {
+    StageArgs a{};
+    a.ed = elem_data(c); a.ph = c.phys; a.fo = forcing(c);
+    a.uin = c.u[slot[irk]].p;
+    a.uout = c.u[slot[irk + 1]].p;
+    a.wd_in = c.wd[c.wdcur].p;
+    a.wd_out = c.wd[1 - c.wdcur].p;
+    a.nprev = irk;
+    for (int i = 0; i < irk; i++) {
+        a.uprev[i] = c.u[slot[i]].p; a.aprev[i] = c.rk.a[irk][i];
+        a.rprev[i] = c.r[i].p; a.bprev[i] = c.rk.b[irk][i];
+    }
+    a.acur = c.rk.a[irk][irk];
+    a.bcur = c.rk.b[irk][irk];
+    a.rout = c.keep_r[irk] ? c.r[irk].p : nullptr;
+    a.irk = irk;
+    return a;
+}
+
+// One time step: NRK stages (stage kernel, then node min/max + limiter/wet-dry when the slope
+// limiter is on) and the device step-counter increment.
+template <int P>
+int launch_step(Ctx& c, bool capturing) {
     const int ne = c.ne, np = c.np;
-    const int B = 256, BS = StageBlock<P>::N;
-    const int ge = (ne + B - 1) / B, gn = (np + B - 1) / B, gs = (ne + BS - 1) / BS;
-    const bool wdon = c.prm.nolifa >= 2;
-    const bool lim = c.prm.slopeflag != 0;
-    const bool sph = c.prm.ics == 2;
-    ElemData E = elem_data(c);
-    Forcing F{};
-    F.nbfr = c.nbfr; F.nffr = c.nffr;
-    F.emo = c.emo.p; F.efa = c.efa.p; F.qnam = c.qnam.p; F.qnph = c.qnph.p;
-    F.argj = c.argj.p; F.rff = c.rff.p; F.fargj = c.fargj.p; F.frff = c.frff.p;
+    const int B = 256;
+    const int ge = (ne + B - 1) / B, gn = (np + B - 1) / B;
+    const bool wdon = c.prm.nolifa >= 2, lim = c.prm.slopeflag != 0, sph = c.prm.ics == 2;
     const int nrk = c.rk.nrk;
     // stage slots: slot(i) holds U(i+1); slot(0) = current stage-1 state
     int slot[KMAXRK + 1];
     for (int i = 0; i <= nrk; i++) slot[i] = (c.ucur + i) % (nrk + 1);
     for (int irk = 0; irk < nrk; irk++) {
-        StageArgs a{};
-        a.ed = E; a.ph = c.phys; a.fo = F;
-        a.uin = c.u[slot[irk]].p;
-        a.uout = c.u[slot[irk + 1]].p;
-        a.wd_in = c.wd[c.wdcur].p;
-        a.wd_out = c.wd[1 - c.wdcur].p;
-        a.nprev = irk;
-        for (int i = 0; i < irk; i++) {
-            a.uprev[i] = c.u[slot[i]].p; a.aprev[i] = c.rk.a[irk][i];
-            a.rprev[i] = c.r[i].p; a.bprev[i] = c.rk.b[irk][i] * c.prm.dtdp;
+        StageArgs a = stage_args(c, irk, slot);
+        for (int i = 0; i < irk; i++)
             if (a.bprev[i] != 0.0 && !c.keep_r[i]) return fail("internal: RHS of an earlier stage was not kept");
-        }
-        a.acur = c.rk.a[irk][irk];
-        a.bcur = c.rk.b[irk][irk] * c.prm.dtdp;
-        a.rout = c.keep_r[irk] ? c.r[irk].p : nullptr;
-        a.stage = row0 + irk;
-        a.rain = c.prm.rainfall == 1 ? 7.0556e-6 : 0.0;
         const bool fuse = wdon && !lim;
-        int ev = prof_begin(c, K_STAGE);
-        if (sph) {
-            if (fuse) hipLaunchKernelGGL((stage_kernel<P, true, true>), dim3(gs), dim3(BS), 0, c.stream, a);
-            else hipLaunchKernelGGL((stage_kernel<P, true, false>), dim3(gs), dim3(BS), 0, c.stream, a);
-        } else {
-            if (fuse) hipLaunchKernelGGL((stage_kernel<P, false, true>), dim3(gs), dim3(BS), 0, c.stream, a);
-            else hipLaunchKernelGGL((stage_kernel<P, false, false>), dim3(gs), dim3(BS), 0, c.stream, a);
-        }
+        int ev = prof_begin(c, K_STAGE, capturing);
+        launch_stage_nt<P>(c, a, sph, fuse, c.block);
         prof_end(c, ev);
         HIPCHK(hipGetLastError());
         if (fuse) c.wdcur = 1 - c.wdcur;
         if (lim) {
-            ev = prof_begin(c, K_MINMAX);
+            ev = prof_begin(c, K_MINMAX, capturing);
             hipLaunchKernelGGL((node_minmax_kernel<P>), dim3(gn), dim3(B), 0, c.stream, np, ne,
                                (const int*)c.n2e_off.p, (const int*)c.n2e.p, (const double*)a.uout, c.nmm.p);
             prof_end(c, ev);
             HIPCHK(hipGetLastError());
-            ev = prof_begin(c, K_LIMIT);
+            ev = prof_begin(c, K_LIMIT, capturing);
             if (wdon) {
-                hipLaunchKernelGGL((limit_wetdry_kernel<P, true, true>), dim3(ge), dim3(B), 0, c.stream, np, E,
-                                   a.uout, (const double*)c.nmm.p, (const uint8_t*)c.wd[c.wdcur].p,
-                                   c.wd[1 - c.wdcur].p, c.prm.h0);
+                hipLaunchKernelGGL((limit_wetdry_kernel<P, true, true>), dim3(ge), dim3(B), 0, c.stream, np, a.ed,
+                                   a.uout, (const double*)c.nmm.p, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p);
                 c.wdcur = 1 - c.wdcur;
             } else {
-                hipLaunchKernelGGL((limit_wetdry_kernel<P, true, false>), dim3(ge), dim3(B), 0, c.stream, np, E,
-                                   a.uout, (const double*)c.nmm.p, (const uint8_t*)c.wd[c.wdcur].p,
-                                   c.wd[1 - c.wdcur].p, c.prm.h0);
+                hipLaunchKernelGGL((limit_wetdry_kernel<P, true, false>), dim3(ge), dim3(B), 0, c.stream, np, a.ed,
+                                   a.uout, (const double*)c.nmm.p, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p);
             }
             prof_end(c, ev);
             HIPCHK(hipGetLastError());
         }
     }
+    hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(1), 0, c.stream, c.step.p, 0, 0);
+    HIPCHK(hipGetLastError());
     c.ucur = slot[nrk];
     return 0;
+}
+
+int launch_steps(Ctx& c, int n, bool capturing) {
+    for (int s = 0; s < n; s++) {
+        const int rc = c.prm.p == 1 ? launch_step<1>(c, capturing) : launch_step<2>(c, capturing);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+// The steps of one table chunk: replays of a captured graph of one slot-rotation period (NRK+1
+// steps, after which the stage slots and wet/dry buffers are back in place), then eager launches
+// for the remainder.
+int run_chunk(Ctx& c, int n) {
+    hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(1), 0, c.stream, c.step.p, 1, 0);
+    HIPCHK(hipGetLastError());
+    const int period = c.rk.nrk + 1;
+    int done = 0;
+    if (c.use_graph && n >= period) {
+        const int key = c.ucur * 2 + c.wdcur;
+        auto it = c.graphs.find(key);
+        if (it == c.graphs.end()) {
+            const int u0 = c.ucur, w0 = c.wdcur;
+            HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+            const int rc = launch_steps(c, period, true);
+            hipGraph_t g = nullptr;
+            const hipError_t ee = hipStreamEndCapture(c.stream, &g);
+            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+            if (ee != hipSuccess) return fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+            if (c.ucur != u0 || c.wdcur != w0) { (void)hipGraphDestroy(g); return fail("internal: graph period mismatch"); }
+            hipGraphExec_t ex = nullptr;
+            const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) return fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+            it = c.graphs.emplace(key, ex).first;
+        }
+        const int reps = n / period;
+        for (int r = 0; r < reps; r++) HIPCHK(hipGraphLaunch(it->second, c.stream));
+        done = reps * period;
+    }
+    return launch_steps(c, n - done, false);
 }
 
 int validate(const dg_params* p) {
     if (!p) return fail("null parameters");
     if (p->p < 1 || p->p > 2) return fail("p must be 1 or 2");
-    if (p->dtdp <= 0) return fail("dtdp must be positive");
+    if (!(p->dtdp > 0)) return fail("dtdp must be positive");
     if (p->ics != 1 && p->ics != 2) return fail("ics must be 1 or 2");
     if (p->nramp < 0 || p->nramp > 1) return fail("only NRAMP 0 or 1 is supported");
     if (p->rainfall < 0 || p->rainfall > 1) return fail("only rainfall 0 or 1 is supported");
     if (p->nolibf < 0 || p->nolibf > 2) return fail("nolibf must be 0, 1 or 2");
     return 0;
+}
+
+int register_ctx(std::unique_ptr<Ctx> c, int* handle) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_ctx.push_back(std::move(c));
+    *handle = (int)g_ctx.size() - 1;
+    return 0;
+}
+
+// EMO_DG(NBFR,NEEDS,2)-style column-major (constituent, edge, component) -> [edge][constituent][2]
+void unpack_cols(int nc, int nedge, const double* src, std::vector<double>& dst) {
+    dst.assign((size_t)nedge * nc * 2, 0.0);
+    for (int L = 0; L < nedge; L++)
+        for (int j = 0; j < nc; j++)
+            for (int k = 0; k < 2; k++)
+                dst[((size_t)L * nc + j) * 2 + k] = src[(size_t)j + (size_t)nc * (L + (size_t)nedge * k)];
 }
 
 }  // namespace
@@ -313,254 +803,51 @@ int dg_gpu_init(const dg_params* prm, int np, int ne, const double* x, const dou
                 const int* nedno, const int* nedel, const int* nedsd, int nieds, const int* niedn,
                 int nleds, const int* nledn, int needs, const int* needn, int nfeds, const int* nfedn,
                 int* handle) {
+    (void)sfac; (void)nedno;
     if (validate(prm)) return 1;
     if (np <= 0 || ne <= 0) return fail("empty mesh");
-    if ((long long)ne * 3 >= (1LL << 31)) return fail("mesh too large for 32-bit indices");
+    if ((long long)ne * 4 >= (1LL << 31)) return fail("mesh too large for 32-bit indices");
     auto c = std::make_unique<Ctx>();
-    c->prm = *prm;
-    c->np = np; c->ne = ne;
-    std::string err;
-    if (!build_ref_tables(prm->p, c->T, err)) return fail(err);
-    if (!build_rk(prm->rk_stage, prm->rk_order, c->rk, err)) return fail(err);
-    c->dof = c->T.dof; c->nagp = c->T.nagp; c->negp = c->T.negp;
-    const int dof = c->dof, nagp = c->nagp, negp = c->negp;
-    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    {
-        std::lock_guard<std::mutex> lk(g_mu);
-        if (!g_tables_set[prm->p]) {
-            DevTables d = to_dev(c->T);
-            if (prm->p == 1) HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_tab1), &d, sizeof d));
-            else HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_tab2), &d, sizeof d));
-            g_tables_set[prm->p] = true;
-        }
-    }
-    // physics constants (src/dgswem.F:131-170, src/rhs_dg_hydro.F:46, src/prep_DG.F:982-983)
-    Phys& P = c->phys;
-    P.g = prm->g;
-    P.nleq = prm->nolicat == 0 ? 0.0 : 1.0;
-    P.leq = prm->nolicat == 0 ? 1.0 : 0.0;
-    P.nleqg = P.nleq * P.g;
-    P.fg_l = P.leq * P.g;
-    P.cf = prm->cf; P.hbreak = prm->hbreak; P.ftheta = prm->ftheta; P.fgamma = prm->fgamma;
-    P.dtdph = 1.0 / prm->dtdp;
-    P.h0 = prm->h0;
-    P.ifnlfa = prm->nolifa == 0 ? 0 : 1;
-    P.iflinbf = prm->nolibf == 0 ? 1 : 0;
-    P.ifhybf = prm->nolibf == 2 ? 1 : 0;
-    P.load_manning = (prm->load_manning && manning) ? 1 : 0;
-    P.rainfall = prm->rainfall;
-    P.fric = (prm->cf + prm->cf + prm->cf) / 3.0;
-    // ---- edges: which slot each side goes to, in the reference's accumulation order
-    std::vector<int> elem_slots((size_t)ne * 3, -1);   // edge id per local edge
-    std::vector<int> kind_of(nedges, 0), list_pos(nedges, 0);
-    for (int i = 0; i < nleds; i++) { kind_of[nledn[i] - 1] = SIDE_LAND; list_pos[nledn[i] - 1] = i; }
-    for (int i = 0; i < nfeds; i++) { kind_of[nfedn[i] - 1] = SIDE_FLOW; list_pos[nfedn[i] - 1] = i; }
-    for (int i = 0; i < needs; i++) { kind_of[needn[i] - 1] = SIDE_OCEAN; list_pos[needn[i] - 1] = i; }
-    for (int i = 0; i < nieds; i++) { kind_of[niedn[i] - 1] = SIDE_INTERNAL; list_pos[niedn[i] - 1] = i; }
-    std::vector<int> sinfo(ne, 0), snbr((size_t)3 * ne, -1);
-    {
-        // per element: its three edges with (kind order, position) sort keys
-        std::vector<std::array<long long, 3>> keys(ne);
-        std::vector<std::array<int, 3>> bitsv(ne), nbv(ne);
-        std::vector<int> cnt(ne, 0);
-        for (int ed = 0; ed < nedges; ed++) {
-            for (int s = 0; s < 2; s++) {
-                const int el = nedel[2 * ed + s] - 1;
-                if (el < 0) continue;
-                const int L = nedsd[2 * ed + s] - 1;
-                if (el >= ne || L < 0 || L > 2) return fail("corrupt edge tables");
-                const int k = kind_of[ed];
-                int bits = L | (s << 2) | (k << 3);
-                int nb = -1;
-                if (k == SIDE_INTERNAL) {
-                    const int other = nedel[2 * ed + (1 - s)] - 1;
-                    const int oled = nedsd[2 * ed + (1 - s)] - 1;
-                    if (other < 0) return fail("internal edge without a second element");
-                    bits |= oled << 5;
-                    nb = other;
-                } else {
-                    if (s != 0) return fail("boundary edge on the second side");
-                    nb = list_pos[ed];
-                }
-                // boundary sides first in the order ocean, land, flux and their list order;
-                // internal sides after, by global edge number
-                const int korder = (k == SIDE_OCEAN) ? 0 : (k == SIDE_LAND) ? 1 : (k == SIDE_FLOW) ? 2 : 3;
-                const long long key = (long long)korder * (1LL << 40) + (k == SIDE_INTERNAL ? ed : list_pos[ed]);
-                const int j = cnt[el]++;
-                if (j >= 3) return fail("element with more than three edges");
-                keys[el][j] = key; bitsv[el][j] = bits; nbv[el][j] = nb;
-            }
-        }
-        for (int e = 0; e < ne; e++) {
-            if (cnt[e] != 3) return fail("element " + std::to_string(e + 1) + " does not have three edges");
-            int ord[3] = {0, 1, 2};
-            std::sort(ord, ord + 3, [&](int a, int b) { return keys[e][a] < keys[e][b]; });
-            int info = 0;
-            for (int j = 0; j < 3; j++) {
-                info |= bitsv[e][ord[j]] << (7 * j);
-                snbr[(size_t)j * ne + e] = nbv[e][ord[j]];
-            }
-            sinfo[e] = info;
-        }
-    }
-    c->needs = needs; c->nfeds = nfeds;
-    // ---- element SoA arrays
-    std::vector<double> xv((size_t)3 * ne), yv((size_t)3 * ne), dpv((size_t)3 * ne), nb((size_t)ne);
-    std::vector<int> nmv((size_t)3 * ne);
-    bool cori_uniform = true;
-    for (int i = 1; i < np; i++) if (corif[i] != corif[0]) { cori_uniform = false; break; }
-    std::vector<double> cori;
-    if (!cori_uniform) cori.resize(ne);
-    for (int e = 0; e < ne; e++) {
-        int n[3];
-        for (int v = 0; v < 3; v++) {
-            n[v] = nm[(size_t)v * ne + e] - 1;
-            if (n[v] < 0 || n[v] >= np) return fail("NM refers to a node out of range");
-            xv[(size_t)v * ne + e] = x[n[v]]; yv[(size_t)v * ne + e] = y[n[v]]; dpv[(size_t)v * ne + e] = dp[n[v]];
-            nmv[(size_t)v * ne + e] = n[v];
-        }
-        const double AREAS = (x[n[0]] - x[n[2]]) * (y[n[1]] - y[n[2]]) + (x[n[2]] - x[n[1]]) * (y[n[0]] - y[n[2]]);
-        if (!(AREAS > 0.0)) return fail("element " + std::to_string(e + 1) + " is not counter-clockwise");
-        if (P.load_manning) nb[e] = (manning[n[0]] + manning[n[1]] + manning[n[2]]) / 3.0;
-        if (!cori_uniform) cori[e] = (corif[n[0]] + corif[n[1]] + corif[n[2]]) / 3.0;
-    }
-    P.cori = (corif[0] + corif[0] + corif[0]) / 3.0;
-    HIPCHK(c->xv.upload(xv.data(), xv.size(), c->stream));
-    HIPCHK(c->yv.upload(yv.data(), yv.size(), c->stream));
-    HIPCHK(c->dpv.upload(dpv.data(), dpv.size(), c->stream));
-    HIPCHK(c->nm.upload(nmv.data(), nmv.size(), c->stream));
-    if (P.load_manning) HIPCHK(c->nbar.upload(nb.data(), nb.size(), c->stream));
-    if (!cori_uniform) HIPCHK(c->cori_el.upload(cori.data(), cori.size(), c->stream));
-    HIPCHK(c->sinfo.upload(sinfo.data(), sinfo.size(), c->stream));
-    HIPCHK(c->snbr.upload(snbr.data(), snbr.size(), c->stream));
-    // spherical factors at quadrature points (src/prep_DG.F:1010-1016,1085-1091)
-    if (prm->ics == 2) {
-        const double R = 6378206.4;
-        std::vector<double> sq((size_t)nagp * ne), se((size_t)3 * negp * ne);
-        for (int e = 0; e < ne; e++) {
-            const double Y1 = yv[e], Y2 = yv[(size_t)ne + e], Y3 = yv[2 * (size_t)ne + e];
-            const double yd[3] = {1.0 / 3.0 * (Y1 + Y2 + Y3), -(1.0 / 6.0 * (Y1 + Y2)) + 1.0 / 3.0 * Y3, -(0.5 * Y1) + 0.5 * Y2};
-            for (int I = 0; I < nagp; I++) {
-                double ye = 0.0;
-                for (int k = 0; k < 3; k++) ye = ye + yd[k] * c->T.phi_area[I][k];
-                sq[(size_t)I * ne + e] = std::cos(prm->sfea0) / std::cos(ye / R);
-            }
-            for (int L = 0; L < 3; L++)
-                for (int I = 0; I < negp; I++) {
-                    double ye = 0.0;
-                    for (int k = 0; k < 3; k++) ye = ye + yd[k] * c->T.phi_edge[L][I][k];
-                    se[((size_t)L * negp + I) * ne + e] = std::cos(prm->sfea0) / std::cos(ye / R);
-                }
-        }
-        HIPCHK(c->sfacq.upload(sq.data(), sq.size(), c->stream));
-        HIPCHK(c->sfaced.upload(se.data(), se.size(), c->stream));
-    }
-    (void)sfac;
-    // ---- node -> element (CSR, element index order) for the limiter and modal2nodal
-    {
-        std::vector<int> off(np + 1, 0), lst((size_t)3 * ne);
-        for (int e = 0; e < ne; e++) for (int v = 0; v < 3; v++) off[nmv[(size_t)v * ne + e] + 1]++;
-        for (int i = 0; i < np; i++) off[i + 1] += off[i];
-        std::vector<int> fill(np, 0);
-        for (int e = 0; e < ne; e++)
-            for (int v = 0; v < 3; v++) {
-                const int n = nmv[(size_t)v * ne + e];
-                lst[off[n] + fill[n]++] = (e << 2) | v;
-            }
-        HIPCHK(c->n2e_off.upload(off.data(), off.size(), c->stream));
-        HIPCHK(c->n2e.upload(lst.data(), lst.size(), c->stream));
-    }
-    HIPCHK(c->nmm.alloc((size_t)6 * np));
-    HIPCHK(c->outbuf.alloc((size_t)3 * np));
-    HIPCHK(c->nodecode.alloc(np));
-    HIPCHK(c->partial.alloc((ne + 255) / 256));
-    // ---- state slots and kept RHS
-    const int nrk = c->rk.nrk;
-    for (int i = 0; i <= nrk; i++) {
-        HIPCHK(c->u[i].alloc((size_t)3 * dof * ne));
-        HIPCHK(hipMemsetAsync(c->u[i].p, 0, (size_t)3 * dof * ne * sizeof(double), c->stream));
-    }
-    for (int i = 0; i < nrk; i++) {
-        c->keep_r[i] = false;
-        for (int s = i + 1; s < nrk; s++) if (c->rk.b[s][i] != 0.0) c->keep_r[i] = true;
-        if (c->keep_r[i]) HIPCHK(c->r[i].alloc((size_t)3 * dof * ne));
-    }
-    for (int i = 0; i < 2; i++) HIPCHK(c->wd[i].alloc(ne));
-    // cold start: ZE=QX=QY=0 except the initial dry check (src/prep_DG.F:1214-1300)
-    {
-        std::vector<double> ze((size_t)dof * ne, 0.0);
-        std::vector<uint8_t> wdf(ne, 1);
-        if (prm->nolifa >= 2) {
-            const double H0 = prm->h0;
-            for (int e = 0; e < ne; e++) {
-                const double d[3] = {dpv[e], dpv[(size_t)ne + e], dpv[2 * (size_t)ne + e]};
-                double z[3] = {0.0, 0.0, 0.0};
-                for (int v = 0; v < 3; v++) if (d[v] < H0) z[v] = std::max(z[v], H0 - d[v]);
-                if ((z[0] + z[1] + z[2]) / 3.0 != 0.0) {
-                    if (!(0.0 > (z[0] + z[1] + z[2]) / 3.0)) {
-                        ze[(size_t)0 * ne + e] = (z[0] + z[1] + z[2]) / 3.0;
-                        ze[(size_t)1 * ne + e] = -(1.0 / 6.0 * (z[0] + z[1])) + 1.0 / 3.0 * z[2];
-                        ze[(size_t)2 * ne + e] = -(0.5 * z[0]) + 0.5 * z[1];
-                    }
-                    wdf[e] = 0;
-                }
-            }
-        }
-        HIPCHK(hipMemcpyAsync(c->u[0].p, ze.data(), ze.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(c->wd[0].p, wdf.data(), ne, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-    }
-    c->ucur = 0; c->wdcur = 0;
-    std::lock_guard<std::mutex> lk(g_mu);
-    g_ctx.push_back(std::move(c));
-    *handle = (int)g_ctx.size() - 1;
-    return 0;
+    if (init_ctx_common(*c, prm, prm->load_manning && manning)) return 1;
+    c->S = 1;
+    c->mem.resize(1);
+    if (build_member(c->mem[0], prm, c->T, np, ne, x, y, dp, corif, manning, nm, nedges, nedel, nedsd,
+                     nieds, niedn, nleds, nledn, needs, needn, nfeds, nfedn))
+        return 1;
+    if (upload_mesh(*c) || upload_initial_state(*c)) return 1;
+    return register_ctx(std::move(c), handle);
 }
 
 int dg_gpu_set_tide(int h, int nbfr, const double* amig, const double* ff, const double* face,
                     const double* per, const double* emo_dg, const double* efa_dg) {
     Ctx* c = get(h);
     if (!c) return 1;
+    if (c->S != 1) return fail("the forcing of an ensemble is set on its members before dg_ens_create");
+    Member& m = c->mem[0];
     if (nbfr < 0) return fail("nbfr < 0");
-    if (c->needs > 0 && nbfr == 0) return fail("non-periodic elevation boundary forcing is not supported");
-    c->nbfr = nbfr;
-    c->amig.assign(amig, amig + nbfr); c->ff.assign(ff, ff + nbfr);
-    c->face.assign(face, face + nbfr); c->per.assign(per, per + nbfr);
-    // EMO_DG(NBFR,NEEDS,2) column-major -> [needs][nbfr][2]
-    std::vector<double> em((size_t)c->needs * nbfr * 2), ef(em.size());
-    for (int L = 0; L < c->needs; L++)
-        for (int j = 0; j < nbfr; j++)
-            for (int k = 0; k < 2; k++) {
-                const size_t src = (size_t)j + (size_t)nbfr * (L + (size_t)c->needs * k);
-                em[((size_t)L * nbfr + j) * 2 + k] = emo_dg[src];
-                ef[((size_t)L * nbfr + j) * 2 + k] = efa_dg[src];
-            }
-    HIPCHK(c->emo.upload(em.data(), em.size(), c->stream));
-    HIPCHK(c->efa.upload(ef.data(), ef.size(), c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    return 0;
+    if (m.needs > 0 && nbfr == 0) return fail("non-periodic elevation boundary forcing is not supported");
+    m.nbfr = nbfr;
+    m.amig.assign(amig, amig + nbfr); m.ff.assign(ff, ff + nbfr);
+    m.face.assign(face, face + nbfr); m.per.assign(per, per + nbfr);
+    unpack_cols(nbfr, m.needs, emo_dg, m.emo);
+    unpack_cols(nbfr, m.needs, efa_dg, m.efa);
+    return upload_forcing(*c);
 }
 
 int dg_gpu_set_flow(int h, int nffr, const double* famig, const double* fff, const double* fface,
                     const double* fper, const double* qnam_dg, const double* qnph_dg) {
     Ctx* c = get(h);
     if (!c) return 1;
-    if (c->nfeds > 0 && nffr == 0) return fail("time-series flux boundary forcing is not supported");
-    c->nffr = nffr;
-    c->famig.assign(famig, famig + nffr); c->fff.assign(fff, fff + nffr);
-    c->fface.assign(fface, fface + nffr); c->fper.assign(fper, fper + nffr);
-    std::vector<double> qa((size_t)c->nfeds * nffr * 2), qp(qa.size());
-    for (int L = 0; L < c->nfeds; L++)
-        for (int j = 0; j < nffr; j++)
-            for (int k = 0; k < 2; k++) {
-                const size_t src = (size_t)j + (size_t)nffr * (L + (size_t)c->nfeds * k);
-                qa[((size_t)L * nffr + j) * 2 + k] = qnam_dg[src];
-                qp[((size_t)L * nffr + j) * 2 + k] = qnph_dg[src];
-            }
-    HIPCHK(c->qnam.upload(qa.data(), qa.size(), c->stream));
-    HIPCHK(c->qnph.upload(qp.data(), qp.size(), c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    return 0;
+    if (c->S != 1) return fail("the forcing of an ensemble is set on its members before dg_ens_create");
+    Member& m = c->mem[0];
+    if (nffr < 0) return fail("nffr < 0");
+    if (m.nfeds > 0 && nffr == 0) return fail("time-series flux boundary forcing is not supported");
+    m.nffr = nffr;
+    m.famig.assign(famig, famig + nffr); m.fff.assign(fff, fff + nffr);
+    m.fface.assign(fface, fface + nffr); m.fper.assign(fper, fper + nffr);
+    unpack_cols(nffr, m.nfeds, qnam_dg, m.qnam);
+    unpack_cols(nffr, m.nfeds, qnph_dg, m.qnph);
+    return upload_forcing(*c);
 }
 
 int dg_gpu_set_state(int h, const double* ze, const double* qx, const double* qy, const int* wdflg) {
@@ -573,11 +860,11 @@ int dg_gpu_set_state(int h, const double* ze, const double* qx, const double* qy
         for (int e = 0; e < ne; e++)
             for (int k = 0; k < dof; k++) buf[((size_t)f * dof + k) * ne + e] = src[f][(size_t)e * dof + k];
     HIPCHK(hipMemcpyAsync(c->u[c->ucur].p, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    std::vector<uint8_t> w;
     if (wdflg) {
-        std::vector<uint8_t> w(ne);
+        w.resize(ne);
         for (int e = 0; e < ne; e++) w[e] = wdflg[e] ? 1 : 0;
         HIPCHK(hipMemcpyAsync(c->wd[c->wdcur].p, w.data(), ne, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
@@ -606,20 +893,14 @@ int dg_gpu_step(int h, int it, int nsteps) {
     Ctx* c = get(h);
     if (!c) return 1;
     if (nsteps <= 0) return 0;
-    if (c->needs > 0 && c->amig.empty() && c->nbfr == 0) return fail("tidal forcing not set (dg_gpu_set_tide)");
-    if (c->nfeds > 0 && c->nffr == 0) return fail("flux forcing not set (dg_gpu_set_flow)");
-    // forcing tables are built per chunk of steps so the host work overlaps nothing but stays small
-    const int chunk = 4096;
-    for (int s0 = 0; s0 < nsteps; s0 += chunk) {
-        const int n = std::min(chunk, nsteps - s0);
-        if (s0 > 0 || c->nbfr || c->nffr) {
-            HIPCHK(hipStreamSynchronize(c->stream));   // previous tables may still be read
-            if (build_stage_tables(*c, it + s0, n)) return 1;
-        }
-        for (int s = 0; s < n; s++) {
-            const int rc = c->prm.p == 1 ? launch_step<1>(*c, s * c->rk.nrk) : launch_step<2>(*c, s * c->rk.nrk);
-            if (rc) return rc;
-        }
+    for (auto& m : c->mem) {
+        if (m.needs > 0 && m.nbfr == 0) return fail("tidal forcing not set (dg_gpu_set_tide)");
+        if (m.nfeds > 0 && m.nffr == 0) return fail("flux forcing not set (dg_gpu_set_flow)");
+    }
+    for (int s0 = 0; s0 < nsteps; s0 += kTableChunk) {
+        const int n = std::min(kTableChunk, nsteps - s0);
+        if (build_stage_tables(*c, it + s0, n)) return 1;
+        if (run_chunk(*c, n)) return 1;
     }
     if (c->prof) return prof_flush(*c);
     return 0;
@@ -638,7 +919,7 @@ int dg_gpu_modal2nodal(int h, double* eta2, double* uu2, double* vv2, int* nodec
     const int np = c->np;
     ElemData E = elem_data(*c);
     const int B = 256, gn = (np + B - 1) / B;
-    int ev = prof_begin(*c, K_M2N);
+    int ev = prof_begin(*c, K_M2N, false);
     if (c->prm.p == 1)
         hipLaunchKernelGGL((modal2nodal_kernel<1>), dim3(gn), dim3(B), 0, c->stream, np, E, (const int*)c->n2e_off.p,
                            (const int*)c->n2e.p, (const double*)c->u[c->ucur].p, (const uint8_t*)c->wd[c->wdcur].p,
@@ -699,12 +980,138 @@ int dg_gpu_kernel_stats(int h, int kid, double* total_ms, long long* launches, c
     return 0;
 }
 
-int dg_gpu_finalize(int h) {
-    std::lock_guard<std::mutex> lk(g_mu);
-    if (h < 0 || h >= (int)g_ctx.size() || !g_ctx[h]) { g_err = "invalid handle"; return 1; }
-    (void)hipStreamSynchronize(g_ctx[h]->stream);
-    g_ctx[h].reset();
+int dg_ens_create(int nmembers, const int* handles, int* ens_handle) {
+    if (nmembers < 1 || nmembers > 65535) return fail("an ensemble has 1 to 65535 members");
+    std::vector<Ctx*> src(nmembers);
+    for (int i = 0; i < nmembers; i++) {
+        src[i] = get(handles[i]);
+        if (!src[i]) return 1;
+        if (src[i]->S != 1) return fail("ensemble members must be single runs");
+    }
+    const dg_params& p0 = src[0]->prm;
+    for (int i = 1; i < nmembers; i++) {
+        const dg_params& q = src[i]->prm;
+        if (q.p != p0.p || q.rk_stage != p0.rk_stage || q.rk_order != p0.rk_order || q.ics != p0.ics ||
+            q.nolibf != p0.nolibf || q.nolifa != p0.nolifa || q.nolicat != p0.nolicat ||
+            q.slopeflag != p0.slopeflag || q.g != p0.g || q.hbreak != p0.hbreak || q.ftheta != p0.ftheta ||
+            q.fgamma != p0.fgamma || src[i]->phys.load_manning != src[0]->phys.load_manning)
+            return fail("ensemble members must share p, the RK scheme, ICS, NOLIBF, NOLIFA, NOLICAT, the "
+                        "slope limiter, G, the hybrid friction constants and whether Manning n is loaded");
+    }
+    for (int i = 0; i < nmembers; i++)
+        for (const Member& m : src[i]->mem) {
+            if (m.needs > 0 && m.nbfr == 0) return fail("member without tidal forcing (dg_gpu_set_tide)");
+            if (m.nfeds > 0 && m.nffr == 0) return fail("member without flux forcing (dg_gpu_set_flow)");
+        }
+    auto c = std::make_unique<Ctx>();
+    if (init_ctx_common(*c, &p0, src[0]->phys.load_manning != 0)) return 1;
+    c->S = nmembers;
+    c->mem.resize(nmembers);
+    for (int i = 0; i < nmembers; i++) c->mem[i] = src[i]->mem[0];
+    if (upload_mesh(*c)) return 1;
+    // each member's current state into the batch
+    const int dof = c->dof;
+    for (int i = 0; i < nmembers; i++) {
+        Ctx& s = *src[i];
+        HIPCHK(hipStreamSynchronize(s.stream));
+        for (int fk = 0; fk < 3 * dof; fk++)
+            HIPCHK(hipMemcpyAsync(c->u[0].p + (size_t)fk * c->ne + c->eoff[i], s.u[s.ucur].p + (size_t)fk * s.ne,
+                                  (size_t)s.ne * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->wd[0].p + c->eoff[i], s.wd[s.wdcur].p, s.ne, hipMemcpyDeviceToDevice, c->stream));
+    }
+    c->ucur = 0; c->wdcur = 0;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return register_ctx(std::move(c), ens_handle);
+}
+
+int dg_ens_layout(int h, int* nmembers, int* elem_offsets, int* node_offsets) {
+    Ctx* c = get(h);
+    if (!c) return 1;
+    if (nmembers) *nmembers = c->S;
+    for (int i = 0; i <= c->S; i++) {
+        if (elem_offsets) elem_offsets[i] = c->eoff[i];
+        if (node_offsets) node_offsets[i] = c->noff[i];
+    }
     return 0;
+}
+
+int dg_gpu_use_graph(int h, int enable) {
+    Ctx* c = get(h);
+    if (!c) return 1;
+    c->use_graph = enable != 0;
+    if (!c->use_graph) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->drop_graphs();
+    }
+    return 0;
+}
+
+int dg_gpu_autotune(int h, int* block) {
+    Ctx* c = get(h);
+    if (!c) return 1;
+    // time the first-stage kernel at each block size; it reads the stage-1 slot and writes only
+    // the stage-2 slot and the spare wet/dry buffer, both rewritten by the next step
+    const int nrk = c->rk.nrk;
+    int slot[KMAXRK + 1];
+    for (int i = 0; i <= nrk; i++) slot[i] = (c->ucur + i) % (nrk + 1);
+    StageArgs a = stage_args(*c, 0, slot);
+    a.rout = nullptr;
+    const bool sph = c->prm.ics == 2, fuse = c->prm.nolifa >= 2 && c->prm.slopeflag == 0;
+    const std::vector<int> cand = c->prm.p == 1 ? std::vector<int>{64, 128, 256} : std::vector<int>{64, 128};
+    if (build_stage_tables(*c, 1, 1)) return 1;
+    hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->step.p, 1, 0);
+    HIPCHK(hipGetLastError());
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return fail("hipEventCreate failed"); }
+    float best = 1e30f;
+    int best_nt = c->block, rc = 0;
+    for (int nt : cand) {
+        float ms = 0.f;
+        for (int rep = 0; rep < 2 && !rc; rep++) {   // the first pass warms caches and code
+            if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail("hipEventRecord failed"); break; }
+            for (int k = 0; k < 3; k++) {
+                if (c->prm.p == 1) launch_stage_nt<1>(*c, a, sph, fuse, nt);
+                else launch_stage_nt<2>(*c, a, sph, fuse, nt);
+            }
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) rc = fail(std::string("autotune: ") + hipGetErrorString(e));
+        }
+        if (rc) break;
+        if (ms < best) { best = ms; best_nt = nt; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (best_nt != c->block) {
+        c->block = best_nt;
+        c->drop_graphs();
+    }
+    if (block) *block = best_nt;
+    return 0;
+}
+
+int dg_gpu_set_block(int h, int block) {
+    Ctx* c = get(h);
+    if (!c) return 1;
+    if (block != 64 && block != 128 && !(block == 256 && c->prm.p == 1))
+        return fail("block size must be 64, 128 or (p=1) 256");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (block != c->block) { c->block = block; c->drop_graphs(); }
+    return 0;
+}
+
+int dg_gpu_finalize(int h) {
+    std::unique_ptr<Ctx> c;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (h < 0 || h >= (int)g_ctx.size() || !g_ctx[h]) { g_err = "invalid handle"; return 1; }
+        c = std::move(g_ctx[h]);
+    }
+    return 0;   // the context's destructor waits for its stream and frees its memory
 }
 
 }  // extern "C"

@@ -37,18 +37,31 @@ struct DevTables {
     double xegp[KMAXEGP];
 };
 
-// Physics switches and constants (src/dgswem.F:131-170 derivations).
+// Physics switches and constants shared by every scenario of a batch
+// (src/dgswem.F:131-170 derivations).
 struct Phys {
     double g, nleq, leq, nleqg, fg_l;
-    double cf, hbreak, ftheta, fgamma, dtdph, h0;
-    double cori;       // uniform Coriolis (used when cori_el == nullptr)
-    double fric;       // uniform FRIC_EL (used without Manning)
-    int ifnlfa, iflinbf, ifhybf, load_manning, rainfall;
+    double hbreak, ftheta, fgamma;
+    int ifnlfa, iflinbf, ifhybf, load_manning;
+};
+
+// Per-scenario scalars (one entry per ensemble member; one entry for a single run).
+struct Scen {
+    double dtdp;       // DTDP
+    double dtdph;      // 1/DTDP (friction cap)
+    double h0;         // H0
+    double cf;         // CF
+    double fric;       // uniform FRIC_EL (without Manning)
+    double cori;       // uniform Coriolis (when cori_el == nullptr)
+    double rain;       // rain source rate (rainfall = 1) or 0
+    double pad;
 };
 
 // Element-major structure-of-arrays mesh data on the device.
 struct ElemData {
     int ne;
+    const uint16_t* __restrict__ sid;   // [ne] scenario of each element, nullptr for one scenario
+    const Scen* __restrict__ scen;      // [nscen]
     const double* __restrict__ xv;      // [3][ne] vertex x
     const double* __restrict__ yv;      // [3][ne] vertex y
     const double* __restrict__ dpv;     // [3][ne] vertex depth DP
@@ -61,9 +74,12 @@ struct ElemData {
     const int* __restrict__ nm;         // [3][ne] 0-based vertex nodes
 };
 
-// Boundary forcing.  Per-stage scalars come from host-built tables indexed by the stage counter.
+// Boundary forcing.  Per-stage scalars come from host-built tables [scenario][row][constituent];
+// the row of the current stage is (*step)*nrk + irk, with the step counter kept on the device so
+// that a captured graph of several steps can be replayed.
 struct Forcing {
-    int nbfr, nffr;
+    int nbfr, nffr, rows, nrk;
+    const int* __restrict__ step;       // device step counter within the current tables
     const double* __restrict__ emo;     // [needs][nbfr][2]
     const double* __restrict__ efa;     // [needs][nbfr][2]
     const double* __restrict__ qnam;    // [nfeds][nffr][2]

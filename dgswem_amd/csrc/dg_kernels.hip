@@ -42,11 +42,10 @@ struct StageArgs {
     const double* uprev[KMAXRK];
     double aprev[KMAXRK];                // ATVD(IRK,I), 0 = term absent
     const double* rprev[KMAXRK];
-    double bprev[KMAXRK];                // BTVD(IRK,I)*DT, 0 = term absent
-    double acur, bcur;                   // ATVD(IRK,IRK), BTVD(IRK,IRK)*DT
+    double bprev[KMAXRK];                // BTVD(IRK,I) (times the scenario's DT in the kernel)
+    double acur, bcur;                   // ATVD(IRK,IRK), BTVD(IRK,IRK)
     double* __restrict__ rout;           // RHS(IRK) kept for later stages, or nullptr
-    int stage;                           // row of the forcing tables
-    double rain;                         // rain source rate (rainfall = 1), else 0
+    int irk;                             // stage index
 };
 
 // ------------------------------------------------------------------------------------------
@@ -168,8 +167,6 @@ __device__ __forceinline__ int wetdry_element(const DevTables& T, double (&U)[3]
 //   SSP-RK update      src/DG_hydro_timestep.F:238-290
 //   element geometry   src/read_input.F:1408, src/prep_DG.F:959-1163, src/calc_normal.F:28-49
 // ------------------------------------------------------------------------------------------
-template <int P> struct StageBlock { static constexpr int N = (P == 1) ? 256 : 128; };
-
 // Trace of the element's modes at a point with basis values ph[] (sum from zero, mode order).
 template <int DOF>
 __device__ __forceinline__ double trace(const double (&u)[DOF], const double* ph) {
@@ -179,12 +176,11 @@ __device__ __forceinline__ double trace(const double (&u)[DOF], const double* ph
     return s;
 }
 
-template <int P, bool SPH, bool WDFUSED>
-__global__ void __launch_bounds__(StageBlock<P>::N) stage_kernel(StageArgs a) {
+template <int P, int NT, bool SPH, bool WDFUSED>
+__global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
     constexpr int DOF = (P + 1) * (P + 2) / 2;
     constexpr int NAGP = (P == 1) ? 3 : 6;
     constexpr int NEGP = P + 2;
-    constexpr int NT = StageBlock<P>::N;
     // per-thread accumulators kept in LDS to bound register use: [0] boundary-edge part of the
     // RHS (accumulated first, like the reference's boundary edge routines), [1] internal-edge
     // slot sum (add_internal_flux)
@@ -196,6 +192,9 @@ __global__ void __launch_bounds__(StageBlock<P>::N) stage_kernel(StageArgs a) {
     const int tid = threadIdx.x;
     const int e = xcd_tile(blockIdx.x, gridDim.x) * NT + tid;
     if (e >= ne) return;
+    const int sid = E.sid ? (int)E.sid[e] : 0;
+    const Scen sc = E.scen[sid];
+    const int row = *a.fo.step * a.fo.nrk + a.irk;
 
     double U[3][DOF];
 #pragma unroll
@@ -337,21 +336,23 @@ __global__ void __launch_bounds__(StageBlock<P>::N) stage_kernel(StageArgs a) {
                     double ze_ex = 0.0;
                     const double xe = T.xegp[I];
                     for (int jj = 0; jj < a.fo.nbfr; jj++) {
-                        const double argj = a.fo.argj[(size_t)a.stage * a.fo.nbfr + jj];
-                        const double rff = a.fo.rff[(size_t)a.stage * a.fo.nbfr + jj];
+                        const size_t t = ((size_t)sid * a.fo.rows + row) * a.fo.nbfr + jj;
+                        const double argj = a.fo.argj[t];
+                        const double rff = a.fo.rff[t];
                         const size_t o = ((size_t)nb * a.fo.nbfr + jj) * 2;
                         const double z1 = a.fo.emo[o] * rff * cos(argj - a.fo.efa[o]);
                         const double z2 = a.fo.emo[o + 1] * rff * cos(argj - a.fo.efa[o + 1]);
                         ze_ex = ze_ex + (0.5 * (z1 + z2) + 0.5 * (z2 - z1) * xe);
                     }
-                    if ((ze_ex * ph.ifnlfa + hb_in) <= 0) ze_ex = fabs(hb_in) + ph.h0;
+                    if ((ze_ex * ph.ifnlfa + hb_in) <= 0) ze_ex = fabs(hb_in) + sc.h0;
                     llf_flux(ph, ze_in, ze_ex, qx_in, qx_in, qy_in, qy_in, hb_in, hb_in, sf, sf, nx, ny, ph.g, f0, f1, f2);
                 } else {
                     double qn_ext = 0.0, qx_ex = 0.0, qy_ex = 0.0;
                     const double xe = T.xegp[I];
                     for (int jj = 0; jj < a.fo.nffr; jj++) {
-                        const double argj = a.fo.fargj[(size_t)a.stage * a.fo.nffr + jj];
-                        const double rff = a.fo.frff[(size_t)a.stage * a.fo.nffr + jj];
+                        const size_t t = ((size_t)sid * a.fo.rows + row) * a.fo.nffr + jj;
+                        const double argj = a.fo.fargj[t];
+                        const double rff = a.fo.frff[t];
                         const size_t o = ((size_t)nb * a.fo.nffr + jj) * 2;
                         const double q1 = a.fo.qnam[o], q2 = a.fo.qnam[o + 1];
                         const double p1 = a.fo.qnph[o], p2 = a.fo.qnph[o + 1];
@@ -382,11 +383,11 @@ __global__ void __launch_bounds__(StageBlock<P>::N) stage_kernel(StageArgs a) {
         for (int k = 0; k < DOF; k++) R[f][k] = acc_lds[0][f * DOF + k][tid] + acc_lds[1][f * DOF + k][tid];
 
     // ---- volume quadrature (RHS_DG_HYDRO)
-    if (a.rain > 0.0) {
+    if (sc.rain > 0.0) {
 #pragma unroll 1
         for (int I = 0; I < NAGP; I++)
 #pragma unroll
-            for (int k = 0; k < DOF; k++) R[0][k] = R[0][k] + T.srfac[I][k] * a.rain;
+            for (int k = 0; k < DOF; k++) R[0][k] = R[0][k] + T.srfac[I][k] * sc.rain;
     }
     if (wd != 0) {
         double drdx, dsdx, drdy, dsdy;
@@ -402,7 +403,7 @@ __global__ void __launch_bounds__(StageBlock<P>::N) stage_kernel(StageArgs a) {
         }
         const double depth_c = hb[0] + U[0][0];
         const double fh_nl_c = 1.0 / (ph.nleq * depth_c + ph.leq);
-        const double cori = E.cori_el ? E.cori_el[e] : ph.cori;
+        const double cori = E.cori_el ? E.cori_el[e] : sc.cori;
         const double nbar = ph.load_manning ? E.nbar[e] : 0.0;
 #pragma unroll 1
         for (int I = 0; I < NAGP; I++) {
@@ -435,17 +436,17 @@ __global__ void __launch_bounds__(StageBlock<P>::N) stage_kernel(StageArgs a) {
             const double gh2 = fg * (0.5 * ze + hbq) + ph.fg_l * ze;
             const double gx = (huu + gh2) * sfq, gy = huv;
             const double hx = huv * sfq, hy = hvv + gh2;
-            double fric = ph.fric;
+            double fric = sc.fric;
             if (ph.load_manning) {
                 fric = ph.g * (nbar * nbar) / pow(depth, 1.0 / 3.0);
-                if (fric < ph.cf) fric = ph.cf;
+                if (fric < sc.cf) fric = sc.cf;
             }
             double tau = fric;
             if (ph.iflinbf == 0) {
                 const double umag = sqrt(u * u + v * v);
                 tau = tau * umag * fh;
                 if (ph.ifhybf == 1) tau = tau * pow(1.0 + pow(ph.hbreak * fh_nl_c, ph.ftheta), ph.fgamma / ph.ftheta);
-                const double cap = 0.9 * ph.dtdph;
+                const double cap = 0.9 * sc.dtdph;
                 if (cap < tau) tau = cap;
             }
             const double sx = -(tau * qx) + fg * dhbx * sfq + cori * qy;
@@ -471,13 +472,14 @@ __global__ void __launch_bounds__(StageBlock<P>::N) stage_kernel(StageArgs a) {
             for (int k = 0; k < DOF; k++) a.rout[(size_t)(f * DOF + k) * ne + e] = R[f][k];
     }
     double W[3][DOF];
+    const double bcur = a.bcur * sc.dtdp;
 #pragma unroll
     for (int f = 0; f < 3; f++)
 #pragma unroll
         for (int k = 0; k < DOF; k++) W[f][k] = 0.0;
 #pragma unroll 1
     for (int i = 0; i < a.nprev; i++) {
-        const double ai = a.aprev[i], bi = a.bprev[i];
+        const double ai = a.aprev[i], bi = a.bprev[i] * sc.dtdp;
         if (ai != 0.0) {
             const double* up = a.uprev[i];
 #pragma unroll
@@ -496,12 +498,12 @@ __global__ void __launch_bounds__(StageBlock<P>::N) stage_kernel(StageArgs a) {
 #pragma unroll
     for (int f = 0; f < 3; f++)
 #pragma unroll
-        for (int k = 0; k < DOF; k++) W[f][k] = W[f][k] + a.acur * U[f][k] + a.bcur * R[f][k];
+        for (int k = 0; k < DOF; k++) W[f][k] = W[f][k] + a.acur * U[f][k] + bcur * R[f][k];
 
     if (WDFUSED) {
         const double D[3] = {E.dpv[e], E.dpv[(size_t)ne + e], E.dpv[2 * (size_t)ne + e]};
         const double dpe_min = fmin(fmin(D[0], D[1]), D[2]);
-        const int nw = wetdry_element<P>(T, W, D, hb, dpe_min, wd, ph.h0);
+        const int nw = wetdry_element<P>(T, W, D, hb, dpe_min, wd, sc.h0);
         a.wd_out[e] = (uint8_t)nw;
     }
 #pragma unroll
@@ -540,7 +542,7 @@ template <int P, bool LIMIT, bool WETDRY>
 __global__ void __launch_bounds__(256) limit_wetdry_kernel(int np, ElemData E, double* __restrict__ u,
                                                            const double* __restrict__ mm,
                                                            const uint8_t* __restrict__ wd_in,
-                                                           uint8_t* __restrict__ wd_out, double h0) {
+                                                           uint8_t* __restrict__ wd_out) {
     constexpr int DOF = (P + 1) * (P + 2) / 2;
     const DevTables& T = tab<P>();
     const int ne = E.ne;
@@ -599,7 +601,8 @@ __global__ void __launch_bounds__(256) limit_wetdry_kernel(int np, ElemData E, d
         const double c13 = 1.0 / 3.0, c16 = 1.0 / 6.0;
         const double hb[3] = {c13 * (D[0] + D[1] + D[2]), -(c16 * (D[0] + D[1])) + c13 * D[2], -(0.5 * D[0]) + 0.5 * D[1]};
         const double dpe_min = fmin(fmin(D[0], D[1]), D[2]);
-        wd_out[e] = (uint8_t)wetdry_element<P>(T, U, D, hb, dpe_min, wd_in[e], h0);
+        const int sid = E.sid ? (int)E.sid[e] : 0;
+        wd_out[e] = (uint8_t)wetdry_element<P>(T, U, D, hb, dpe_min, wd_in[e], E.scen[sid].h0);
     }
 #pragma unroll
     for (int f = 0; f < 3; f++)
@@ -660,6 +663,12 @@ __global__ void __launch_bounds__(256) modal2nodal_kernel(int np, ElemData E, co
     }
     out[n] = e2; out[(size_t)np + n] = u2; out[2 * (size_t)np + n] = v2;
     nodecode[n] = nc;
+}
+
+// Device step counter used by the forcing-table lookups (a kernel, not a memset, so that the
+// sequence can be captured into a HIP graph).
+__global__ void step_counter_kernel(int* __restrict__ step, int set, int value) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step = set ? value : *step + 1;
 }
 
 // Water volume per element: 0.5*AREAS*(ze_1 + hb_1), reduced per block.

@@ -28,7 +28,8 @@ K_STAGE, K_MINMAX, K_LIMIT, K_M2N = 0, 1, 2, 3
 EXPORTED = ["dg_create_edge_data", "dg_gpu_init", "dg_gpu_set_tide", "dg_gpu_set_flow",
             "dg_gpu_set_state", "dg_gpu_get_state", "dg_gpu_step", "dg_gpu_synchronize",
             "dg_gpu_modal2nodal", "dg_gpu_mass", "dg_gpu_profile", "dg_gpu_kernel_stats",
-            "dg_gpu_last_error", "dg_gpu_finalize"]
+            "dg_ens_create", "dg_ens_layout", "dg_gpu_use_graph", "dg_gpu_autotune",
+            "dg_gpu_set_block", "dg_gpu_last_error", "dg_gpu_finalize"]
 
 
 class DgParams(C.Structure):
@@ -70,6 +71,11 @@ def load_library(path: str = LIB_PATH):
     L.dg_gpu_profile.argtypes = [C.c_int, C.c_int]
     L.dg_gpu_kernel_stats.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_longlong), C.c_char_p, C.c_int]
+    L.dg_ens_create.argtypes = [C.c_int, _ip, C.POINTER(C.c_int)]
+    L.dg_ens_layout.argtypes = [C.c_int, C.POINTER(C.c_int), C.c_void_p, C.c_void_p]
+    L.dg_gpu_use_graph.argtypes = [C.c_int, C.c_int]
+    L.dg_gpu_autotune.argtypes = [C.c_int, C.POINTER(C.c_int)]
+    L.dg_gpu_set_block.argtypes = [C.c_int, C.c_int]
     L.dg_gpu_finalize.argtypes = [C.c_int]
     for name in EXPORTED:
         getattr(L, name).restype = C.c_char_p if name == "dg_gpu_last_error" else C.c_int
@@ -161,7 +167,92 @@ def qnam_dg(case: Case) -> tuple:
     return qa.reshape(-1, order="F"), qp.reshape(-1, order="F")
 
 
-class GpuSolver:
+class _Handle:
+    """Operations common to a single run and an ensemble (one library handle each)."""
+
+    h = -1
+    ne = np_ = dofh = 0
+    it = 0
+
+    def close(self):
+        if getattr(self, "h", -1) >= 0:
+            load_library().dg_gpu_finalize(self.h)
+            self.h = -1
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n: int = 1, sync: bool = True) -> None:
+        """Advance n time steps (DG_HYDRO_TIMESTEP(IT) for IT = it+1 .. it+n)."""
+        L = load_library()
+        _check(L.dg_gpu_step(self.h, self.it + 1, n))
+        self.it += n
+        if sync:
+            _check(L.dg_gpu_synchronize(self.h))
+
+    def synchronize(self) -> None:
+        _check(load_library().dg_gpu_synchronize(self.h))
+
+    def state(self):
+        """(ZE, QX, QY) as [element][mode] arrays and WDFLG."""
+        n = self.ne * self.dofh
+        ze, qx, qy = np.empty(n), np.empty(n), np.empty(n)
+        wd = np.empty(self.ne, np.int32)
+        _check(load_library().dg_gpu_get_state(self.h, ze, qx, qy, wd))
+        s = (self.ne, self.dofh)
+        return ze.reshape(s), qx.reshape(s), qy.reshape(s), wd
+
+    def set_state(self, ze, qx, qy, wdflg=None):
+        w = _i(wdflg) if wdflg is not None else None
+        _check(load_library().dg_gpu_set_state(self.h, _f(ze).reshape(-1), _f(qx).reshape(-1),
+                                                _f(qy).reshape(-1),
+                                                w.ctypes.data if w is not None else None))
+
+    def nodal(self):
+        """ETA2, UU2, VV2 and NODECODE of modal2nodal."""
+        e, u, v = np.empty(self.np_), np.empty(self.np_), np.empty(self.np_)
+        nc = np.empty(self.np_, np.int32)
+        _check(load_library().dg_gpu_modal2nodal(self.h, e, u, v, nc))
+        return e, u, v, nc
+
+    def mass(self) -> float:
+        m = C.c_double()
+        _check(load_library().dg_gpu_mass(self.h, C.byref(m)))
+        return m.value
+
+    def profile(self, enable: bool) -> None:
+        _check(load_library().dg_gpu_profile(self.h, 1 if enable else 0))
+
+    def kernel_stats(self, kid: int):
+        t = C.c_double(); n = C.c_longlong(); name = C.create_string_buffer(64)
+        _check(load_library().dg_gpu_kernel_stats(self.h, kid, C.byref(t), C.byref(n), name, 64))
+        return name.value.decode(), t.value, n.value
+
+    def use_graph(self, enable: bool = True) -> None:
+        _check(load_library().dg_gpu_use_graph(self.h, 1 if enable else 0))
+
+    def autotune(self) -> int:
+        b = C.c_int()
+        _check(load_library().dg_gpu_autotune(self.h, C.byref(b)))
+        return b.value
+
+    def set_block(self, block: int) -> None:
+        _check(load_library().dg_gpu_set_block(self.h, block))
+
+    def layout(self):
+        """Element and node offsets of the members (two entries for a single run)."""
+        L = load_library()
+        n = C.c_int()
+        _check(L.dg_ens_layout(self.h, C.byref(n), None, None))
+        eo = np.empty(n.value + 1, np.int32); no = np.empty(n.value + 1, np.int32)
+        _check(L.dg_ens_layout(self.h, C.byref(n), eo.ctypes.data, no.ctypes.data))
+        return eo, no
+
+
+class GpuSolver(_Handle):
     """One DG-SWEM run on the GPU (one library handle)."""
 
     def __init__(self, case: Case):
@@ -193,56 +284,30 @@ class GpuSolver:
                                      _f(p.fper), _f(qa), _f(qp)))
         self.it = 0
 
-    def close(self):
-        if getattr(self, "h", -1) >= 0:
-            load_library().dg_gpu_finalize(self.h)
-            self.h = -1
 
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+class Ensemble(_Handle):
+    """Members (GpuSolver runs, possibly of different meshes and time steps) advanced together:
+    one stage-kernel launch per RK stage covers every member.  The members keep their own
+    handles; the ensemble starts from their current states and all must be at the same step."""
 
-    def step(self, n: int = 1, sync: bool = True) -> None:
+    def __init__(self, members: list):
+        if not members:
+            raise GpuError("an ensemble needs at least one member")
+        its = {m.it for m in members}
+        if len(its) != 1:
+            raise GpuError("ensemble members must be at the same time step")
         L = load_library()
-        _check(L.dg_gpu_step(self.h, self.it + 1, n))
-        self.it += n
-        if sync:
-            _check(L.dg_gpu_synchronize(self.h))
+        hs = np.array([m.h for m in members], np.int32)
+        h = C.c_int(-1)
+        _check(L.dg_ens_create(len(members), hs, C.byref(h)))
+        self.h = h.value
+        self.nmembers = len(members)
+        self.dofh = members[0].dofh
+        self.eoff, self.noff = self.layout()
+        self.ne, self.np_ = int(self.eoff[-1]), int(self.noff[-1])
+        self.it = its.pop()
 
-    def synchronize(self) -> None:
-        _check(load_library().dg_gpu_synchronize(self.h))
-
-    def state(self):
-        n = self.ne * self.dofh
-        ze, qx, qy = np.empty(n), np.empty(n), np.empty(n)
-        wd = np.empty(self.ne, np.int32)
-        _check(load_library().dg_gpu_get_state(self.h, ze, qx, qy, wd))
-        s = (self.ne, self.dofh)
-        return ze.reshape(s), qx.reshape(s), qy.reshape(s), wd
-
-    def set_state(self, ze, qx, qy, wdflg=None):
-        w = _i(wdflg) if wdflg is not None else None
-        _check(load_library().dg_gpu_set_state(self.h, _f(ze).reshape(-1), _f(qx).reshape(-1),
-                                                _f(qy).reshape(-1),
-                                                w.ctypes.data if w is not None else None))
-
-    def nodal(self):
-        e, u, v = np.empty(self.np_), np.empty(self.np_), np.empty(self.np_)
-        nc = np.empty(self.np_, np.int32)
-        _check(load_library().dg_gpu_modal2nodal(self.h, e, u, v, nc))
-        return e, u, v, nc
-
-    def mass(self) -> float:
-        m = C.c_double()
-        _check(load_library().dg_gpu_mass(self.h, C.byref(m)))
-        return m.value
-
-    def profile(self, enable: bool) -> None:
-        _check(load_library().dg_gpu_profile(self.h, 1 if enable else 0))
-
-    def kernel_stats(self, kid: int):
-        t = C.c_double(); n = C.c_longlong(); name = C.create_string_buffer(64)
-        _check(load_library().dg_gpu_kernel_stats(self.h, kid, C.byref(t), C.byref(n), name, 64))
-        return name.value.decode(), t.value, n.value
+    def split(self, arrays, nodes: bool = False):
+        """Split per-element (or per-node) arrays of the batch into per-member lists."""
+        off = self.noff if nodes else self.eoff
+        return [[a[off[i]:off[i + 1]] for a in arrays] for i in range(self.nmembers)]

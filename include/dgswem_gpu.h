@@ -116,6 +116,37 @@ int dg_gpu_profile(int handle, int enable);
 int dg_gpu_kernel_stats(int handle, int kernel_id, double* total_ms, long long* launches,
                         char* name, int name_len);
 
+/* ---- Batched ensembles, graph replay and launch tuning.  None of these has a counterpart in the
+ * reference, which runs one scenario per process: an ensemble replaces N separate dgswem_serial
+ * runs (one DG_HYDRO_TIMESTEP per member per step) with one stage kernel launch per RK stage over
+ * all members. */
+
+/* Batch `nmembers` single-run handles (each set up with dg_gpu_init, dg_gpu_set_tide/flow and
+ * optionally dg_gpu_set_state) into a new handle.  Meshes may differ in size; members must share
+ * p, the RK scheme, ICS, NOLIBF, NOLIFA, NOLICAT, the slope limiter, G, the hybrid friction
+ * constants and whether Manning n is loaded.  DTDP, H0, CF, STATIM, DRAMP, RAINFALL, the Coriolis
+ * parameter and the boundary forcing are per member.  The member handles stay valid and
+ * independent; the ensemble copies their mesh, forcing and current state.  All ensemble entry
+ * points (step, set/get_state, modal2nodal, mass) work on the concatenation of the members:
+ * element e of member m is element elem_offsets[m] + e, node n is node node_offsets[m] + n. */
+int dg_ens_create(int nmembers, const int* member_handles, int* ens_handle);
+
+/* Member count and the nmembers+1 element and node offsets of a handle (0, NE for one run). */
+int dg_ens_layout(int handle, int* nmembers, int* elem_offsets, int* node_offsets);
+
+/* enable=1: dg_gpu_step replays HIP graphs of NRK+1 steps (after which the rotating stage buffers
+ * are back in place) instead of launching kernels one by one.  Graphs are captured on first use
+ * and dropped when the forcing or the block size changes.  Kernel timing (dg_gpu_profile) sees
+ * only the eager remainder of each call. */
+int dg_gpu_use_graph(int handle, int enable);
+
+/* Time the stage kernel at each supported block size on the handle's current state and keep the
+ * fastest; writes it to *block.  Touches only scratch stage buffers: the state is unchanged. */
+int dg_gpu_autotune(int handle, int* block);
+
+/* Set the stage-kernel block size explicitly: 64, 128, or 256 (p=1 only). */
+int dg_gpu_set_block(int handle, int block);
+
 /* Message of the last failure on this thread ("" if none). */
 const char* dg_gpu_last_error(void);
 
