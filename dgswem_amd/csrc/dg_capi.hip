@@ -40,9 +40,9 @@ int fail(const std::string& m) { g_err = m; return 1; }
         if (e_ != hipSuccess) return fail(std::string(#x) + ": " + hipGetErrorString(e_));          \
     } while (0)
 
-enum { K_STAGE = 0, K_MINMAX = 1, K_LIMIT = 2, K_M2N = 3, K_COUNT = 4 };
+enum { K_STAGE = 0, K_MINMAX = 1, K_LIMIT = 2, K_M2N = 3, K_MET = 4, K_COUNT = 5 };
 const char* kKernelNames[K_COUNT] = {"stage_kernel", "node_minmax_kernel", "limit_wetdry_kernel",
-                                     "modal2nodal_kernel"};
+                                     "modal2nodal_kernel", "met_kernel"};
 constexpr int kTableChunk = 2048;   // steps of forcing rows per table upload
 
 template <typename T>
@@ -80,6 +80,10 @@ struct Member {
     std::vector<double> emo, efa, qnam, qnph;   // [edge][constituent][2]
     std::vector<double> ze0;                    // [dof][ne] cold-start elevation modes
     std::vector<uint8_t> wd0;                   // cold-start wet/dry flags
+    // NWS=8 Holland vortex
+    bool met = false;
+    HollandTrack track;
+    std::vector<double> mxc, myc, mcosy, mcor;  // [np] node lon/lat (deg), cos(lat), Coriolis
 };
 
 struct Ctx {
@@ -111,6 +115,11 @@ struct Ctx {
     DevBuf<double> emo, efa, qnam, qnph, argj, rff, fargj, frff;
     DevBuf<int> step;                            // device step counter within the current tables
     int rows = 0;                                // table rows per scenario
+    // NWS=8 meteorological forcing
+    bool met = false;
+    DevBuf<double> ramp, mrow, mxc, myc, mcosy, mcor, mwx, mwy, mwm, mcd, mprn;
+    DevBuf<uint16_t> nsid;
+    int next_it = 1;                             // IT of the next step (for the autotuner's tables)
     // launch configuration
     int block = 256;
     // graphs of one slot-rotation period, keyed by (ucur, wdcur) at capture
@@ -209,7 +218,18 @@ Forcing forcing(Ctx& c) {
     F.step = c.step.p;
     F.emo = c.emo.p; F.efa = c.efa.p; F.qnam = c.qnam.p; F.qnph = c.qnph.p;
     F.argj = c.argj.p; F.rff = c.rff.p; F.fargj = c.fargj.p; F.frff = c.frff.p;
+    F.ramp = c.ramp.p;
     return F;
+}
+
+Met met_args(Ctx& c) {
+    Met m{};
+    m.np = c.np; m.rows = kTableChunk;
+    m.row = c.mrow.p;
+    m.nsid = c.S > 1 ? c.nsid.p : nullptr;
+    m.xc = c.mxc.p; m.yc = c.myc.p; m.cosy = c.mcosy.p; m.cor = c.mcor.p;
+    m.wx = c.mwx.p; m.wy = c.mwy.p; m.wm = c.mwm.p; m.cd = c.mcd.p; m.prn = c.mprn.p;
+    return m;
 }
 
 // --------------------------------------------------------------------------------------------
@@ -416,7 +436,38 @@ int upload_forcing(Ctx& c) {
     HIPCHK(c.rff.alloc((size_t)c.S * rows * std::max(c.nbfr, 1)));
     HIPCHK(c.fargj.alloc((size_t)c.S * rows * std::max(c.nffr, 1)));
     HIPCHK(c.frff.alloc((size_t)c.S * rows * std::max(c.nffr, 1)));
+    HIPCHK(c.ramp.alloc((size_t)c.S * rows));
     c.rows = (int)rows;
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return 0;
+}
+
+// Node arrays of the Holland forcing of all members (all members have it, or none).
+int upload_met(Ctx& c) {
+    c.met = c.mem[0].met;
+    for (auto& m : c.mem)
+        if (m.met != c.met) return fail("ensemble members must all have NWS=8 forcing, or none");
+    if (!c.met) return 0;
+    const int np = c.np;
+    std::vector<double> xc(np), yc(np), cy(np), co(np);
+    std::vector<uint16_t> ns(np);
+    for (int s = 0; s < c.S; s++) {
+        const Member& m = c.mem[s];
+        for (int i = 0; i < m.np; i++) {
+            const int g = c.noff[s] + i;
+            xc[g] = m.mxc[i]; yc[g] = m.myc[i]; cy[g] = m.mcosy[i]; co[g] = m.mcor[i];
+            ns[g] = (uint16_t)s;
+        }
+    }
+    c.drop_graphs();
+    HIPCHK(hipStreamSynchronize(c.stream));
+    HIPCHK(c.mxc.upload(xc.data(), np, c.stream));
+    HIPCHK(c.myc.upload(yc.data(), np, c.stream));
+    HIPCHK(c.mcosy.upload(cy.data(), np, c.stream));
+    HIPCHK(c.mcor.upload(co.data(), np, c.stream));
+    if (c.S > 1) HIPCHK(c.nsid.upload(ns.data(), np, c.stream));
+    for (DevBuf<double>* b : {&c.mwx, &c.mwy, &c.mwm, &c.mcd, &c.mprn}) HIPCHK(b->alloc(np));
+    HIPCHK(c.mrow.alloc((size_t)c.S * kTableChunk * MET_ROW));
     HIPCHK(hipStreamSynchronize(c.stream));
     return 0;
 }
@@ -511,6 +562,7 @@ int upload_mesh(Ctx& c) {
     for (int i = 0; i < 2; i++) HIPCHK(c.wd[i].alloc(ne));
     // the host vectors go out of scope: wait for the copies
     HIPCHK(hipStreamSynchronize(c.stream));
+    if (upload_met(c)) return 1;
     return upload_forcing(c);
 }
 
@@ -568,23 +620,30 @@ int upload_initial_state(Ctx& c) {
 // Per-stage forcing rows for steps it..it+n-1 of every member (host arithmetic of
 // src/DG_hydro_timestep.F:79-100, src/ocean_edge_hydro.F:59-75, src/flow_edge_hydro.F:45-62).
 int build_stage_tables(Ctx& c, int it, int n) {
-    if (c.nbfr == 0 && c.nffr == 0) return 0;
+    if (c.nbfr == 0 && c.nffr == 0 && !c.met) return 0;
     if (n > kTableChunk) return fail("internal: table chunk overflow");
     const int nrk = c.rk.nrk, rows = c.rows;
     std::vector<double> aj((size_t)c.S * rows * std::max(c.nbfr, 1), 0.0), rf(aj.size(), 0.0);
     std::vector<double> fa((size_t)c.S * rows * std::max(c.nffr, 1), 0.0), fr(fa.size(), 0.0);
+    std::vector<double> rp, mr;
+    if (c.met) { rp.assign((size_t)c.S * rows, 0.0); mr.assign((size_t)c.S * kTableChunk * MET_ROW, 0.0); }
+    std::string err;
     for (int sc = 0; sc < c.S; sc++) {
         const Member& m = c.mem[sc];
         for (int s = 0; s < n; s++) {
             const int IT = it + s;
             const double itdt = IT * m.prm.dtdp;
             const double time_a = itdt + m.prm.statim * 86400.0;
+            // storm scalars at TIME_A (MET_FORCING is called with TIME_A at every stage)
+            if (c.met && !holland_row(m.track, time_a, m.prm.g, &mr[((size_t)sc * kTableChunk + s) * MET_ROW], err))
+                return fail(err);
             for (int irk = 0; irk < nrk; irk++) {
                 const size_t row = (size_t)sc * rows + (size_t)s * nrk + irk;
                 const double timedg = time_a - m.prm.dtdp + c.rk.dtvd[irk] * m.prm.dtdp;
                 double ramp = 1.0;
                 if (m.prm.nramp == 1)
                     ramp = std::tanh((2.0 * ((IT - 1) + c.rk.dtvd[irk]) * m.prm.dtdp / 86400.0) / m.prm.dramp);
+                if (c.met) rp[row] = ramp;
                 for (int jj = 0; jj < m.nbfr; jj++) {
                     const int ncyc = (m.per[jj] == 0.0) ? 0 : (int)(timedg / m.per[jj]);
                     aj[row * c.nbfr + jj] = m.amig[jj] * (timedg - ncyc * m.per[jj]) + m.face[jj];
@@ -608,20 +667,30 @@ int build_stage_tables(Ctx& c, int it, int n) {
         HIPCHK(hipMemcpyAsync(c.fargj.p, fa.data(), fa.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
         HIPCHK(hipMemcpyAsync(c.frff.p, fr.data(), fr.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
     }
+    if (c.met) {
+        HIPCHK(hipMemcpyAsync(c.ramp.p, rp.data(), rp.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        HIPCHK(hipMemcpyAsync(c.mrow.p, mr.data(), mr.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    }
     HIPCHK(hipStreamSynchronize(c.stream));
     return 0;
 }
 
-template <int P, int NT>
-void launch_stage(Ctx& c, const StageArgs& a, bool sph, bool fuse) {
+template <int P, int NT, bool MET>
+void launch_stage_met(Ctx& c, const StageArgs& a, bool sph, bool fuse) {
     const int grid = (c.ne + NT - 1) / NT;
     if (sph) {
-        if (fuse) hipLaunchKernelGGL((stage_kernel<P, NT, true, true>), dim3(grid), dim3(NT), 0, c.stream, a);
-        else hipLaunchKernelGGL((stage_kernel<P, NT, true, false>), dim3(grid), dim3(NT), 0, c.stream, a);
+        if (fuse) hipLaunchKernelGGL((stage_kernel<P, NT, true, true, MET>), dim3(grid), dim3(NT), 0, c.stream, a);
+        else hipLaunchKernelGGL((stage_kernel<P, NT, true, false, MET>), dim3(grid), dim3(NT), 0, c.stream, a);
     } else {
-        if (fuse) hipLaunchKernelGGL((stage_kernel<P, NT, false, true>), dim3(grid), dim3(NT), 0, c.stream, a);
-        else hipLaunchKernelGGL((stage_kernel<P, NT, false, false>), dim3(grid), dim3(NT), 0, c.stream, a);
+        if (fuse) hipLaunchKernelGGL((stage_kernel<P, NT, false, true, MET>), dim3(grid), dim3(NT), 0, c.stream, a);
+        else hipLaunchKernelGGL((stage_kernel<P, NT, false, false, MET>), dim3(grid), dim3(NT), 0, c.stream, a);
     }
+}
+
+template <int P, int NT>
+void launch_stage(Ctx& c, const StageArgs& a, bool sph, bool fuse) {
+    if (c.met) launch_stage_met<P, NT, true>(c, a, sph, fuse);
+    else launch_stage_met<P, NT, false>(c, a, sph, fuse);
 }
 
 // Block sizes: p=1 64/128/256 threads; p=2 64/128 (its LDS accumulators double per thread).
@@ -640,6 +709,7 @@ void launch_stage_nt(Ctx& c, const StageArgs& a, bool sph, bool fuse, int nt) {
 StageArgs stage_args(Ctx& c, int irk, const int slot[]) {
     StageArgs a{};
     a.ed = elem_data(c); a.ph = c.phys; a.fo = forcing(c);
+    if (c.met) a.mt = met_args(c);
     a.uin = c.u[slot[irk]].p;
     a.uout = c.u[slot[irk + 1]].p;
     a.wd_in = c.wd[c.wdcur].p;
@@ -668,6 +738,12 @@ int launch_step(Ctx& c, bool capturing) {
     // stage slots: slot(i) holds U(i+1); slot(0) = current stage-1 state
     int slot[KMAXRK + 1];
     for (int i = 0; i <= nrk; i++) slot[i] = (c.ucur + i) % (nrk + 1);
+    if (c.met) {
+        int ev = prof_begin(c, K_MET, capturing);
+        hipLaunchKernelGGL(met_kernel, dim3(gn), dim3(B), 0, c.stream, met_args(c), (const int*)c.step.p);
+        prof_end(c, ev);
+        HIPCHK(hipGetLastError());
+    }
     for (int irk = 0; irk < nrk; irk++) {
         StageArgs a = stage_args(c, irk, slot);
         for (int i = 0; i < irk; i++)
@@ -850,6 +926,41 @@ int dg_gpu_set_flow(int h, int nffr, const double* famig, const double* fff, con
     return upload_forcing(*c);
 }
 
+int dg_gpu_set_holland(int h, int nrec, const int* rec, const int* ref, int storm_number, double bladj,
+                       const double* slam, const double* sfea) {
+    Ctx* c = get(h);
+    if (!c) return 1;
+    if (c->S != 1) return fail("the forcing of an ensemble is set on its members before dg_ens_create");
+    Member& m = c->mem[0];
+    std::string err;
+    HollandTrack t;
+    if (!build_holland_track(nrec, rec, ref, storm_number, bladj, t, err)) return fail(err);
+    if (!(bladj > 0.0)) return fail("BLAdj must be positive");
+    const double rad2deg = 180.0 / 3.141592653589793;
+    const double pi = 4.0 * std::atan(1.0), deg2rad = pi / 180.0, omega = 2.0 * pi / 86164.2;
+    m.mxc.resize(m.np); m.myc.resize(m.np); m.mcosy.resize(m.np); m.mcor.resize(m.np);
+    for (int i = 0; i < m.np; i++) {
+        m.mxc[i] = slam[i] * rad2deg;
+        m.myc[i] = sfea[i] * rad2deg;
+        m.mcosy[i] = std::cos(m.myc[i] * deg2rad);
+        m.mcor[i] = 2.0 * omega * std::sin(m.myc[i] * deg2rad);
+    }
+    m.track = t;
+    m.met = true;
+    if (upload_met(*c) || upload_forcing(*c)) return 1;
+    return 0;
+}
+
+int dg_holland_storm_at(int nrec, const int* rec, const int* ref, int storm_number, double bladj,
+                        double g, int ntimes, const double* times, double* rows) {
+    std::string err;
+    HollandTrack t;
+    if (!build_holland_track(nrec, rec, ref, storm_number, bladj, t, err)) return fail(err);
+    for (int k = 0; k < ntimes; k++)
+        if (!holland_row(t, times[k], g, rows + (size_t)k * MET_ROW, err)) return fail(err);
+    return 0;
+}
+
 int dg_gpu_set_state(int h, const double* ze, const double* qx, const double* qy, const int* wdflg) {
     Ctx* c = get(h);
     if (!c) return 1;
@@ -902,6 +1013,7 @@ int dg_gpu_step(int h, int it, int nsteps) {
         if (build_stage_tables(*c, it + s0, n)) return 1;
         if (run_chunk(*c, n)) return 1;
     }
+    c->next_it = it + nsteps;
     if (c->prof) return prof_flush(*c);
     return 0;
 }
@@ -1058,9 +1170,14 @@ int dg_gpu_autotune(int h, int* block) {
     a.rout = nullptr;
     const bool sph = c->prm.ics == 2, fuse = c->prm.nolifa >= 2 && c->prm.slopeflag == 0;
     const std::vector<int> cand = c->prm.p == 1 ? std::vector<int>{64, 128, 256} : std::vector<int>{64, 128};
-    if (build_stage_tables(*c, 1, 1)) return 1;
+    if (build_stage_tables(*c, c->next_it, 1)) return 1;
     hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->step.p, 1, 0);
     HIPCHK(hipGetLastError());
+    if (c->met) {
+        hipLaunchKernelGGL(met_kernel, dim3((c->np + 255) / 256), dim3(256), 0, c->stream, met_args(*c),
+                           (const int*)c->step.p);
+        HIPCHK(hipGetLastError());
+    }
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return fail("hipEventCreate failed"); }

@@ -35,6 +35,7 @@ struct DevTables {
     double m_inv[KMAXDOF];
     double wegp[KMAXEGP];
     double xegp[KMAXEGP];
+    double psi[KMAXAGP][3];   // linear nodal basis at the area points (wind stress interpolation)
 };
 
 // Physics switches and constants shared by every scenario of a batch
@@ -88,6 +89,26 @@ struct Forcing {
     const double* __restrict__ rff;     // [nstage][nbfr]
     const double* __restrict__ fargj;   // [nstage][nffr]
     const double* __restrict__ frff;    // [nstage][nffr]
+    const double* __restrict__ ramp;    // [scenario][row] RAMPDG of each stage (meteorological forcing)
+};
+
+// NWS=8 meteorological forcing at the nodes.  The node kernel evaluates the Holland profile once
+// per step at TIME_A (the reference evaluates it at every stage with the same TIME_A) and keeps the
+// ramp-free factors; the stage kernel forms WSX2 = RampMete*0.001293*Cd*Wx*|W| and
+// PR2 = RampMete*P with the stage's ramp, in the reference's operation order.
+struct Met {
+    int np, rows;                        // rows: steps per scenario in the storm table
+    const double* __restrict__ row;     // [scenario][rows][MET_ROW] storm scalars per step
+    const uint16_t* __restrict__ nsid;  // [np] scenario of each node, nullptr for one scenario
+    const double* __restrict__ xc;      // [np] longitude (deg)
+    const double* __restrict__ yc;      // [np] latitude (deg)
+    const double* __restrict__ cosy;    // [np] cos(latitude)
+    const double* __restrict__ cor;     // [np] 2 omega sin(latitude)
+    double* __restrict__ wx;            // [np] 10-m wind (m/s), x
+    double* __restrict__ wy;            // [np] y
+    double* __restrict__ wm;            // [np] |W|
+    double* __restrict__ cd;            // [np] drag coefficient
+    double* __restrict__ prn;           // [np] surface pressure (m of water)
 };
 
 // Local Lax-Friedrichs flux (src/numerical_flux.F:750-1022), operation order kept.

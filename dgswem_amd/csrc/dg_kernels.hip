@@ -13,6 +13,7 @@
 // Arithmetic follows the reference statement by statement (listed per kernel) and is compiled with
 // -ffp-contract=off, so each product and sum is rounded as in dgswem_serial.
 #include "dg_device.hpp"
+#include "dg_setup.hpp"
 
 namespace dg {
 
@@ -34,6 +35,7 @@ struct StageArgs {
     ElemData ed;
     Phys ph;
     Forcing fo;
+    Met mt;
     const double* __restrict__ uin;      // U(IRK) [3][dof][ne]
     double* __restrict__ uout;           // U(IRK+1)
     const uint8_t* __restrict__ wd_in;   // WDFLG during the stage
@@ -176,7 +178,7 @@ __device__ __forceinline__ double trace(const double (&u)[DOF], const double* ph
     return s;
 }
 
-template <int P, int NT, bool SPH, bool WDFUSED>
+template <int P, int NT, bool SPH, bool WDFUSED, bool MET>
 __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
     constexpr int DOF = (P + 1) * (P + 2) / 2;
     constexpr int NAGP = (P == 1) ? 3 : 6;
@@ -185,6 +187,8 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
     // RHS (accumulated first, like the reference's boundary edge routines), [1] internal-edge
     // slot sum (add_internal_flux)
     __shared__ double acc_lds[2][3 * DOF][NT];
+    // wind stress at the element's vertices and the pressure gradient sums (MET)
+    __shared__ double met_lds[MET ? 8 : 1][NT];
     const DevTables& T = tab<P>();
     const ElemData& E = a.ed;
     const Phys& ph = a.ph;
@@ -401,6 +405,27 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
             drdy = 1.0 / AREA * (X0 - X2);
             dsdy = 1.0 / AREA * (X1 - X0);
         }
+        if (MET) {
+            // nodal WSX2/WSY2/PR2 of this stage (src/met_forcing.F:507-509) and the
+            // QP-independent sums PR2(Ni)*DPSIDX(i) of src/rhs_dg_hydro.F:117-122,366-372
+            const double ramp = a.fo.ramp[(size_t)sid * a.fo.rows + row];
+            const double drpsi[3] = {-1.0 / 2.0, 1.0 / 2.0, 0.0}, dspsi[3] = {-1.0 / 2.0, 0.0, 1.0 / 2.0};
+            double pdx = 0.0, pdy = 0.0;
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                const int n = E.nm[(size_t)v * ne + e];
+                const double cd = a.mt.cd[n], wm = a.mt.wm[n];
+                met_lds[v][tid] = ramp * 0.001293 * cd * a.mt.wx[n] * wm;
+                met_lds[3 + v][tid] = ramp * 0.001293 * cd * a.mt.wy[n] * wm;
+                const double pr = ramp * a.mt.prn[n];
+                const double dpsidx = drpsi[v] * drdx + dspsi[v] * dsdx;
+                const double dpsidy = drpsi[v] * drdy + dspsi[v] * dsdy;
+                if (v == 0) { pdx = pr * dpsidx; pdy = pr * dpsidy; }
+                else { pdx = pdx + pr * dpsidx; pdy = pdy + pr * dpsidy; }
+            }
+            met_lds[6][tid] = pdx;
+            met_lds[7][tid] = pdy;
+        }
         const double depth_c = hb[0] + U[0][0];
         const double fh_nl_c = 1.0 / (ph.nleq * depth_c + ph.leq);
         const double cori = E.cori_el ? E.cori_el[e] : sc.cori;
@@ -449,8 +474,17 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
                 const double cap = 0.9 * sc.dtdph;
                 if (cap < tau) tau = cap;
             }
-            const double sx = -(tau * qx) + fg * dhbx * sfq + cori * qy;
-            const double sy = -(tau * qy) + fg * dhby - cori * qx;
+            double sx = -(tau * qx) + fg * dhbx * sfq + cori * qy;
+            double sy = -(tau * qy) + fg * dhby - cori * qx;
+            if (MET) {
+                // wind stress and atmospheric pressure (src/rhs_dg_hydro.F:362-373)
+                const double fw = 1.0 / f1;
+                const double* ps = T.psi[I];
+                sx = sx + fw * (met_lds[0][tid] * ps[0] + met_lds[1][tid] * ps[1] + met_lds[2][tid] * ps[2]) -
+                     ph.g * sfq * depth * met_lds[6][tid];
+                sy = sy + fw * (met_lds[3][tid] * ps[0] + met_lds[4][tid] * ps[1] + met_lds[5][tid] * ps[2]) -
+                     ph.g * depth * met_lds[7][tid];
+            }
             R[1][0] = R[1][0] + T.srfac[I][0] * sx;
             R[2][0] = R[2][0] + T.srfac[I][0] * sy;
 #pragma unroll
@@ -667,6 +701,39 @@ __global__ void __launch_bounds__(256) modal2nodal_kernel(int np, ElemData E, co
 
 // Device step counter used by the forcing-table lookups (a kernel, not a memset, so that the
 // sequence can be captured into a HIP graph).
+// ------------------------------------------------------------------------------------------
+// NWS=8 node kernel: HollandGet's per-node profile (src/wind.F:2062-2131) and the wind speed and
+// Garratt drag of MET_FORCING (src/met_forcing.F:488-500, src/wind.F:135-141), at TIME_A of the
+// current step.  The stage's ramp is applied in the stage kernel.
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) met_kernel(Met m, const int* __restrict__ step) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= m.np) return;
+    const int sid = m.nsid ? (int)m.nsid[n] : 0;
+    const double* r = m.row + ((size_t)sid * m.rows + *step) * MET_ROW;
+    const double re = 6378206.4;
+    const double deg2rad = 3.141592653589793 / 180.0;
+    const double lat = r[MR_LAT], lon = r[MR_LON], spd = r[MR_SPD], B = r[MR_B];
+    const double dx = (m.xc[n] - lon) * deg2rad, dy = (m.yc[n] - lat) * deg2rad;
+    const double theta = atan2(dy, dx);
+    const double s1 = sin(dy / 2.0), s2 = sin(dx / 2.0);
+    const double rad = re * (2.0 * asin(sqrt(s1 * s1 + r[MR_COSLAT] * m.cosy[n] * (s2 * s2))));
+    const double cor = m.cor[n];
+    const double rb = pow(r[MR_RMW] * 1000.0 / rad, B);
+    const double press = (r[MR_CPRESS] + r[MR_DEFICIT] * exp(-rb)) / r[MR_RHOG];
+    double vr = sqrt(rb * exp(1.0 - rb) * (spd * spd) + (rad * rad) * (cor * cor) / 4.0) - rad * cor / 2.0;
+    const double tsx = (fabs(vr) / spd) * r[MR_TVX], tsy = (fabs(vr) / spd) * r[MR_TVY];
+    vr = vr * r[MR_WMULT];
+    double wx = -(vr * sin(theta)), wy = vr * cos(theta);
+    wx = wx * r[MR_BLADJ]; wy = wy * r[MR_BLADJ];
+    wx = wx * 0.88; wy = wy * 0.88;
+    wx = wx + tsx; wy = wy + tsy;
+    const double wmag = sqrt(wx * wx + wy * wy);
+    double cd = 0.001 * (0.75 + 0.067 * wmag);
+    if (cd > 0.0035) cd = 0.0035;
+    m.wx[n] = wx; m.wy[n] = wy; m.wm[n] = wmag; m.cd[n] = cd; m.prn[n] = press;
+}
+
 __global__ void step_counter_kernel(int* __restrict__ step, int set, int value) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *step = set ? value : *step + 1;
 }

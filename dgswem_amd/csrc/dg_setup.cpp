@@ -10,6 +10,8 @@
 //   mass inverse       src/prep_DG.F:712-721
 //   RK_TIME            src/prep_DG.F:1551-2078
 //   CREATE_EDGE_DATA   src/create_edge_data.F:31-635
+//   Holland track      src/wind.F:315-348 (TIMECONV), 2153-2450 (GetHollandStormData),
+//                      1964-2061 (HollandGet's storm-wide scalars)
 #include "dg_setup.hpp"
 
 #include <cmath>
@@ -158,6 +160,12 @@ bool build_ref_tables(int p, RefTables& t, std::string& err) {
             t.mw[q][k] = t.m_inv[k] * t.wagp[q];
             t.srfac[q][k] = t.m_inv[k] * t.wagp[q] * t.phi_area[q][k];
         }
+    // linear nodal basis at the area points (src/prep_DG.F:940-942)
+    for (int q = 0; q < t.nagp; q++) {
+        t.psi[q][0] = -1.0 / 2.0 * (t.xagp[q] + t.yagp[q]);
+        t.psi[q][1] = 1.0 / 2.0 * (t.xagp[q] + 1.0);
+        t.psi[q][2] = 1.0 / 2.0 * (t.yagp[q] + 1.0);
+    }
     return true;
 }
 
@@ -289,6 +297,136 @@ bool create_edge_data(int np, int ne, const int* nm, int nope, const int* nvdll,
             for (int l = 0; l < 3; l++)
                 if (ed.neled[(size_t)e * 3 + l] == i) ed.nedsd[2 * (size_t)i + s] = l;
         }
+    return true;
+}
+
+namespace {
+
+// seconds since the start of the year, minutes and seconds zero
+double timeconv(int iyr, int imo, int iday, int ihr) {
+    double t = (double)((iday - 1) * 86400 + ihr * 3600);
+    const int ileap = (iyr / 4) * 4;
+    static const int add[12] = {0, 31, 0, 31, 30, 31, 30, 31, 31, 30, 31, 30};
+    for (int m = 2; m <= 12; m++) {
+        if (imo < m) break;
+        if (m == 3) t = t + (ileap == iyr ? 29 : 28) * 86400;
+        else t = t + add[m - 1] * 86400;
+    }
+    return t;
+}
+
+bool same_date(const int* a, const int* b) { return a[0] == b[0] && a[1] == b[1] && a[2] == b[2] && a[3] == b[3]; }
+
+}  // namespace
+
+bool build_holland_track(int nrec, const int* rec, const int* ref, int storm_number, double bladj,
+                         HollandTrack& t, std::string& err) {
+    const double re = 6378206.4;
+    const double pi = 4.0 * std::atan(1.0);
+    const double deg2rad = pi / 180.0;
+    if (nrec < 2) { err = "the storm track needs at least two records"; return false; }
+    const double ref_time = timeconv(ref[0], ref[1], ref[2], ref[3]);
+    t = HollandTrack{};
+    t.storm_number = storm_number;
+    t.bladj = bladj;
+    // kept records: the last non-repeated line is compared with the next
+    int prev_date[4] = {0, 0, 0, 0}, prev_inc = 0;
+    double prev_fcst = 0.0, prev_time = 0.0;
+    int now_cp = 0, now_rmw = 0, now_rrp = 0;
+    for (int r = 0; r < nrec; r++) {
+        const int* R = rec + 12 * r;
+        const int kept = (int)t.time.size();
+        int cp = R[9], rrp = R[10], rmw = R[11];
+        double time, fcst;
+        if (R[4] == 0) {                    // BEST: hindcast/nowcast
+            if (kept > 0 && same_date(R, prev_date)) continue;
+            now_cp = cp; now_rmw = rmw; now_rrp = rrp;
+            time = timeconv(R[0], R[1], R[2], R[3]) - ref_time;
+            fcst = R[5];
+        } else if (R[4] == 1) {             // OFCL: forecast from the last nowcast
+            if (kept == 0) { err = "the first storm track record must be a BEST (hindcast) line"; return false; }
+            if ((R[5] == 0 && same_date(R, prev_date)) || (R[5] != 0 && R[5] == prev_inc)) continue;
+            fcst = R[5];
+            if (R[5] == 0) {
+                time = timeconv(R[0], R[1], R[2], R[3]) - ref_time;
+            } else {
+                fcst = fcst * 3600.0;
+                time = prev_time + (fcst - prev_fcst);
+            }
+            cp = now_cp; rmw = now_rmw; rrp = now_rrp;
+        } else {
+            err = "invalid storm track record type (valid: BEST hindcast, OFCL forecast)";
+            return false;
+        }
+        double lat = R[6], lon = R[7], spd = R[8];
+        double cpd = cp, rrpd = rrp, rmwd = rmw;
+        lat = lat / 10.0;
+        lon = lon / 10.0;
+        lon = -1.0 * lon;
+        cpd = cpd * 100.0;
+        rrpd = rrpd * 1.852000003180799 * 1000.0;
+        rmwd = rmwd * 1.852000003180799;
+        spd = spd * 0.51444444;
+        t.time.push_back(time); t.lat.push_back(lat); t.lon.push_back(lon); t.cpress.push_back(cpd);
+        t.spd.push_back(spd); t.rrp.push_back(rrpd); t.rmw.push_back(rmwd);
+        for (int k = 0; k < 4; k++) prev_date[k] = R[k];
+        prev_inc = R[5];
+        prev_fcst = fcst;
+        prev_time = time;
+    }
+    t.n = (int)t.time.size();
+    if (t.n < 2) { err = "the storm track needs at least two distinct times"; return false; }
+    t.tvx.assign(t.n, 0.0);
+    t.tvy.assign(t.n, 0.0);
+    for (int k = 1; k < t.n; k++) {
+        const double dt = t.time[k] - t.time[k - 1];
+        const double s = std::sin((t.lon[k] * deg2rad - t.lon[k - 1] * deg2rad) / 2.0);
+        t.tvx[k] = std::copysign((re * (2.0 * std::asin(std::sqrt(std::cos(t.lat[k] * deg2rad) *
+                                                                 std::cos(t.lat[k - 1] * deg2rad) * (s * s))))) / dt,
+                                 t.lon[k] - t.lon[k - 1]);
+        const double q = std::sin((t.lat[k] * deg2rad - t.lat[k - 1] * deg2rad) / 2.0);
+        t.tvy[k] = std::copysign((re * (2.0 * std::asin(std::sqrt(q * q)))) / dt, t.lat[k] - t.lat[k - 1]);
+    }
+    t.tvx[0] = t.tvx[1];
+    t.tvy[0] = t.tvy[1];
+    return true;
+}
+
+bool holland_row(const HollandTrack& t, double time, double g, double* row, std::string& err) {
+    // the interval [CastTime(i-1), CastTime(i)) holding the time; the reference finds it once and
+    // then advances one record per call, which is the same interval while a time step is shorter
+    // than the record spacing
+    int i = 1;
+    while (i < t.n && !(time >= t.time[i - 1] && time < t.time[i])) i++;
+    if (i >= t.n) {
+        err = "the storm track (fort.22) does not cover model time " + std::to_string(time) + " s";
+        return false;
+    }
+    const double w = (time - t.time[i - 1]) / (t.time[i] - t.time[i - 1]);
+    const double cpress = t.cpress[i - 1] + w * (t.cpress[i] - t.cpress[i - 1]);
+    const double lon = t.lon[i - 1] + w * (t.lon[i] - t.lon[i - 1]);
+    const double lat = t.lat[i - 1] + w * (t.lat[i] - t.lat[i - 1]);
+    double spd = t.spd[i - 1] + w * (t.spd[i] - t.spd[i - 1]);
+    const double rmw = t.rmw[i - 1] + w * (t.rmw[i] - t.rmw[i - 1]);
+    const double tvx = t.tvx[i - 1] + w * (t.tvx[i] - t.tvx[i - 1]);
+    const double tvy = t.tvy[i - 1] + w * (t.tvy[i] - t.tvy[i - 1]);
+    double deficit = 101300.0 - cpress;
+    if (deficit < 100.0) deficit = 100.0;
+    const double ts = std::sqrt(tvx * tvx + tvy * tvy);
+    spd = spd - ts;
+    spd = spd / t.bladj;
+    double B = 1.15 * 2.718281828459045 * (spd * spd) / deficit;
+    if (B < 1.0) B = 1.0;
+    if (B > 2.5) B = 2.5;
+    const double pi = 4.0 * std::atan(1.0);
+    const double deg2rad = pi / 180.0;
+    for (int k = 0; k < MET_ROW; k++) row[k] = 0.0;
+    row[MR_LAT] = lat; row[MR_LON] = lon; row[MR_CPRESS] = cpress; row[MR_DEFICIT] = deficit;
+    row[MR_SPD] = spd; row[MR_RMW] = rmw; row[MR_TVX] = tvx; row[MR_TVY] = tvy; row[MR_B] = B;
+    row[MR_COSLAT] = std::cos(lat * deg2rad);
+    row[MR_WMULT] = t.storm_number == 2 ? 1.2 : 1.0;
+    row[MR_BLADJ] = t.bladj;
+    row[MR_RHOG] = 1000.0 * g;
     return true;
 }
 

@@ -25,6 +25,7 @@ struct RefTables {
     double m_inv[MAXDOF]{};
     double mw[MAXAGP][MAXDOF]{};            // M_INV(K)*WAGP(I): XFAC = mw*dphi/dx
     double srfac[MAXAGP][MAXDOF]{};         // M_INV(K)*WAGP(I)*PHI_AREA(K,I)
+    double psi[MAXAGP][3]{};                // linear nodal basis PSI1-3 at the area points
 };
 bool build_ref_tables(int p, RefTables& t, std::string& err);
 
@@ -47,5 +48,27 @@ struct EdgeData {
 bool create_edge_data(int np, int ne, const int* nm_colmajor_1based, int nope, const int* nvdll,
                       const int* nbdv, int nbou, const int* ibtype, const int* nvell,
                       const int* nbvv, EdgeData& ed, std::string& err);
+
+// NWS=8 storm track after the first-call processing of GetHollandStormData
+// (src/wind.F:2185-2410): repeated lines dropped, forecasts given the nowcast's pressure and radii,
+// SI units, cast times relative to the wind reference time, translation velocities.
+struct HollandTrack {
+    int n = 0, storm_number = 0;
+    double bladj = 1.0;
+    std::vector<double> time, lat, lon, cpress, spd, rrp, rmw, tvx, tvy;
+};
+// rec: [nrec][12] year month day hour cast(0 BEST, 1 OFCL) fcstinc lat(0.1N) lon(0.1W) spd(kt)
+// cpress(mbar) rrp(nm) rmw(nm); ref: IREFYR IREFMO IREFDAY IREFHR.
+bool build_holland_track(int nrec, const int* rec, const int* ref, int storm_number, double bladj,
+                         HollandTrack& t, std::string& err);
+
+// Per-step storm scalars for the node kernel: the track interpolated at TIME_A and the
+// node-independent part of HollandGet (src/wind.F:2020-2061).
+constexpr int MET_ROW = 16;
+enum {
+    MR_LAT, MR_LON, MR_CPRESS, MR_DEFICIT, MR_SPD, MR_RMW, MR_TVX, MR_TVY, MR_B, MR_COSLAT,
+    MR_WMULT, MR_BLADJ, MR_RHOG
+};
+bool holland_row(const HollandTrack& t, double time, double g, double* row, std::string& err);
 
 }  // namespace dg

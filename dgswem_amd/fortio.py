@@ -131,6 +131,8 @@ class Case:
     land_segs: List[Tuple[int, np.ndarray]]      # (IBTYPE, NBVV) per land segment (1-based)
     barrier_back: dict = field(default_factory=dict)  # segment index -> IBCONN (types 4/24/5/25)
     manning: Optional[np.ndarray] = None         # ManningsN per node when loaded from fort.13
+    storm: Optional["HollandStorm"] = None        # NWS=8 track and control line
+    lonlat: Optional[np.ndarray] = None          # [NP,2] fort.14 lon/lat in degrees (ICS=2)
 
     @property
     def ne(self) -> int:
@@ -307,9 +309,9 @@ def read_case(dirname: str) -> Case:
     p.ncor = f15.ints(1)
     p.ntip = f15.ints(1)
     p.nws = f15.ints(1)
-    if p.nws != 0:
-        raise NotImplementedError("NWS != 0 met forcing files are not read; use the parametric "
-                                  "vortex forcing of dgswem_amd.forcing instead")
+    if p.nws not in (0, 8):
+        raise NotImplementedError("only NWS=0 and NWS=8 (Holland vortex) meteorological forcing "
+                                  "are supported")
     p.nramp = f15.ints(1)
     if p.nramp > 1:
         raise NotImplementedError("NRAMP > 1")
@@ -318,6 +320,12 @@ def read_case(dirname: str) -> Case:
     p.dtdp = f15.reals(1)
     p.statim = f15.reals(1)
     p.reftim = f15.reals(1)
+    storm = None
+    if p.nws == 8:                                     # src/read_input.F:1082-1094
+        t = f15.values(6)
+        from .holland import HollandStorm, read_fort22
+        storm = HollandStorm(int(t[0]), int(t[1]), int(t[2]), int(t[3]), int(t[4]), _freal(t[5]))
+        storm.records = read_fort22(os.path.join(dirname, "fort.22"))
     p.rnday = f15.reals(1)
     p.nt = int(p.rnday * (86400.0 / p.dtdp) + 0.5)      # src/read_input.F:1156
     p.dramp = f15.reals(1)
@@ -334,6 +342,7 @@ def read_case(dirname: str) -> Case:
     p.slam0, p.sfea0 = slam0 * DEG2RAD, sfea0 * DEG2RAD
     x = np.empty(n_nodes); y = np.empty(n_nodes); dp = np.empty(n_nodes)
     slam = np.empty(n_nodes); sfea = np.empty(n_nodes)
+    lonlat = np.empty((n_nodes, 2)) if p.ics == 2 else None
     for i in range(n_nodes):
         t = f14.values(4)
         j = int(t[0]) - 1
@@ -344,15 +353,11 @@ def read_case(dirname: str) -> Case:
             slam[j] = p.slam0 + a / (R_EARTH * math.cos(p.sfea0))       # INVCP
             sfea[j] = b / R_EARTH
         else:
-            slam[j] = DEG2RAD * a
-            sfea[j] = DEG2RAD * b
-            x[j] = R_EARTH * (slam[j] - p.slam0) * math.cos(p.sfea0)   # CPP
-            y[j] = sfea[j] * R_EARTH
+            lonlat[j] = a, b
     if p.ics == 1:
         sfac = np.ones(n_nodes)
     else:
-        c0 = math.cos(p.sfea0)
-        sfac = np.array([c0 / math.cos(s) for s in sfea])
+        slam, sfea, x, y, sfac = spherical_coords(lonlat, p.slam0, p.sfea0)
     if p.nolifa in (0, 1):
         dp = np.where(dp < p.h0, p.h0, dp)
     nm = np.empty((ne, 3), dtype=np.int32)
@@ -383,7 +388,7 @@ def read_case(dirname: str) -> Case:
     if p.ncor == 0:
         corif = np.full(n_nodes, p.cori)
     else:
-        corif = np.array([2.0 * 7.29212e-5 * math.sin(s) for s in sfea])
+        corif = coriolis(sfea)
 
     # ---- tidal potential constituents (read and skipped: NTIP=0 on the configured paths)
     p_ntif = f15.ints(1)
@@ -477,7 +482,27 @@ def read_case(dirname: str) -> Case:
 
     return Case(title=title, params=p, dg=dg, x=x, y=y, slam=slam, sfea=sfea, dp=dp, sfac=sfac,
                 corif=corif, nm=nm, open_segs=open_segs, land_segs=land_segs, manning=manning,
-                barrier_back=barrier_back)
+                barrier_back=barrier_back, storm=storm, lonlat=lonlat)
+
+
+def spherical_coords(lonlat: np.ndarray, slam0: float, sfea0: float):
+    """ICS=2 node coordinates from fort.14 lon/lat degrees: radians, the CPP projection and
+    SFAC (src/read_input.F:1322-1348)."""
+    slam = np.empty(len(lonlat)); sfea = np.empty(len(lonlat))
+    x = np.empty(len(lonlat)); y = np.empty(len(lonlat)); sfac = np.empty(len(lonlat))
+    c0 = math.cos(sfea0)
+    for j, (a, b) in enumerate(lonlat):
+        slam[j] = DEG2RAD * float(a)
+        sfea[j] = DEG2RAD * float(b)
+        x[j] = R_EARTH * (slam[j] - slam0) * c0
+        y[j] = sfea[j] * R_EARTH
+        sfac[j] = c0 / math.cos(sfea[j])
+    return slam, sfea, x, y, sfac
+
+
+def coriolis(sfea: np.ndarray) -> np.ndarray:
+    """NCOR=1 Coriolis parameter per node (src/read_input.F:1825-1830)."""
+    return np.array([2.0 * 7.29212e-5 * math.sin(s) for s in sfea])
 
 
 BARRIER_TYPES = (3, 13, 23, 4, 24, 5, 25)
@@ -513,6 +538,8 @@ def write_fort14(path: str, case: Case, title: str = "grid") -> None:
         for i in range(case.np_):
             if case.params.ics == 1:
                 a, b = case.x[i], case.y[i]
+            elif case.lonlat is not None:
+                a, b = case.lonlat[i]
             else:
                 a, b = case.slam[i] / DEG2RAD, case.sfea[i] / DEG2RAD
             f.write(f"{i + 1} {float(a)!r} {float(b)!r} {float(case.dp[i])!r}\n")

@@ -22,13 +22,13 @@ _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
 # kernel ids of dg_gpu_kernel_stats
-K_STAGE, K_MINMAX, K_LIMIT, K_M2N = 0, 1, 2, 3
+K_STAGE, K_MINMAX, K_LIMIT, K_M2N, K_MET = 0, 1, 2, 3, 4
 
 # every symbol include/dgswem_gpu.h declares
 EXPORTED = ["dg_create_edge_data", "dg_gpu_init", "dg_gpu_set_tide", "dg_gpu_set_flow",
             "dg_gpu_set_state", "dg_gpu_get_state", "dg_gpu_step", "dg_gpu_synchronize",
             "dg_gpu_modal2nodal", "dg_gpu_mass", "dg_gpu_profile", "dg_gpu_kernel_stats",
-            "dg_ens_create", "dg_ens_layout", "dg_gpu_use_graph", "dg_gpu_autotune",
+            "dg_gpu_set_holland", "dg_holland_storm_at", "dg_ens_create", "dg_ens_layout", "dg_gpu_use_graph", "dg_gpu_autotune",
             "dg_gpu_set_block", "dg_gpu_last_error", "dg_gpu_finalize"]
 
 
@@ -71,6 +71,9 @@ def load_library(path: str = LIB_PATH):
     L.dg_gpu_profile.argtypes = [C.c_int, C.c_int]
     L.dg_gpu_kernel_stats.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_longlong), C.c_char_p, C.c_int]
+    L.dg_gpu_set_holland.argtypes = [C.c_int, C.c_int, _ip, _ip, C.c_int, C.c_double, _dp, _dp]
+    L.dg_holland_storm_at.argtypes = [C.c_int, _ip, _ip, C.c_int, C.c_double, C.c_double, C.c_int,
+                                      _dp, _dp]
     L.dg_ens_create.argtypes = [C.c_int, _ip, C.POINTER(C.c_int)]
     L.dg_ens_layout.argtypes = [C.c_int, C.POINTER(C.c_int), C.c_void_p, C.c_void_p]
     L.dg_gpu_use_graph.argtypes = [C.c_int, C.c_int]
@@ -165,6 +168,16 @@ def qnam_dg(case: Case) -> tuple:
         for L, (a, b) in enumerate(pairs):
             qa[i, L] = (p.qnam[i, a], p.qnam[i, b]); qp[i, L] = (p.qnph[i, a], p.qnph[i, b])
     return qa.reshape(-1, order="F"), qp.reshape(-1, order="F")
+
+
+def holland_storm_at(storm, g: float, times) -> np.ndarray:
+    """The library's storm scalars at model times (host code only; see dg_holland_storm_at)."""
+    t = _f(times)
+    rows = np.zeros((len(t), 16))
+    _check(load_library().dg_holland_storm_at(len(storm.records), _i(storm.records.reshape(-1)),
+                                              _i(storm.ref()), storm.storm_number,
+                                              float(storm.bladj), float(g), len(t), t, rows))
+    return rows
 
 
 class _Handle:
@@ -282,6 +295,13 @@ class GpuSolver(_Handle):
             qa, qp = qnam_dg(case)
             _check(L.dg_gpu_set_flow(self.h, p.nffr, _f(p.famig), _f(p.fff), _f(p.fface),
                                      _f(p.fper), _f(qa), _f(qp)))
+        if p.nws == 8:
+            st = case.storm
+            _check(L.dg_gpu_set_holland(self.h, len(st.records), _i(st.records.reshape(-1)),
+                                        _i(st.ref()), st.storm_number, float(st.bladj),
+                                        _f(case.slam), _f(case.sfea)))
+        elif p.nws != 0:
+            raise GpuError(f"NWS={p.nws} meteorological forcing is not supported")
         self.it = 0
 
 

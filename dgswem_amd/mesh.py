@@ -20,7 +20,7 @@ from typing import Optional
 
 import numpy as np
 
-from .fortio import DEG2RAD, PI, Case, DGOptions, Params
+from .fortio import DEG2RAD, PI, R_EARTH, Case, DGOptions, Params
 
 # Tidal constituent angular frequencies (rad/s): M2, S2, N2, K1, O1
 CONSTITUENTS = [("M2", 1.405189e-4), ("S2", 1.454441e-4), ("N2", 1.378797e-4),
@@ -147,7 +147,8 @@ def delaunay_basin(n_elem: int = 1_000_000, seed: int = 0, lx: float = 1.0e6, ly
                    depth_range=(5.0, 4000.0), shelf: bool = True, shelf_frac: float = 0.06,
                    land_height: float = 2.0, n_const: int = 5, p: int = 1, cfl: float = 0.3,
                    slopeflag: int = 5, wetdry: bool = True, nsteps: int = 2000,
-                   manning_range=(0.02, 0.03), refine: float = 3.0) -> Case:
+                   manning_range=(0.02, 0.03), refine: float = 3.0, ics: int = 1,
+                   lon_c: float = -75.03, lat_c: float = 27.97, storm=None) -> Case:
     """Seeded Delaunay basin of about `n_elem` triangles (see module docstring).
 
     The open boundary is x=0 (west); the coast x=lx (east) is land with a shelf that rises from the
@@ -155,6 +156,11 @@ def delaunay_basin(n_elem: int = 1_000_000, seed: int = 0, lx: float = 1.0e6, ly
     `shelf_frac` of the domain; north and south sides are land.  Point density grows by a factor
     `1+refine` toward the coast.  The time step is `cfl` times the smallest element inradius over
     the gravity wave speed of its deepest node.
+
+    With ics=2 the basin is placed on the sphere, centred at (lon_c, lat_c) degrees, which is also
+    the CPP projection centre; nodes are stored as fort.14 lon/lat and projected exactly as the
+    reference's reader does, and the Coriolis parameter varies with latitude (NCOR=1).  `storm`
+    (a holland.HollandStorm) switches on NWS=8 Holland vortex forcing.
     """
     from scipy.spatial import Delaunay
 
@@ -253,8 +259,8 @@ def delaunay_basin(n_elem: int = 1_000_000, seed: int = 0, lx: float = 1.0e6, ly
     rin = area2 / per
     hmax = np.maximum(np.max(dp[tri], axis=1), 0.5)
     dt = float(cfl * np.min(rin / np.sqrt(9.81 * hmax)))
-    prm = _params(ics=1, nolibf=1, nolifa=2 if wetdry else 1, nolica=1, nolicat=1, nwp=1, ncor=0,
-                  ntip=0, nws=0, nramp=1, g=9.81, tau0=0.005, dtdp=dt, statim=0.0, reftim=0.0,
+    prm = _params(ics=ics, nolibf=1, nolifa=2 if wetdry else 1, nolica=1, nolicat=1, nwp=1,
+                  ncor=1 if ics == 2 else 0, ntip=0, nws=8 if storm is not None else 0, nramp=1, g=9.81, tau0=0.005, dtdp=dt, statim=0.0, reftim=0.0,
                   rnday=nsteps * dt / 86400.0, dramp=0.5, h0=0.1, cf=0.0025, cori=0.0,
                   nt=nsteps, nbfr=n_const, amig=np.array([c[1] for c in const]),
                   ff=np.ones(n_const), face=np.zeros(n_const),
@@ -263,10 +269,62 @@ def delaunay_basin(n_elem: int = 1_000_000, seed: int = 0, lx: float = 1.0e6, ly
     dg = DGOptions(pl=p, ph=p, fluxtype=2, rk_stage=2, rk_order=2, slopeflag=slopeflag)
     if not wetdry:
         dp = np.where(dp < prm.h0, prm.h0, dp)
+    if ics == 2:
+        from .fortio import coriolis, spherical_coords
+        slam0, sfea0 = _exact_radians(lon_c), _exact_radians(lat_c)
+        prm.slam0, prm.sfea0 = slam0, sfea0
+        c0 = math.cos(sfea0)
+        lonlat = np.stack([lon_c + (X - 0.5 * lx) / (R_EARTH * c0) / DEG2RAD,
+                           lat_c + (Y - 0.5 * ly) / R_EARTH / DEG2RAD], axis=1)
+        slam, sfea, x, y, sfac = spherical_coords(lonlat, slam0, sfea0)
+        return Case(title=f"storm{len(nm)}", params=prm, dg=dg, x=x, y=y, slam=slam, sfea=sfea,
+                    dp=dp, sfac=sfac, corif=coriolis(sfea), nm=nm, open_segs=[open_seg],
+                    land_segs=[(0, land_seg)], manning=manning, storm=storm, lonlat=lonlat)
+    if storm is not None:
+        raise ValueError("Holland vortex forcing needs spherical coordinates (ics=2)")
     return Case(title=f"delaunay{len(nm)}", params=prm, dg=dg, x=X.copy(), y=Y.copy(),
                 slam=X / 6378206.4, sfea=Y / 6378206.4, dp=dp, sfac=np.ones(len(X)),
                 corif=np.zeros(len(X)), nm=nm, open_segs=[open_seg], land_segs=[(0, land_seg)],
                 manning=manning)
+
+
+def _exact_radians(deg: float) -> float:
+    """DEG2RAD*d for a d near `deg` whose radians survive the fort.15 round trip
+    (written as repr(r/DEG2RAD), read back as DEG2RAD*value)."""
+    d = float(deg)
+    for _ in range(64):
+        r = DEG2RAD * d
+        if DEG2RAD * float(repr(r / DEG2RAD)) == r:
+            return r
+        d = math.nextafter(d, math.inf)
+    raise RuntimeError("no round-trip-exact CPP centre found")
+
+
+def storm_basin(n_elem: int, seed: int, nsteps: int = 1000, p: int = 1, slopeflag: int = 5,
+                wetdry: bool = True, curved: Optional[bool] = None, **kw) -> Case:
+    """Spherical Delaunay basin (see delaunay_basin, ics=2) under a seeded Holland hurricane:
+    central pressure U(920, 990) mbar, radius to maximum wind U(20, 60) km, translation speed
+    U(4, 8) m/s toward a heading U(20, 100) degrees, on a straight track or one turning by
+    U(-40, 40) degrees a day.  At the run's start the eye is inside the basin's western
+    two thirds."""
+    from .holland import HollandStorm, synthetic_track
+    rng = np.random.default_rng(10_000 + seed)
+    lon_c, lat_c = kw.pop("lon_c", -75.03), kw.pop("lat_c", 27.97)
+    lx, ly = kw.get("lx", 1.0e6), kw.get("ly", 6.0e5)
+    half_lon = 0.5 * lx / (R_EARTH * math.cos(math.radians(lat_c))) / DEG2RAD
+    half_lat = 0.5 * ly / R_EARTH / DEG2RAD
+    lon = lon_c + half_lon * rng.uniform(-0.8, 0.3)
+    lat = lat_c + half_lat * rng.uniform(-0.6, 0.6)
+    if curved is None:
+        curved = bool(rng.integers(0, 2))
+    recs = synthetic_track(rng, lon, lat, heading_deg=rng.uniform(20.0, 100.0),
+                           speed_ms=rng.uniform(4.0, 8.0),
+                           turn_deg_per_day=rng.uniform(-40.0, 40.0) if curved else 0.0,
+                           pmin_mb=rng.uniform(920.0, 990.0), rmax_km=rng.uniform(20.0, 60.0),
+                           ref=(2020, 8, 25, 0))
+    storm = HollandStorm(2020, 8, 25, 0, 1, 0.9, recs)
+    return delaunay_basin(n_elem=n_elem, seed=seed, p=p, nsteps=nsteps, slopeflag=slopeflag,
+                          wetdry=wetdry, ics=2, lon_c=lon_c, lat_c=lat_c, storm=storm, **kw)
 
 
 # ----------------------------------------------------------------------------------------------
@@ -296,8 +354,14 @@ def write_run_dir(d: str, case: Case, dghot: int = 0, dghotspool: int = 0,
     a(str(1 if case.manning is not None else 0))
     if case.manning is not None:
         a("mannings_n_at_sea_floor")
-    a(str(p.ncor)); a("0"); a("0"); a(str(p.nramp))
-    a(_r(p.g)); a(_r(p.tau0)); a(_r(p.dtdp)); a(_r(p.statim)); a(_r(p.reftim)); a(_r(rnday))
+    a(str(p.ncor)); a("0"); a(str(p.nws)); a(str(p.nramp))
+    a(_r(p.g)); a(_r(p.tau0)); a(_r(p.dtdp)); a(_r(p.statim)); a(_r(p.reftim))
+    if p.nws == 8:
+        st = case.storm
+        a(f"{st.irefyr} {st.irefmo} {st.irefday} {st.irefhr} {st.storm_number} {_r(st.bladj)}")
+    elif p.nws != 0:
+        raise ValueError("only NWS=0 and NWS=8 run directories are written")
+    a(_r(rnday))
     a(_r(p.dramp)); a("0.35 0.30 0.35")
     a(_r(p.h0) if p.nolifa != 2 else f"{_r(p.h0)} 0 0 0.01")
     a(f"{_r(p.slam0 / DEG2RAD)} {_r(p.sfea0 / DEG2RAD)}")
@@ -325,10 +389,18 @@ def write_run_dir(d: str, case: Case, dghot: int = 0, dghotspool: int = 0,
             for j in range(p.qnam.shape[1]):
                 a(f"{_r(p.qnam[i, j])} {_r(p.qnph[i, j] / DEG2RAD)}")
     a("0 0.0 1.0 0"); a("0"); a("0 0.0 1.0 0"); a("0")
+    if p.nws != 0:
+        a("0 0.0 1.0 0"); a("0")                      # meteorological stations
     a(f"1 0.0 {_r(rnday)} {spool}")
-    a("0 0.0 1.0 0"); a("0"); a("0.0 0.0 0 0.0"); a("0 0 0 0"); a("0 0"); a("1 0 1.E-10 25")
+    a("0 0.0 1.0 0")                                  # global velocity output
+    if p.nws != 0:
+        a("0 0.0 1.0 0")                              # global wind output
+    a("0"); a("0.0 0.0 0 0.0"); a("0 0 0 0"); a("0 0"); a("1 0 1.E-10 25")
     with open(os.path.join(d, "fort.15"), "w") as f:
         f.write("\n".join(L) + "\n")
+    if p.nws == 8:
+        from .holland import write_fort22
+        write_fort22(os.path.join(d, "fort.22"), case.storm.records)
     if case.manning is not None:
         with open(os.path.join(d, "fort.13"), "w") as f:
             f.write("attrs\n" + f"{case.np_}\n1\nmannings_n_at_sea_floor\nunitless\n1\n0.0\n")

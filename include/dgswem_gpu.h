@@ -111,10 +111,31 @@ int dg_gpu_mass(int handle, double* mass);
 /* Kernel timing with HIP events on the handle's stream.  With enable=1 every kernel launch of
  * dg_gpu_step is bracketed by events; dg_gpu_kernel_stats returns the summed milliseconds and
  * launch count of kernel `kernel_id` (0 stage kernel, 1 node min/max, 2 limiter+wet/dry,
- * 3 modal2nodal) since the last reset.  No counterpart in the reference (it has no timers). */
+ * 3 modal2nodal, 4 NWS=8 node forcing) since the last reset.  No counterpart in the reference
+ * (it has no timers). */
 int dg_gpu_profile(int handle, int enable);
 int dg_gpu_kernel_stats(int handle, int kernel_id, double* total_ms, long long* launches,
                         char* name, int name_len);
+
+/* NWS=8 Holland (1980) vortex wind stress and atmospheric pressure.  Replaces GetHollandStormData
+ * (src/wind.F:2153-2450), HollandGet (src/wind.F:1964-2137), WindDrag (src/wind.F:116-141), the
+ * NWS=8 branch of MET_FORCING (src/met_forcing.F:479-520) and the wind/pressure source terms of
+ * RHS_DG_HYDRO (src/rhs_dg_hydro.F:362-373).  rec: nrec fort.22 records of 12 ints each, the
+ * FORMAT 14 fields in order (year, month, day, hour, cast type 0=BEST 1=OFCL, forecast increment
+ * h, latitude 0.1 deg N, longitude 0.1 deg W, max wind kt, central pressure mbar, RRP nm, RMW nm);
+ * ref: IREFYR IREFMO IREFDAY IREFHR; storm_number and bladj as on the fort.15 NWS=8 line; slam and
+ * sfea: node longitude and latitude in radians (SLAM, SFEA).  The profile is evaluated on the GPU
+ * once per step at TIME_A; a step whose TIME_A the track does not cover fails with an error. */
+int dg_gpu_set_holland(int handle, int nrec, const int* rec, const int* ref, int storm_number,
+                       double bladj, const double* slam, const double* sfea);
+
+/* Host-only view of the same track processing, for checking: the storm scalars used at each of
+ * ntimes model times (TIME_A, s) -- latitude, longitude (deg), central pressure and pressure
+ * deficit (Pa), boundary-layer maximum wind speed (m/s), radius to maximum wind (km), translation
+ * velocity (m/s, x and y), Holland B, cos(latitude), wind multiplier, BLAdj, RHOWAT0*G -- as
+ * rows[ntimes][16].  Touches no GPU. */
+int dg_holland_storm_at(int nrec, const int* rec, const int* ref, int storm_number, double bladj,
+                        double g, int ntimes, const double* times, double* rows);
 
 /* ---- Batched ensembles, graph replay and launch tuning.  None of these has a counterpart in the
  * reference, which runs one scenario per process: an ensemble replaces N separate dgswem_serial

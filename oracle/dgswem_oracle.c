@@ -108,6 +108,14 @@ typedef struct oracle {
     double timedg, rampdg;
     /* limiter scratch */
     double *zemin, *zemax, *qxmin, *qxmax, *qymin, *qymax;
+    /* NWS=8 Holland vortex: processed track, node coordinates and the stage's node forcing */
+    int nws, hl_n, hl_ptr, hl_first, storm_number;
+    double bladj, wind_ref_time;
+    double *hl_time, *hl_lat, *hl_lon, *hl_cpress, *hl_spd, *hl_rrp, *hl_rmw, *hl_tvx, *hl_tvy;
+    double *slam, *sfea;
+    double *wsx2, *wsy2, *pr2;
+    double drpsi[3], dspsi[3];
+    int met_error;
 } oracle_t;
 
 #define ZE(s, l, k) o->ze[((size_t)(s) * o->ne + (l)) * o->dofh + (k)]
@@ -587,6 +595,8 @@ int oracle_prep(oracle_t *o) {
         o->psi1[q] = -1.0 / 2.0 * (o->xagp[q] + o->yagp[q]);
         o->psi2[q] = 1.0 / 2.0 * (o->xagp[q] + 1.0);
         o->psi3[q] = 1.0 / 2.0 * (o->yagp[q] + 1.0);
+        o->drpsi[0] = -1.0 / 2.0; o->drpsi[1] = 1.0 / 2.0; o->drpsi[2] = 0.0;   /* src/prep_DG.F:948-953 */
+        o->dspsi[0] = -1.0 / 2.0; o->dspsi[1] = 0.0; o->dspsi[2] = 1.0 / 2.0;
     }
     o->edgeq = (double *)xcalloc(3 * (size_t)negp * dofh, sizeof(double));
     for (int l = 0; l < 3; l++) for (int q = 0; q < negp; q++) for (int k = 0; k < dofh; k++)
@@ -983,6 +993,201 @@ static void internal_edges(oracle_t *o) {
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* NWS=8 Holland vortex                                                                        */
+/* ------------------------------------------------------------------------------------------ */
+/* TIMECONV with IMIN=0, SEC=0: seconds since the start of the year (src/wind.F:315-348). */
+static double timeconv(int iyr, int imo, int iday, int ihr) {
+    double t = (double)((iday - 1) * 86400 + ihr * 3600 + 0 * 60) + 0.0;
+    int ileap = (iyr / 4) * 4;
+    if (imo >= 2) t = t + 31 * 86400;
+    if (ileap == iyr && imo >= 3) t = t + 29 * 86400;
+    if (ileap != iyr && imo >= 3) t = t + 28 * 86400;
+    if (imo >= 4) t = t + 31 * 86400;
+    if (imo >= 5) t = t + 30 * 86400;
+    if (imo >= 6) t = t + 31 * 86400;
+    if (imo >= 7) t = t + 30 * 86400;
+    if (imo >= 8) t = t + 31 * 86400;
+    if (imo >= 9) t = t + 31 * 86400;
+    if (imo >= 10) t = t + 30 * 86400;
+    if (imo >= 11) t = t + 31 * 86400;
+    if (imo == 12) t = t + 30 * 86400;
+    return t;
+}
+
+/* The track part of GetHollandStormData's first call (src/wind.F:2185-2410): drop repeated
+ * hindcast lines and repeated forecast increments, give forecasts the nowcast's pressure and radii,
+ * convert units and compute the translation velocities.  rec is [nrec][12] (see holland.py);
+ * ref = IREFYR IREFMO IREFDAY IREFHR.  Returns 0, or -1 for an invalid record sequence. */
+int oracle_set_holland(oracle_t *o, int nrec, const int *rec, const int *ref, int storm_number,
+                       double bladj, const double *slam, const double *sfea) {
+    const double re = 6378206.4;
+    double pi = 4.0 * atan(1.0);
+    double deg2rad = pi / 180.0;
+    o->nws = 8;
+    o->storm_number = storm_number;
+    o->bladj = bladj;
+    o->wind_ref_time = timeconv(ref[0], ref[1], ref[2], ref[3]);
+    o->slam = dup_d(slam, o->np);
+    o->sfea = dup_d(sfea, o->np);
+    int n = nrec > 0 ? nrec : 1;
+    int *iy = (int *)xcalloc(n, sizeof(int)), *im = (int *)xcalloc(n, sizeof(int));
+    int *id = (int *)xcalloc(n, sizeof(int)), *ih = (int *)xcalloc(n, sizeof(int));
+    int *ifc = (int *)xcalloc(n, sizeof(int));
+    int *icp = (int *)xcalloc(n, sizeof(int)), *irrp = (int *)xcalloc(n, sizeof(int));
+    int *irmw = (int *)xcalloc(n, sizeof(int));
+    double *fcst = (double *)xcalloc(n, sizeof(double));
+    o->hl_time = (double *)xcalloc(n, sizeof(double));
+    o->hl_lat = (double *)xcalloc(n, sizeof(double)); o->hl_lon = (double *)xcalloc(n, sizeof(double));
+    o->hl_cpress = (double *)xcalloc(n, sizeof(double)); o->hl_spd = (double *)xcalloc(n, sizeof(double));
+    o->hl_rrp = (double *)xcalloc(n, sizeof(double)); o->hl_rmw = (double *)xcalloc(n, sizeof(double));
+    o->hl_tvx = (double *)xcalloc(n, sizeof(double)); o->hl_tvy = (double *)xcalloc(n, sizeof(double));
+    int i = 0, pl = 0, now_cp = 0, now_rmw = 0, now_rrp = 0, bad = 0;
+    for (int r = 0; r < nrec && !bad; r++) {
+        const int *R = rec + 12 * r;
+        iy[i] = R[0]; im[i] = R[1]; id[i] = R[2]; ih[i] = R[3];
+        ifc[i] = R[5]; icp[i] = R[9]; irrp[i] = R[10]; irmw[i] = R[11];
+        int lat = R[6], lon = R[7], spd = R[8];
+        if (R[4] == 0) {            /* BEST */
+            if (i > 0 && iy[i] == iy[i - 1] && im[i] == im[i - 1] && id[i] == id[i - 1] && ih[i] == ih[i - 1])
+                continue;
+            now_cp = icp[i]; now_rmw = irmw[i]; now_rrp = irrp[i];
+            o->hl_time[i] = timeconv(iy[i], im[i], id[i], ih[i]) - o->wind_ref_time;
+            fcst[i] = ifc[i];
+        } else if (R[4] == 1) {     /* OFCL */
+            if (i == 0) { bad = 1; break; }
+            if ((ifc[i] == 0 && iy[i] == iy[i - 1] && im[i] == im[i - 1] && id[i] == id[i - 1] && ih[i] == ih[i - 1]) ||
+                (ifc[i] != 0 && ifc[i] == ifc[i - 1]))
+                continue;
+            fcst[i] = ifc[i];
+            if (ifc[i] == 0) {
+                o->hl_time[i] = timeconv(iy[i], im[i], id[i], ih[i]) - o->wind_ref_time;
+            } else {
+                fcst[i] = fcst[i] * 3600.0;
+                o->hl_time[i] = o->hl_time[i - 1] + (fcst[i] - fcst[i - 1]);
+            }
+            icp[i] = now_cp; irmw[i] = now_rmw; irrp[i] = now_rrp;
+        } else {
+            bad = 1; break;
+        }
+        o->hl_lat[i] = lat; o->hl_lon[i] = lon; o->hl_spd[i] = spd;
+        o->hl_cpress[i] = icp[i]; o->hl_rrp[i] = irrp[i]; o->hl_rmw[i] = irmw[i];
+        o->hl_lat[i] = o->hl_lat[i] / 10.0;
+        o->hl_lon[i] = o->hl_lon[i] / 10.0;
+        o->hl_lon[i] = -1.0 * o->hl_lon[i];
+        o->hl_cpress[i] = o->hl_cpress[i] * 100.0;
+        o->hl_rrp[i] = o->hl_rrp[i] * 1.852000003180799 * 1000.0;
+        o->hl_rmw[i] = o->hl_rmw[i] * 1.852000003180799;
+        o->hl_spd[i] = o->hl_spd[i] * 0.51444444;
+        pl = i + 1;
+        i++;
+    }
+    free(iy); free(im); free(id); free(ih); free(ifc); free(icp); free(irrp); free(irmw); free(fcst);
+    if (bad || pl < 2) return -1;
+    double *LAT = o->hl_lat, *LON = o->hl_lon, *T = o->hl_time;
+    for (int k = 1; k < pl; k++) {
+        double s = sin((LON[k] * deg2rad - LON[k - 1] * deg2rad) / 2.0);
+        o->hl_tvx[k] = copysign((re * (2.0 * asin(sqrt(cos(LAT[k] * deg2rad) * cos(LAT[k - 1] * deg2rad) * (s * s))))) /
+                                    (T[k] - T[k - 1]),
+                                LON[k] - LON[k - 1]);
+        double q = sin((LAT[k] * deg2rad - LAT[k - 1] * deg2rad) / 2.0);
+        o->hl_tvy[k] = copysign((re * (2.0 * asin(sqrt(q * q)))) / (T[k] - T[k - 1]), LAT[k] - LAT[k - 1]);
+    }
+    o->hl_tvx[0] = o->hl_tvx[1];
+    o->hl_tvy[0] = o->hl_tvy[1];
+    o->hl_n = pl;
+    o->hl_first = 1;
+    o->wsx2 = (double *)xcalloc(o->np, sizeof(double));
+    o->wsy2 = (double *)xcalloc(o->np, sizeof(double));
+    o->pr2 = (double *)xcalloc(o->np, sizeof(double));
+    return 0;
+}
+
+/* MET_FORCING for NWS=8 at TIME_A with RampMete = RAMPDG (src/met_forcing.F:479-520): the track
+ * interpolation of GetHollandStormData (src/wind.F:2363-2431, including its one-record-per-call
+ * pointer advance), HollandGet (src/wind.F:1964-2137) and the Garratt drag law WindDrag
+ * (src/wind.F:116-141). */
+/* Storm scalars at `time`: s = {lat, lon, cpress, deficit, spd (adjusted), rmw, tvx, tvy, B}.
+ * Returns nonzero when the track does not cover the time. */
+static int storm_at(oracle_t *o, double time, double *s) {
+    const double ambient = 101300.0, rho_air = 1.15, e = 2.718281828459045;
+    double *T = o->hl_time;
+    if (o->hl_first) {
+        int i = 1;
+        while (!(time >= T[i - 1] && time < T[i])) {
+            i++;
+            if (i >= o->hl_n) return 1;
+        }
+        o->hl_ptr = i;
+        o->hl_first = 0;
+    }
+    int i = o->hl_ptr;
+    if (time >= T[i]) i++;
+    if (i >= o->hl_n) return 1;
+    o->hl_ptr = i;
+    double w = (time - T[i - 1]) / (T[i] - T[i - 1]);
+    double cpress = o->hl_cpress[i - 1] + w * (o->hl_cpress[i] - o->hl_cpress[i - 1]);
+    double lon = o->hl_lon[i - 1] + w * (o->hl_lon[i] - o->hl_lon[i - 1]);
+    double lat = o->hl_lat[i - 1] + w * (o->hl_lat[i] - o->hl_lat[i - 1]);
+    double spd = o->hl_spd[i - 1] + w * (o->hl_spd[i] - o->hl_spd[i - 1]);
+    double rmw = o->hl_rmw[i - 1] + w * (o->hl_rmw[i] - o->hl_rmw[i - 1]);
+    double tvx = o->hl_tvx[i - 1] + w * (o->hl_tvx[i] - o->hl_tvx[i - 1]);
+    double tvy = o->hl_tvy[i - 1] + w * (o->hl_tvy[i] - o->hl_tvy[i - 1]);
+    double deficit = ambient - cpress;
+    if (deficit < 100.0) deficit = 100.0;
+    double ts = sqrt(tvx * tvx + tvy * tvy);
+    spd = spd - ts;
+    spd = spd / o->bladj;
+    double B = rho_air * e * (spd * spd) / deficit;
+    if (B < 1.0) B = 1.0;
+    if (B > 2.5) B = 2.5;
+    s[0] = lat; s[1] = lon; s[2] = cpress; s[3] = deficit; s[4] = spd; s[5] = rmw; s[6] = tvx;
+    s[7] = tvy; s[8] = B;
+    return 0;
+}
+
+/* Storm scalars at increasing times (test access to the track interpolation). */
+int oracle_storm_at(oracle_t *o, double time, double *s) { return storm_at(o, time, s); }
+
+static void met_forcing(oracle_t *o, double time) {
+    const double re = 6378206.4;
+    const double sampling = 0.88, rad2deg = 180.0 / 3.141592653589793;
+    double pi = 4.0 * atan(1.0);
+    double deg2rad = pi / 180.0;
+    double omega = 2.0 * pi / 86164.2;
+    double S[9];
+    if (storm_at(o, time, S)) { o->met_error = 1; return; }
+    double lat = S[0], lon = S[1], cpress = S[2], deficit = S[3], spd = S[4], rmw = S[5];
+    double tvx = S[6], tvy = S[7], B = S[8];
+    double wmult = o->storm_number == 2 ? 1.2 : 1.0;
+    double rampm = o->rampdg;
+    double rhog = 1000.0 * o->g;
+#pragma omp parallel for num_threads(o->nthreads) schedule(static)
+    for (int n = 0; n < o->np; n++) {
+        double xc = o->slam[n] * rad2deg, yc = o->sfea[n] * rad2deg;
+        double dx = (xc - lon) * deg2rad, dy = (yc - lat) * deg2rad;
+        double theta = atan2(dy, dx);
+        double s1 = sin(dy / 2.0), s2 = sin(dx / 2.0);
+        double rad = re * (2.0 * asin(sqrt(s1 * s1 + cos(lat * deg2rad) * cos(yc * deg2rad) * (s2 * s2))));
+        double cor = 2.0 * omega * sin(yc * deg2rad);
+        double rb = pow(rmw * 1000.0 / rad, B);
+        double press = (cpress + deficit * exp(-rb)) / rhog;
+        double vr = sqrt(rb * exp(1.0 - rb) * (spd * spd) + (rad * rad) * (cor * cor) / 4.0) - rad * cor / 2.0;
+        double tsx = (fabs(vr) / spd) * tvx, tsy = (fabs(vr) / spd) * tvy;
+        vr = vr * wmult;
+        double wx = -(vr * sin(theta)), wy = vr * cos(theta);
+        wx = wx * o->bladj; wy = wy * o->bladj;
+        wx = wx * sampling; wy = wy * sampling;
+        wx = wx + tsx; wy = wy + tsy;
+        double wmag = sqrt(wx * wx + wy * wy);
+        double cd = 0.001 * (0.75 + 0.067 * wmag);
+        if (cd > 0.0035) cd = 0.0035;
+        o->wsx2[n] = rampm * 0.001293 * cd * wx * wmag;
+        o->wsy2[n] = rampm * 0.001293 * cd * wy * wmag;
+        o->pr2[n] = rampm * press;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* volume RHS                                                                                  */
 /* ------------------------------------------------------------------------------------------ */
 static void rhs_volume(oracle_t *o) {
@@ -1040,6 +1245,19 @@ static void rhs_volume(oracle_t *o) {
             }
             double sx = -(tau * qx_in) + fg * dhb_x * sfq + o->cori_el[L] * qy_in;
             double sy = -(tau * qy_in) + fg * dhb_y - o->cori_el[L] * qx_in;
+            if (o->nws != 0) {
+                /* wind stress and pressure gradient (src/rhs_dg_hydro.F:362-373) */
+                double dpsidx[3], dpsidy[3];
+                for (int k = 0; k < 3; k++) {
+                    dpsidx[k] = o->drpsi[k] * o->drdx[L] + o->dspsi[k] * o->dsdx[L];
+                    dpsidy[k] = o->drpsi[k] * o->drdy[L] + o->dspsi[k] * o->dsdy[L];
+                }
+                double fw = 1.0 / f1;
+                sx = sx + fw * (o->wsx2[n1] * o->psi1[I] + o->wsx2[n2] * o->psi2[I] + o->wsx2[n3] * o->psi3[I]) -
+                     o->g * sfq * depth * (o->pr2[n1] * dpsidx[0] + o->pr2[n2] * dpsidx[1] + o->pr2[n3] * dpsidx[2]);
+                sy = sy + fw * (o->wsy2[n1] * o->psi1[I] + o->wsy2[n2] * o->psi2[I] + o->wsy2[n3] * o->psi3[I]) -
+                     o->g * depth * (o->pr2[n1] * dpsidy[0] + o->pr2[n2] * dpsidy[1] + o->pr2[n3] * dpsidy[2]);
+            }
             size_t t0 = ((size_t)L * nagp + I) * dofh;
             RQX(irk, L, 0) = RQX(irk, L, 0) + o->srfac[t0] * sx;
             RQY(irk, L, 0) = RQY(irk, L, 0) + o->srfac[t0] * sy;
@@ -1254,6 +1472,7 @@ static void stage_time(oracle_t *o, int it, int irk) {
 void oracle_rhs(oracle_t *o, int it, int irk) {
     o->irk = irk;
     stage_time(o, it, irk);
+    if (o->nws == 8) met_forcing(o, it * o->dtdp + o->statim * 86400.0);
     if (o->needs > 0) ocean_edges(o);
     if (o->nleds > 0) land_edges(o);
     if (o->nfeds > 0) flow_edges(o);
@@ -1383,6 +1602,16 @@ void oracle_get_tables(const oracle_t *o, double *phi_area, double *phi_edge, do
     for (int q = 0; q < o->nagp; q++) { xagp[q] = o->xagp[q]; yagp[q] = o->yagp[q]; wagp[q] = o->wagp[q]; }
     for (int i = 0; i < o->nrk; i++) { dtvd[i] = o->dtvd[i]; for (int j = 0; j < o->nrk; j++) { atvd[i * o->nrk + j] = o->atvd[i][j]; btvd[i * o->nrk + j] = o->btvd[i][j]; } }
 }
+/* Nonzero once a met-forcing time fell outside the track (the reference STOPs there). */
+int oracle_met_error(const oracle_t *o) { return o->met_error; }
+
+/* The stage's node forcing WSX2, WSY2, PR2 (after the last oracle_rhs). */
+void oracle_get_met(const oracle_t *o, double *wsx2, double *wsy2, double *pr2) {
+    memcpy(wsx2, o->wsx2, o->np * sizeof(double));
+    memcpy(wsy2, o->wsy2, o->np * sizeof(double));
+    memcpy(pr2, o->pr2, o->np * sizeof(double));
+}
+
 double oracle_mass(const oracle_t *o) {
     /* sum_e A_e * (ze_1 + hb_1): total water volume (mode 1 is the cell mean) */
     double m = 0.0;
@@ -1392,7 +1621,9 @@ double oracle_mass(const oracle_t *o) {
 
 void oracle_free(oracle_t *o) {
     if (!o) return;
-    void *ptrs[] = {o->x, o->y, o->dp, o->sfac, o->corif, o->manning, o->nm, o->open_len, o->open_nodes,
+    void *ptrs[] = {o->hl_time, o->hl_lat, o->hl_lon, o->hl_cpress, o->hl_spd, o->hl_rrp, o->hl_rmw,
+                    o->hl_tvx, o->hl_tvy, o->slam, o->sfea, o->wsx2, o->wsy2, o->pr2,
+                    o->x, o->y, o->dp, o->sfac, o->corif, o->manning, o->nm, o->open_len, o->open_nodes,
                     o->land_type, o->land_len, o->land_nodes, o->amig, o->ff, o->face, o->per, o->emo, o->efa,
                     o->famig, o->fff, o->fface, o->fper, o->qnam, o->qnph, o->nedno, o->nedel, o->nedsd,
                     o->neled, o->niedn, o->nledn, o->needn, o->nfedn, o->elem_edge, o->xlen, o->cosnx,

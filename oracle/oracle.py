@@ -51,6 +51,13 @@ def lib():
         L.oracle_mass.argtypes = [C.c_void_p]
         L.oracle_mass.restype = C.c_double
         L.oracle_free.argtypes = [C.c_void_p]
+        L.oracle_set_holland.argtypes = [C.c_void_p, C.c_int, _ip, _ip, C.c_int, C.c_double, _dp, _dp]
+        L.oracle_set_holland.restype = C.c_int
+        L.oracle_met_error.argtypes = [C.c_void_p]
+        L.oracle_met_error.restype = C.c_int
+        L.oracle_get_met.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.oracle_storm_at.argtypes = [C.c_void_p, C.c_double, _dp]
+        L.oracle_storm_at.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -104,6 +111,13 @@ class Oracle:
         rc = L.oracle_prep(self.h)
         if rc != 0:
             raise RuntimeError("oracle_prep failed")
+        if p.nws == 8:
+            st = case.storm
+            rc = L.oracle_set_holland(self.h, len(st.records), _i(st.records.reshape(-1)),
+                                      _i(st.ref()), st.storm_number, st.bladj, _f(case.slam),
+                                      _f(case.sfea))
+            if rc != 0:
+                raise RuntimeError("invalid fort.22 track (first record must be BEST, >= 2 times)")
         self.it = 0
 
     def __del__(self):
@@ -114,6 +128,22 @@ class Oracle:
     def step(self, n: int = 1):
         lib().oracle_run(self.h, self.it + 1, n)
         self.it += n
+        if lib().oracle_met_error(self.h):
+            raise RuntimeError("the storm track does not cover the model time")
+
+    def storm_at(self, time: float):
+        """Storm scalars {lat, lon, cpress, deficit, spd, rmw, tvx, tvy, B} at `time` (call with
+        increasing times: the track pointer advances like the reference's)."""
+        out = np.empty(9)
+        if lib().oracle_storm_at(self.h, float(time), out):
+            raise RuntimeError("the storm track does not cover the time")
+        return out
+
+    def met(self):
+        """WSX2, WSY2, PR2 node arrays of the last stage (NWS=8)."""
+        a, b, c = np.empty(self.np_), np.empty(self.np_), np.empty(self.np_)
+        lib().oracle_get_met(self.h, a, b, c)
+        return a, b, c
 
     def state(self):
         ze = np.empty(self.ne * self.dofh); qx = np.empty_like(ze); qy = np.empty_like(ze)

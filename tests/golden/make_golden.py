@@ -76,11 +76,12 @@ def generated_case(name: str, case, steps) -> None:
         case.params.nt = n
         with tempfile.TemporaryDirectory() as d:
             mesh.write_run_dir(d, case, dghot=1, dghotspool=n, nspoolge=n)
-            if name.startswith("delaunay") and n == steps[0]:
+            if name.startswith(("delaunay", "storm")) and n == steps[0]:
                 keep = os.path.join(HERE, name)
                 os.makedirs(keep, exist_ok=True)
-                for f in ("fort.14", "fort.15", "fort.13", "fort.dg"):
-                    shutil.copy(os.path.join(d, f), keep)
+                for f in ("fort.14", "fort.15", "fort.13", "fort.dg", "fort.22"):
+                    if os.path.exists(os.path.join(d, f)):
+                        shutil.copy(os.path.join(d, f), keep)
             run_ref(d)
             dump(d, fortio.read_case(d), name, n)
 
@@ -110,6 +111,17 @@ def reference_case(name: str, steps) -> None:
             dump(d, fortio.read_case(d), name, n)
 
 
+def storm_case():
+    """NWS=8 Holland vortex over a 300 km x 200 km spherical basin with wetting and drying; the
+    track mixes repeated BEST lines with an OFCL advisory (repeated increments included), so the
+    run uses a hindcast-to-forecast interval with the nowcast's pressure and radii."""
+    from dgswem_amd.holland import forecast_track
+    case = mesh.storm_basin(3000, seed=3, nsteps=500, lx=3.0e5, ly=2.0e5, depth_range=(5.0, 200.0))
+    case.params.dramp = 0.02
+    case.storm.records = forecast_track(case.storm.records, issue=3)
+    return case
+
+
 def main() -> None:
     if not os.path.exists(BIN):
         raise SystemExit(f"{BIN} missing: run oracle/build_ref.sh first")
@@ -123,6 +135,7 @@ def main() -> None:
     generated_case("delaunay4k", mesh.delaunay_basin(n_elem=4000, seed=1, lx=6.0e4, ly=3.6e4,
                                                      depth_range=(2.0, 30.0), cfl=0.3),
                    [100, 1000])
+    generated_case("storm3k", storm_case(), [100, 500])
     reference_case("quarter_annular", [100, 2000])
     reference_case("wetdry", [50, 500])
     reference_case("mass_conservation", [1000])

@@ -157,3 +157,40 @@ def test_ensemble_errors():
     ens.close()
     for h in (a, b, a2):
         h.close()
+
+
+def test_storm_ensemble_matches_single_runs_bitwise():
+    """Members under their own Holland hurricanes (NWS=8, spherical, variable Coriolis): the node
+    forcing kernel runs once per step over all members' nodes."""
+    cases = [load_case("storm3k")]
+    for ne, seed in ((2000, 21), (5000, 22)):
+        cases.append(mesh.storm_basin(ne, seed=seed, nsteps=60, lx=3.0e5, ly=2.0e5,
+                                      depth_range=(5.0, 200.0)))
+    steps = 60
+    solo = []
+    for c in cases:
+        g = gpu.GpuSolver(c)
+        g.step(steps)
+        solo.append(g.state())
+        g.close()
+    ens = gpu.Ensemble([gpu.GpuSolver(c) for c in cases])
+    ens.use_graph(True)
+    ens.step(steps)
+    for i, (p, s) in enumerate(zip(ens.split(ens.state()), solo)):
+        print(f"storm member {i}", flush=True)
+        assert_same(p, s)
+    ens.close()
+
+
+def test_storm_ensemble_rejects_mixed_forcing():
+    a = gpu.GpuSolver(load_case("storm3k"))
+    b = gpu.GpuSolver(mesh.storm_basin(2000, seed=21, nsteps=10, lx=3.0e5, ly=2.0e5,
+                                       depth_range=(5.0, 200.0)))
+    c = mesh.delaunay_basin(n_elem=2000, seed=21, nsteps=10, lx=3.0e5, ly=2.0e5,
+                            depth_range=(5.0, 200.0), ics=2)
+    nc = gpu.GpuSolver(c)
+    with pytest.raises(gpu.GpuError, match="NWS=8"):
+        gpu.Ensemble([a, nc])
+    gpu.Ensemble([a, b]).close()
+    for h in (a, b, nc):
+        h.close()
