@@ -1,0 +1,118 @@
+"""Gauge-misfit loss and its gradient through the library's discrete adjoint, as a
+torch.autograd.Function.
+
+    loss = GaugeMisfit.apply(manning, dp, problem)
+    loss.backward()          # manning.grad, dp.grad
+
+`GaugeMisfit.forward` runs the forward model on the GPU (dg_adj_gauges) and evaluates
+J = weight * sum_k sum_g (eta_g(U^k) - d_g^k)^2; `backward` runs dg_adj_gradient: a checkpointed
+forward sweep and the hand-written adjoint kernels of the stage RHS and SSP-RK stages.  Both start
+from the problem's initial state, which is restored before every evaluation.  Tensors may live on
+any device; the library takes host copies.  torch is only the autograd front end here.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import gpu
+from .fortio import Case
+
+
+@dataclass
+class MisfitProblem:
+    """A run (its initial state), gauges, the observed gauge series and the loss weight."""
+    solver: gpu.GpuSolver
+    nsteps: int
+    data: np.ndarray                       # [nsteps, ng]
+    weight: float = 1.0
+    interval: int = 0                      # checkpoint spacing (0: about sqrt(nsteps))
+    state0: tuple = field(default=None)    # initial (ze, qx, qy, wdflg)
+    it0: int = 0
+
+    def reset(self) -> None:
+        ze, qx, qy, wd = self.state0
+        self.solver.set_state(ze, qx, qy, wd)
+        self.solver.it = self.it0
+
+    def loss(self, manning: np.ndarray, dp: np.ndarray) -> float:
+        s = self.solver
+        s.set_manning(manning); s.set_depth(dp)
+        self.reset()
+        eta = s.gauges(self.nsteps)
+        self.reset()
+        d = eta - self.data
+        return float(self.weight * np.sum(d * d))
+
+    def gradient(self, manning: np.ndarray, dp: np.ndarray):
+        s = self.solver
+        s.set_manning(manning); s.set_depth(dp)
+        self.reset()
+        return s.gradient(self.data, self.weight, self.interval)
+
+
+def seeded_gauges(case: Case, ng: int, seed: int):
+    """ng gauge points: seeded elements and seeded points inside them (reference coordinates)."""
+    rng = np.random.default_rng(seed)
+    elem = rng.choice(case.ne, size=ng, replace=False) + 1
+    a = rng.uniform(0.1, 0.8, ng); b = rng.uniform(0.1, 0.9, ng) * (1.0 - a)
+    r = -1.0 + 2.0 * a; s = -1.0 + 2.0 * b
+    return elem.astype(np.int32), r, s
+
+
+def twin_problem(case: Case, nsteps: int, ng: int = 50, seed: int = 0, perturb: float = 0.2,
+                 weight: float | None = None, interval: int = 0) -> MisfitProblem:
+    """Gauge data from a twin run whose Manning field is perturbed by a seeded smooth relative
+    factor (1 + perturb * field), then a problem set up at the unperturbed parameters."""
+    s = gpu.GpuSolver(case)
+    elem, r, sr = seeded_gauges(case, ng, seed)
+    s.set_gauges(elem, r, sr)
+    state0 = s.state()
+    rng = np.random.default_rng(seed + 1)
+    kx, ky, ph = rng.uniform(1.0, 3.0), rng.uniform(1.0, 3.0), rng.uniform(0.0, 2 * np.pi)
+    x = (case.x - case.x.min()) / np.ptp(case.x); y = (case.y - case.y.min()) / np.ptp(case.y)
+    f = np.sin(2 * np.pi * kx * x + ph) * np.cos(2 * np.pi * ky * y)
+    s.set_manning(case.manning * (1.0 + perturb * f))
+    data = s.gauges(nsteps)
+    s.set_manning(case.manning)
+    w = weight if weight is not None else 1.0 / (nsteps * ng)
+    p = MisfitProblem(solver=s, nsteps=nsteps, data=data, weight=w, interval=interval,
+                      state0=state0, it0=0)
+    p.reset()
+    return p
+
+
+def _make_function():
+    import torch
+
+    class GaugeMisfit(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, manning, dp, problem: MisfitProblem):
+            mn = manning.detach().to("cpu", torch.float64).numpy()
+            d = dp.detach().to("cpu", torch.float64).numpy()
+            ctx.problem = problem
+            ctx.params = (mn.copy(), d.copy())
+            ctx.devices = (manning.device, dp.device)
+            return torch.tensor(problem.loss(mn, d), dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            mn, d = ctx.params
+            _, gn, gdp = ctx.problem.gradient(mn, d)
+            g = float(grad_out)
+            return (torch.from_numpy(g * gn).to(ctx.devices[0]),
+                    torch.from_numpy(g * gdp).to(ctx.devices[1]), None)
+
+    return GaugeMisfit
+
+
+_FN = None
+
+
+def gauge_misfit(manning, dp, problem: MisfitProblem):
+    """torch front end: differentiable loss of torch float64 tensors `manning` and `dp` (per node)."""
+    global _FN
+    if _FN is None:
+        _FN = _make_function()
+    return _FN.apply(manning, dp, problem)

@@ -1,0 +1,589 @@
+// Discrete adjoint of the DG hydro step (gfx950, FP64): the reverse of the stage RHS with respect
+// to the state, the per-node Manning n and the per-node depth DP, plus the gauge-misfit loss.
+//
+// The reference has no adjoint; this differentiates the operations of its explicit step as this
+// library evaluates them (OCEAN/LAND/FLOW/INTERNAL_EDGE_HYDRO with the LLF flux
+// src/numerical_flux.F:750-1022, RHS_DG_HYDRO src/rhs_dg_hydro.F:18-473, the SSP-RK update
+// src/DG_hydro_timestep.F:238-290).  Non-smooth operations take the branch the forward values
+// select: the LLF wave-speed bound differentiates through the largest of its six candidates, the
+// Manning friction floor CF and the 0.9/DT friction cap pass no derivative when active, and
+// Eirik's dry-ocean-edge fix differentiates |h|.  The configurations differentiated are those
+// without the slope limiter and without wetting and drying (every element wet).
+//
+// Kernel layout follows the forward stage kernel: one thread per element, gathers only.  The
+// cotangent of an internal edge flux is the sum of what both neighbours' RHS cotangents ask of it;
+// each element forms that sum from its own and its neighbour's cotangents and keeps the part of
+// the flux derivative that belongs to its own trace, so no scatter or atomics are needed.
+#include "dg_device.hpp"
+
+namespace dg {
+
+// Reverse of llf_flux: given fb = d(loss)/d(f0,f1,f2), returns the cotangents of
+// (ze_in, ze_ex, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_ex) in bar[0..7].
+__device__ __forceinline__ void llf_flux_adj(const Phys& P, double zi, double ze, double xi, double xe,
+                                             double yi, double ye, double hi, double he, double sfi,
+                                             double sfe, double nx, double ny, double G,
+                                             const double fb[3], double bar[8]) {
+    const double NLEQ = P.nleq, LEQ = P.leq, FG_L = P.fg_l;
+    const double NLEQG = G * NLEQ;
+    // ---- forward values needed by the reverse sweep
+    const double j1 = ze - zi, j2 = xe - xi, j3 = ye - yi;
+    const double side_z[2] = {zi, ze}, side_x[2] = {xi, xe}, side_y[2] = {yi, ye}, side_h[2] = {hi, he};
+    const double side_s[2] = {sfi, sfe};
+    double ht[2], fh[2], u[2], v[2], un[2], c[2], K[2];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        ht[s] = side_z[s] * NLEQ + side_h[s];
+        fh[s] = 1.0 / (NLEQ * ht[s] + LEQ);
+        u[s] = side_x[s] * fh[s];
+        v[s] = side_y[s] * fh[s];
+        un[s] = (u[s] * nx + v[s] * ny) * NLEQ;
+        const double nxs = nx * side_s[s];
+        K[s] = ny * ny + nxs * nxs;
+        c[s] = sqrt(G * ht[s] * K[s]);
+    }
+    // largest wave-speed candidate, in the forward's comparison order
+    const double cand[6] = {fabs(un[0] + c[0]), fabs(un[0]), fabs(un[0] - c[0]),
+                            fabs(un[1] + c[1]), fabs(un[1]), fabs(un[1] - c[1])};
+    int arg = 0;
+    double em = cand[0];
+#pragma unroll
+    for (int q = 1; q < 6; q++)
+        if (cand[q] > em) { em = cand[q]; arg = q; }
+    const double nx2 = NLEQ + LEQ * nx * nx, ny2 = NLEQ + LEQ * ny * ny, nxy = nx * ny;
+    // ---- reverse
+    const double a1b = fb[0], a2b = fb[1], a3b = fb[2];
+    const double emb = -0.5 * j1 * fb[0] - 0.5 * (j2 * nx2 + LEQ * j3 * nxy) * fb[1] -
+                       0.5 * (j3 * ny2 + LEQ * j2 * nxy) * fb[2];
+    const double j1b = -0.5 * em * fb[0];
+    const double j2b = -0.5 * em * (nx2 * fb[1] + LEQ * nxy * fb[2]);
+    const double j3b = -0.5 * em * (LEQ * nxy * fb[1] + ny2 * fb[2]);
+    double unb[2] = {0.0, 0.0}, cb[2] = {0.0, 0.0};
+    {
+        const int s = arg / 3, q = arg % 3;
+        const double val = q == 0 ? un[s] + c[s] : (q == 1 ? un[s] : un[s] - c[s]);
+        const double sg = val > 0.0 ? 1.0 : (val < 0.0 ? -1.0 : 0.0);
+        unb[s] += sg * emb;
+        if (q == 0) cb[s] += sg * emb;
+        else if (q == 2) cb[s] -= sg * emb;
+    }
+    // flux averages: a = 0.5*((f_in + f_ex).n)
+    const double fx1b = 0.5 * nx * a1b, fy1b = 0.5 * ny * a1b;
+    const double fx2b = 0.5 * nx * a2b, fy2b = 0.5 * ny * a2b;
+    const double fx3b = 0.5 * nx * a3b, fy3b = 0.5 * ny * a3b;
+    double zb[2], xb[2], yb[2], hbb[2];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const double z = side_z[s], x = side_x[s], y = side_y[s], h = side_h[s], sf = side_s[s];
+        const double fnl = NLEQ + LEQ * h;
+        const double fu = NLEQ * x, fv = NLEQ * y, fg = NLEQG * z;
+        double zbar = 0.0, xbar = 0.0, ybar = 0.0, hbar = 0.0;
+        // fy3 = hvv + gh2; fx3 = huv*sf; fy2 = huv; fx2 = (huu + gh2)*sf
+        const double hvvb = fy3b;
+        double gh2b = fy3b + sf * fx2b;
+        const double huvb = sf * fx3b + fy2b;
+        const double huub = sf * fx2b;
+        // gh2 = fg*(0.5*z + h) + FG_L*z
+        double fgb = (0.5 * z + h) * gh2b;
+        zbar += (0.5 * fg + FG_L) * gh2b;
+        hbar += fg * gh2b;
+        // huu = fu*u, hvv = fv*v, huv = fu*v
+        double fub = u[s] * huub + v[s] * huvb;
+        double ub = fu * huub;
+        const double fvb = v[s] * hvvb;
+        double vb = fv * hvvb + fu * huvb;
+        // un = (u*nx + v*ny)*NLEQ
+        ub += nx * NLEQ * unb[s];
+        vb += ny * NLEQ * unb[s];
+        // c = sqrt(G*ht*K)
+        double htb = c[s] > 0.0 ? G * K[s] / (2.0 * c[s]) * cb[s] : 0.0;
+        // u = x*fh, v = y*fh
+        xbar += fh[s] * ub;
+        ybar += fh[s] * vb;
+        const double fhb = x * ub + y * vb;
+        // fh = 1/(NLEQ*ht + LEQ)
+        htb += -NLEQ * fh[s] * fh[s] * fhb;
+        // fg = NLEQG*z, fu = NLEQ*x, fv = NLEQ*y
+        zbar += NLEQG * fgb;
+        xbar += NLEQ * fub;
+        ybar += NLEQ * fvb;
+        // fx1 = x*fnl*sf, fy1 = y*fnl
+        xbar += fnl * sf * fx1b;
+        ybar += fnl * fy1b;
+        const double fnlb = x * sf * fx1b + y * fy1b;
+        hbar += LEQ * fnlb;
+        // ht = z*NLEQ + h
+        zbar += NLEQ * htb;
+        hbar += htb;
+        zb[s] = zbar; xb[s] = xbar; yb[s] = ybar; hbb[s] = hbar;
+    }
+    // jumps
+    bar[0] = zb[0] - j1b;
+    bar[1] = zb[1] + j1b;
+    bar[2] = xb[0] - j2b;
+    bar[3] = xb[1] + j2b;
+    bar[4] = yb[0] - j3b;
+    bar[5] = yb[1] + j3b;
+    bar[6] = hbb[0];
+    bar[7] = hbb[1];
+}
+
+struct AdjArgs {
+    ElemData ed;
+    Phys ph;
+    Forcing fo;
+    const double* __restrict__ u;        // stage state U(IRK) [3][dof][ne]
+    int irk;                             // stage index
+    int nsrc;                            // later-stage cotangents entering this stage
+    const double* src[KMAXRK];           // cotangent of U(I'+1), I' >= IRK
+    double acoef[KMAXRK];                // ATVD(I',IRK)
+    double bcoef[KMAXRK];                // BTVD(I',IRK) (times the scenario's DT in the kernel)
+    double* __restrict__ out;            // cotangent of U(IRK)
+    double* __restrict__ gdp;            // [3][ne] accumulated cotangent of the vertex depths
+    double* __restrict__ gnbar;          // [ne] accumulated cotangent of the element mean Manning n
+};
+
+// RHS cotangent of element `el` (its own stage contribution): sum_j b_j DT src_j.
+template <int DOF>
+__device__ __forceinline__ void load_lambda(const AdjArgs& a, int ne, int el, double dtdp, double (&L)[3][DOF]) {
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+#pragma unroll
+        for (int k = 0; k < DOF; k++) L[f][k] = 0.0;
+    for (int j = 0; j < a.nsrc; j++) {
+        const double b = a.bcoef[j] * dtdp;
+        if (b == 0.0) continue;
+        const double* s = a.src[j];
+#pragma unroll
+        for (int f = 0; f < 3; f++)
+#pragma unroll
+            for (int k = 0; k < DOF; k++) L[f][k] = L[f][k] + b * s[(size_t)(f * DOF + k) * ne + el];
+    }
+}
+
+template <int P, int NT, bool SPH>
+__global__ void __launch_bounds__(NT) adj_stage_kernel(AdjArgs a) {
+    constexpr int DOF = (P + 1) * (P + 2) / 2;
+    constexpr int NAGP = (P == 1) ? 3 : 6;
+    constexpr int NEGP = P + 2;
+    __shared__ double ub_lds[3 * DOF][NT];     // cotangent of this element's modes
+    const DevTables& T = tab<P>();
+    const ElemData& E = a.ed;
+    const Phys& ph = a.ph;
+    const int ne = E.ne;
+    const int tid = threadIdx.x;
+    const int e = xcd_tile(blockIdx.x, gridDim.x) * NT + tid;
+    if (e >= ne) return;
+    const int sid = E.sid ? (int)E.sid[e] : 0;
+    const Scen sc = E.scen[sid];
+    const int row = *a.fo.step * a.fo.nrk + a.irk;
+    const double c13 = 1.0 / 3.0, c16 = 1.0 / 6.0;
+
+    double U[3][DOF], Lm[3][DOF];
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+#pragma unroll
+        for (int k = 0; k < DOF; k++) U[f][k] = a.u[(size_t)(f * DOF + k) * ne + e];
+    load_lambda<DOF>(a, ne, e, sc.dtdp, Lm);
+#pragma unroll
+    for (int j = 0; j < 3 * DOF; j++) ub_lds[j][tid] = 0.0;
+    double hbb[3] = {0.0, 0.0, 0.0};
+    double AREAS, hb[3];
+    {
+        const double X0 = E.xv[e], X1 = E.xv[(size_t)ne + e], X2 = E.xv[2 * (size_t)ne + e];
+        const double Y0 = E.yv[e], Y1 = E.yv[(size_t)ne + e], Y2 = E.yv[2 * (size_t)ne + e];
+        const double D0 = E.dpv[e], D1 = E.dpv[(size_t)ne + e], D2 = E.dpv[2 * (size_t)ne + e];
+        AREAS = (X0 - X2) * (Y1 - Y2) + (X2 - X1) * (Y0 - Y2);
+        hb[0] = c13 * (D0 + D1 + D2); hb[1] = -(c16 * (D0 + D1)) + c13 * D2; hb[2] = -(0.5 * D0) + 0.5 * D1;
+    }
+    const int info = E.sinfo[e];
+#pragma unroll 1
+    for (int slot = 0; slot < 3; slot++) {
+        const int bits = side_bits(info, slot);
+        const int L = bits & 3, role = (bits >> 2) & 1, kind = (bits >> 3) & 3, nled = (bits >> 5) & 3;
+        const int nb = E.snbr[(size_t)slot * ne + e];
+        int va = (L + 1) % 3, vb = (L + 2) % 3;
+        if (role) { const int t = va; va = vb; vb = t; }
+        const double dy = E.yv[(size_t)vb * ne + e] - E.yv[(size_t)va * ne + e];
+        const double dx = E.xv[(size_t)vb * ne + e] - E.xv[(size_t)va * ne + e];
+        const double xlen = sqrt(dy * dy + dx * dx);
+        const double nx = dy / xlen, ny = -dx / xlen;
+        const double tx = -ny, ty = nx;
+        if (kind == SIDE_INTERNAL) {
+            double V[3][DOF], Ln[3][DOF];
+#pragma unroll
+            for (int f = 0; f < 3; f++)
+#pragma unroll
+                for (int k = 0; k < DOF; k++) V[f][k] = a.u[(size_t)(f * DOF + k) * ne + nb];
+            const int sidn = E.sid ? (int)E.sid[nb] : 0;
+            load_lambda<DOF>(a, ne, nb, E.scen[sidn].dtdp, Ln);
+            double hbn[3], areas_n;
+            {
+                const double d0 = E.dpv[nb], d1 = E.dpv[(size_t)ne + nb], d2 = E.dpv[2 * (size_t)ne + nb];
+                hbn[0] = c13 * (d0 + d1 + d2); hbn[1] = -(c16 * (d0 + d1)) + c13 * d2; hbn[2] = -(0.5 * d0) + 0.5 * d1;
+                const double X0 = E.xv[nb], X1 = E.xv[(size_t)ne + nb], X2 = E.xv[2 * (size_t)ne + nb];
+                const double Y0 = E.yv[nb], Y1 = E.yv[(size_t)ne + nb], Y2 = E.yv[2 * (size_t)ne + nb];
+                areas_n = (X0 - X2) * (Y1 - Y2) + (X2 - X1) * (Y0 - Y2);
+            }
+            const double edfac = xlen / AREAS, edfac_n = xlen / areas_n;
+#pragma unroll 1
+            for (int I = 0; I < NEGP; I++) {
+                const int gi = I, gx = NEGP - 1 - I;
+                const int gown = role ? gx : gi, gnb = role ? gi : gx;
+                const double* po = T.phi_edge[L][gown];
+                const double* pn = T.phi_edge[nled][gnb];
+                const double zo = trace<DOF>(U[0], po), xo = trace<DOF>(U[1], po), yo = trace<DOF>(U[2], po);
+                const double zn = trace<DOF>(V[0], pn), xn = trace<DOF>(V[1], pn), yn = trace<DOF>(V[2], pn);
+                const double ho = trace<3>(hb, po), hn = trace<3>(hbn, pn);
+                double so = 1.0, sn = 1.0;
+                if (SPH) {
+                    so = E.sfaced[((size_t)L * NEGP + gown) * ne + e];
+                    sn = E.sfaced[((size_t)nled * NEGP + gnb) * ne + nb];
+                }
+                // flux cotangent: the "in" element's RHS took -w_in*F, the "ex" element's +w_ex*F
+                const double* eqo = T.edgeq[L][gown];
+                const double* eqn = T.edgeq[nled][gnb];
+                double fb[3];
+#pragma unroll
+                for (int f = 0; f < 3; f++) {
+                    double so_ = 0.0, sn_ = 0.0;
+#pragma unroll
+                    for (int k = 0; k < DOF; k++) {
+                        so_ = so_ + edfac * eqo[k] * Lm[f][k];
+                        sn_ = sn_ + edfac_n * eqn[k] * Ln[f][k];
+                    }
+                    fb[f] = role ? (so_ - sn_) : (sn_ - so_);
+                }
+                double bar[8];
+                if (role) llf_flux_adj(ph, zn, zo, xn, xo, yn, yo, hn, ho, sn, so, nx, ny, ph.g, fb, bar);
+                else llf_flux_adj(ph, zo, zn, xo, xn, yo, yn, ho, hn, so, sn, nx, ny, ph.g, fb, bar);
+                const int s = role ? 1 : 0;   // this element's side of the flux arguments
+                const double zb = bar[0 + s], xb = bar[2 + s], yb = bar[4 + s], hbar = bar[6 + s];
+#pragma unroll
+                for (int k = 0; k < DOF; k++) {
+                    ub_lds[k][tid] += zb * po[k];
+                    ub_lds[DOF + k][tid] += xb * po[k];
+                    ub_lds[2 * DOF + k][tid] += yb * po[k];
+                }
+#pragma unroll
+                for (int k = 0; k < 3; k++) hbb[k] += hbar * po[k];
+            }
+        } else {
+#pragma unroll 1
+            for (int I = 0; I < NEGP; I++) {
+                const double* pv = T.phi_edge[L][I];
+                const double ze_in = trace<DOF>(U[0], pv), qx_in = trace<DOF>(U[1], pv), qy_in = trace<DOF>(U[2], pv);
+                const double hb_in = trace<3>(hb, pv);
+                const double sf = SPH ? E.sfaced[((size_t)L * NEGP + I) * ne + e] : 1.0;
+                // flux cotangent: RHS(K) -= w_K*F with w_K = 2 M_INV(K)/AREAS*XLEN*PHI*WEGP
+                double fb[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+                for (int k = 0; k < DOF; k++) {
+                    const double w = 2.0 * T.m_inv[k] / AREAS * xlen * pv[k] * T.wegp[I];
+                    fb[0] = fb[0] - w * Lm[0][k];
+                    fb[1] = fb[1] - w * Lm[1][k];
+                    fb[2] = fb[2] - w * Lm[2][k];
+                }
+                double bar[8];
+                double zb = 0.0, xb = 0.0, yb = 0.0, hbar = 0.0;
+                if (kind == SIDE_LAND) {
+                    const double qn_int = qx_in * nx + qy_in * ny, qt_int = qx_in * tx + qy_in * ty;
+                    const double qn_ext = -qn_int, qt_ext = qt_int;
+                    const double den = 1.0 / (nx * ty - ny * tx);
+                    const double qx_ex = (ty * qn_ext - ny * qt_ext) * den;
+                    const double qy_ex = (-tx * qn_ext + nx * qt_ext) * den;
+                    llf_flux_adj(ph, ze_in, ze_in, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_in, sf, sf, nx, ny, ph.g, fb, bar);
+                    zb = bar[0] + bar[1];
+                    hbar = bar[6] + bar[7];
+                    // mirror: (qx_ex, qy_ex) from (qx_in, qy_in)
+                    const double qnb = (ty * bar[3] - tx * bar[5]) * den;
+                    const double qtb = (-ny * bar[3] + nx * bar[5]) * den;
+                    const double qnib = -qnb, qtib = qtb;
+                    xb = bar[2] + nx * qnib + tx * qtib;
+                    yb = bar[4] + ny * qnib + ty * qtib;
+                } else if (kind == SIDE_OCEAN) {
+                    double ze_ex = 0.0;
+                    const double xe = T.xegp[I];
+                    for (int jj = 0; jj < a.fo.nbfr; jj++) {
+                        const size_t t = ((size_t)sid * a.fo.rows + row) * a.fo.nbfr + jj;
+                        const double argj = a.fo.argj[t];
+                        const double rff = a.fo.rff[t];
+                        const size_t o = ((size_t)nb * a.fo.nbfr + jj) * 2;
+                        const double z1 = a.fo.emo[o] * rff * cos(argj - a.fo.efa[o]);
+                        const double z2 = a.fo.emo[o + 1] * rff * cos(argj - a.fo.efa[o + 1]);
+                        ze_ex = ze_ex + (0.5 * (z1 + z2) + 0.5 * (z2 - z1) * xe);
+                    }
+                    const bool eirik = (ze_ex * ph.ifnlfa + hb_in) <= 0;
+                    if (eirik) ze_ex = fabs(hb_in) + sc.h0;
+                    llf_flux_adj(ph, ze_in, ze_ex, qx_in, qx_in, qy_in, qy_in, hb_in, hb_in, sf, sf, nx, ny, ph.g, fb, bar);
+                    zb = bar[0];
+                    xb = bar[2] + bar[3];
+                    yb = bar[4] + bar[5];
+                    hbar = bar[6] + bar[7];
+                    if (eirik) hbar += (hb_in > 0.0 ? 1.0 : (hb_in < 0.0 ? -1.0 : 0.0)) * bar[1];
+                } else {
+                    double qx_ex = 0.0, qy_ex = 0.0, qn_ext = 0.0;
+                    const double xe = T.xegp[I];
+                    for (int jj = 0; jj < a.fo.nffr; jj++) {
+                        const size_t t = ((size_t)sid * a.fo.rows + row) * a.fo.nffr + jj;
+                        const double argj = a.fo.fargj[t];
+                        const double rff = a.fo.frff[t];
+                        const size_t o = ((size_t)nb * a.fo.nffr + jj) * 2;
+                        const double q1 = a.fo.qnam[o], q2 = a.fo.qnam[o + 1];
+                        const double p1 = a.fo.qnph[o], p2 = a.fo.qnph[o + 1];
+                        const double qnam_gp = 0.5 * (q1 + q2) + 0.5 * (q2 - q1) * xe;
+                        const double qnph_gp = 0.5 * (p1 + p2) + 0.5 * (p2 - p1) * xe;
+                        qn_ext = qn_ext + qnam_gp * rff * cos(argj - qnph_gp);
+                        qx_ex = -(ty * qn_ext) / (nx * ty - ny * tx);
+                        qy_ex = -(-tx * qn_ext) / (nx * ty - ny * tx);
+                    }
+                    llf_flux_adj(ph, ze_in, ze_in, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_in, sf, sf, nx, ny, ph.g, fb, bar);
+                    zb = bar[0] + bar[1];
+                    xb = bar[2];
+                    yb = bar[4];
+                    hbar = bar[6] + bar[7];
+                }
+#pragma unroll
+                for (int k = 0; k < DOF; k++) {
+                    ub_lds[k][tid] += zb * pv[k];
+                    ub_lds[DOF + k][tid] += xb * pv[k];
+                    ub_lds[2 * DOF + k][tid] += yb * pv[k];
+                }
+#pragma unroll
+                for (int k = 0; k < 3; k++) hbb[k] += hbar * pv[k];
+            }
+        }
+    }
+
+    // ---- volume quadrature (every element wet)
+    double nbarb = 0.0;
+    {
+        double drdx, dsdx, drdy, dsdy;
+        {
+            const double X0 = E.xv[e], X1 = E.xv[(size_t)ne + e], X2 = E.xv[2 * (size_t)ne + e];
+            const double Y0 = E.yv[e], Y1 = E.yv[(size_t)ne + e], Y2 = E.yv[2 * (size_t)ne + e];
+            double AREA = (X0 - X2) * (Y1 - Y2) + (X2 - X1) * (Y0 - Y2);
+            AREA = AREA * 0.5;
+            drdx = 1.0 / AREA * (Y2 - Y0);
+            dsdx = 1.0 / AREA * (Y0 - Y1);
+            drdy = 1.0 / AREA * (X0 - X2);
+            dsdy = 1.0 / AREA * (X1 - X0);
+        }
+        const double depth_c = hb[0] + U[0][0];
+        const double fh_nl_c = 1.0 / (ph.nleq * depth_c + ph.leq);
+        const double cori = E.cori_el ? E.cori_el[e] : sc.cori;
+        const double nbar = ph.load_manning ? E.nbar[e] : 0.0;
+        double depth_cb = 0.0;
+#pragma unroll 1
+        for (int I = 0; I < NAGP; I++) {
+            const double* pa = T.phi_area[I];
+            double ze = U[0][0], qx = U[1][0], qy = U[2][0];
+#pragma unroll
+            for (int k = 1; k < DOF; k++) {
+                ze = ze + U[0][k] * pa[k];
+                qx = qx + U[1][k] * pa[k];
+                qy = qy + U[2][k] * pa[k];
+            }
+            double hbq = 0.0, dhbx = 0.0, dhby = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double dpxk = T.drphi[I][k] * drdx + T.dsphi[I][k] * dsdx;
+                const double dpyk = T.drphi[I][k] * drdy + T.dsphi[I][k] * dsdy;
+                hbq = hbq + hb[k] * pa[k];
+                dhbx = dhbx + hb[k] * dpxk;
+                dhby = dhby + hb[k] * dpyk;
+            }
+            const double sfq = SPH ? E.sfacq[(size_t)I * ne + e] : 1.0;
+            const double depth = ze + hbq;
+            const double f1 = ph.nleq + ph.leq * hbq;
+            const double fu = ph.nleq * qx, fv = ph.nleq * qy;
+            const double fg = ph.nleqg * ze;
+            const double fh = 1.0 / (ph.nleq * depth + ph.leq);
+            const double u = qx * fh, v = qy * fh;
+            double fric = sc.fric;
+            bool fric_active = false;
+            double pw = 0.0;
+            if (ph.load_manning) {
+                pw = pow(depth, 1.0 / 3.0);
+                fric = ph.g * (nbar * nbar) / pw;
+                if (fric < sc.cf) fric = sc.cf;
+                else fric_active = true;
+            }
+            double tau = fric, umag = 0.0, hyb = 1.0, hyb_in = 0.0;
+            bool capped = false;
+            if (ph.iflinbf == 0) {
+                umag = sqrt(u * u + v * v);
+                tau = tau * umag * fh;
+                if (ph.ifhybf == 1) {
+                    hyb_in = 1.0 + pow(ph.hbreak * fh_nl_c, ph.ftheta);
+                    hyb = pow(hyb_in, ph.fgamma / ph.ftheta);
+                    tau = tau * hyb;
+                }
+                const double cap = 0.9 * sc.dtdph;
+                if (cap < tau) { tau = cap; capped = true; }
+            }
+            // cotangents of the quadrature-point fluxes and sources from RHS(K)
+            double fxb = 0.0, fyb = 0.0, gxb = 0.0, gyb = 0.0, hxb = 0.0, hyb_ = 0.0;
+            double sxb = T.srfac[I][0] * Lm[1][0], syb = T.srfac[I][0] * Lm[2][0];
+#pragma unroll
+            for (int k = 1; k < DOF; k++) {
+                const double dpxk = T.drphi[I][k] * drdx + T.dsphi[I][k] * dsdx;
+                const double dpyk = T.drphi[I][k] * drdy + T.dsphi[I][k] * dsdy;
+                const double xf = T.mw[I][k] * dpxk, yf = T.mw[I][k] * dpyk, sr = T.srfac[I][k];
+                fxb += xf * Lm[0][k]; fyb += yf * Lm[0][k];
+                gxb += xf * Lm[1][k]; gyb += yf * Lm[1][k]; sxb += sr * Lm[1][k];
+                hxb += xf * Lm[2][k]; hyb_ += yf * Lm[2][k]; syb += sr * Lm[2][k];
+            }
+            double zeb = 0.0, qxb = 0.0, qyb = 0.0, hbqb = 0.0, dhbxb = 0.0, dhbyb = 0.0, depthb = 0.0;
+            // sx = -(tau*qx) + fg*dhbx*sfq + cori*qy ; sy = -(tau*qy) + fg*dhby - cori*qx
+            double taub = -qx * sxb - qy * syb;
+            qxb += -tau * sxb - cori * syb;
+            qyb += cori * sxb - tau * syb;
+            double fgb = dhbx * sfq * sxb + dhby * syb;
+            dhbxb += fg * sfq * sxb;
+            dhbyb += fg * syb;
+            // friction
+            double fricb = 0.0, ub = 0.0, vb = 0.0, fhb = 0.0;
+            if (ph.iflinbf == 0) {
+                if (!capped) {
+                    // tau = fric*umag*fh*hyb
+                    fricb += umag * fh * hyb * taub;
+                    const double umagb = fric * fh * hyb * taub;
+                    fhb += fric * umag * hyb * taub;
+                    if (ph.ifhybf == 1) {
+                        const double hybb = fric * umag * fh * taub;
+                        // hyb = (1 + (hbreak*fh_nl_c)^ftheta)^(fgamma/ftheta)
+                        const double dh_in = (ph.fgamma / ph.ftheta) * pow(hyb_in, ph.fgamma / ph.ftheta - 1.0);
+                        const double x = ph.hbreak * fh_nl_c;
+                        const double dx_ = x > 0.0 ? ph.ftheta * pow(x, ph.ftheta - 1.0) : 0.0;
+                        const double fhcb = hybb * dh_in * dx_ * ph.hbreak;
+                        depth_cb += -ph.nleq * fh_nl_c * fh_nl_c * fhcb;
+                    }
+                    if (umag > 0.0) { ub += u / umag * umagb; vb += v / umag * umagb; }
+                }
+            } else {
+                fricb += taub;
+            }
+            if (ph.load_manning && fric_active) {
+                // fric = g*nbar^2/depth^(1/3)
+                nbarb += ph.g * 2.0 * nbar / pw * fricb;
+                depthb += -ph.g * (nbar * nbar) / (3.0 * pw * depth) * fricb;
+            }
+            // hy = hvv + gh2 ; hx = huv*sfq ; gy = huv ; gx = (huu + gh2)*sfq ; fx = qx*f1*sfq ; fy = qy*f1
+            const double hvvb = hyb_;
+            const double gh2b = hyb_ + sfq * gxb;
+            const double huvb = sfq * hxb + gyb;
+            const double huub = sfq * gxb;
+            fgb += (0.5 * ze + hbq) * gh2b;
+            zeb += (0.5 * fg + ph.fg_l) * gh2b;
+            hbqb += fg * gh2b;
+            const double fub = u * huub + v * huvb;
+            ub += fu * huub;
+            const double fvb = v * hvvb;
+            vb += fv * hvvb + fu * huvb;
+            qxb += fh * ub; qyb += fh * vb;
+            fhb += qx * ub + qy * vb;
+            depthb += -ph.nleq * fh * fh * fhb;
+            zeb += ph.nleqg * fgb;
+            qxb += ph.nleq * fub;
+            qyb += ph.nleq * fvb;
+            qxb += f1 * sfq * fxb;
+            qyb += f1 * fyb;
+            const double f1b = qx * sfq * fxb + qy * fyb;
+            hbqb += ph.leq * f1b;
+            zeb += depthb;
+            hbqb += depthb;
+            // traces
+            ub_lds[0][tid] += zeb;
+            ub_lds[DOF][tid] += qxb;
+            ub_lds[2 * DOF][tid] += qyb;
+#pragma unroll
+            for (int k = 1; k < DOF; k++) {
+                ub_lds[k][tid] += zeb * pa[k];
+                ub_lds[DOF + k][tid] += qxb * pa[k];
+                ub_lds[2 * DOF + k][tid] += qyb * pa[k];
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double dpxk = T.drphi[I][k] * drdx + T.dsphi[I][k] * dsdx;
+                const double dpyk = T.drphi[I][k] * drdy + T.dsphi[I][k] * dsdy;
+                hbb[k] += hbqb * pa[k] + dhbxb * dpxk + dhbyb * dpyk;
+            }
+        }
+        // depth_c = hb[0] + U[0][0]
+        hbb[0] += depth_cb;
+        ub_lds[0][tid] += depth_cb;
+    }
+    // ---- cotangent of U(IRK): later stages' ATVD terms plus this stage's RHS reverse
+    for (int j = 0; j < a.nsrc; j++) {
+        const double ac = a.acoef[j];
+        if (ac == 0.0) continue;
+        const double* s = a.src[j];
+#pragma unroll
+        for (int f = 0; f < 3 * DOF; f++) ub_lds[f][tid] += ac * s[(size_t)f * ne + e];
+    }
+#pragma unroll
+    for (int f = 0; f < 3 * DOF; f++) a.out[(size_t)f * ne + e] = ub_lds[f][tid];
+    // parameters: element-mean Manning n and the vertex depths through the P1 bathymetry modes
+    if (a.gnbar) a.gnbar[e] += nbarb;
+    if (a.gdp) {
+        const double D0b = c13 * hbb[0] - c16 * hbb[1] - 0.5 * hbb[2];
+        const double D1b = c13 * hbb[0] - c16 * hbb[1] + 0.5 * hbb[2];
+        const double D2b = c13 * hbb[0] + c13 * hbb[1];
+        a.gdp[e] += D0b;
+        a.gdp[(size_t)ne + e] += D1b;
+        a.gdp[2 * (size_t)ne + e] += D2b;
+    }
+}
+
+// Elevation at the gauges: eta_g = sum_K ZE(K, elem_g) * phi_K(gauge), written to out[g].
+template <int P>
+__global__ void gauge_kernel(int ng, int ne, const int* __restrict__ gel, const double* __restrict__ gphi,
+                             const double* __restrict__ u, double* __restrict__ out) {
+    constexpr int DOF = (P + 1) * (P + 2) / 2;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    const int e = gel[g];
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < DOF; k++) s = s + u[(size_t)k * ne + e] * gphi[(size_t)g * DOF + k];
+    out[g] = s;
+}
+
+// Cotangent seed of the gauge misfit sum_g w (eta_g - d_g)^2 into lam (one thread, gauges in
+// order, so gauges sharing an element add up deterministically).
+template <int P>
+__global__ void gauge_seed_kernel(int ng, int ne, const int* __restrict__ gel, const double* __restrict__ gphi,
+                                  const double* __restrict__ eta, const double* __restrict__ data, double w,
+                                  double* __restrict__ lam) {
+    constexpr int DOF = (P + 1) * (P + 2) / 2;
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int g = 0; g < ng; g++) {
+        const double r = 2.0 * w * (eta[g] - data[g]);
+        const int e = gel[g];
+#pragma unroll
+        for (int k = 0; k < DOF; k++) lam[(size_t)k * ne + e] += r * gphi[(size_t)g * DOF + k];
+    }
+}
+
+// Node gradients from the element accumulators: dJ/dDP(n) = sum over incident elements of the
+// vertex-depth cotangent; dJ/dn(n) = sum of the element-mean cotangents / 3.
+__global__ void __launch_bounds__(256) param_gather_kernel(int np, int ne, const int* __restrict__ off,
+                                                           const int* __restrict__ lst,
+                                                           const double* __restrict__ gdp,
+                                                           const double* __restrict__ gnbar,
+                                                           double* __restrict__ out_dp, double* __restrict__ out_n) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= np) return;
+    double sdp = 0.0, sn = 0.0;
+    for (int j = off[n]; j < off[n + 1]; j++) {
+        const int ent = lst[j];
+        const int e = ent >> 2, v = ent & 3;
+        sdp += gdp[(size_t)v * ne + e];
+        sn += gnbar[e];
+    }
+    out_dp[n] = sdp;
+    out_n[n] = sn / 3.0;
+}
+
+}  // namespace dg

@@ -25,6 +25,7 @@
 
 // the kernels are compiled in this translation unit
 #include "dg_kernels.hip"
+#include "dg_adjoint.hip"
 
 using namespace dg;
 
@@ -122,6 +123,11 @@ struct Ctx {
     int next_it = 1;                             // IT of the next step (for the autotuner's tables)
     // launch configuration
     int block = 256;
+    // adjoint: gauges, checkpoints, segment stage states, cotangents, parameter accumulators
+    int ng = 0;
+    DevBuf<int> gel;
+    DevBuf<double> gphi, geta, gdata, ckpt, seg, lam, gdp, gnb, pout;
+    DevBuf<double> ubar[KMAXRK];
     // graphs of one slot-rotation period, keyed by (ucur, wdcur) at capture
     bool use_graph = false;
     std::map<int, hipGraphExec_t> graphs;
@@ -787,6 +793,36 @@ int launch_steps(Ctx& c, int n, bool capturing) {
     return 0;
 }
 
+// One step with explicit state pointers: st[0] input, st[1..NRK-1] the intermediate stage states,
+// st[NRK] output (no limiter, no wetting and drying: the adjoint's configurations).
+template <int P>
+int launch_step_explicit(Ctx& c, double* const* st) {
+    const int nrk = c.rk.nrk;
+    const bool sph = c.prm.ics == 2;
+    for (int irk = 0; irk < nrk; irk++) {
+        StageArgs a{};
+        a.ed = elem_data(c); a.ph = c.phys; a.fo = forcing(c);
+        a.uin = st[irk];
+        a.uout = st[irk + 1];
+        a.wd_in = c.wd[c.wdcur].p;
+        a.wd_out = c.wd[1 - c.wdcur].p;
+        a.nprev = irk;
+        for (int i = 0; i < irk; i++) {
+            a.uprev[i] = st[i]; a.aprev[i] = c.rk.a[irk][i];
+            a.rprev[i] = c.r[i].p; a.bprev[i] = c.rk.b[irk][i];
+        }
+        a.acur = c.rk.a[irk][irk];
+        a.bcur = c.rk.b[irk][irk];
+        a.rout = c.keep_r[irk] ? c.r[irk].p : nullptr;
+        a.irk = irk;
+        launch_stage_nt<P>(c, a, sph, false, c.block);
+        HIPCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(1), 0, c.stream, c.step.p, 0, 0);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 // The steps of one table chunk: replays of a captured graph of one slot-rotation period (NRK+1
 // steps, after which the stage slots and wet/dry buffers are back in place), then eager launches
 // for the remainder.
@@ -818,6 +854,121 @@ int run_chunk(Ctx& c, int n) {
         done = reps * period;
     }
     return launch_steps(c, n - done, false);
+}
+
+// ---- discrete adjoint driver ------------------------------------------------------------
+int adj_check(Ctx& c) {
+    if (c.S != 1) return fail("the adjoint runs on single runs, not ensembles");
+    if (c.prm.slopeflag != 0) return fail("the adjoint differentiates runs without the slope limiter");
+    if (c.prm.nolifa >= 2) return fail("the adjoint differentiates runs without wetting and drying");
+    if (c.met) return fail("the adjoint does not differentiate runs with NWS=8 forcing");
+    if (c.ng == 0) return fail("no gauges set (dg_adj_set_gauges)");
+    return 0;
+}
+
+template <int P, int NT>
+void launch_adj(Ctx& c, const AdjArgs& a) {
+    const int grid = (c.ne + NT - 1) / NT;
+    if (c.prm.ics == 2) hipLaunchKernelGGL((adj_stage_kernel<P, NT, true>), dim3(grid), dim3(NT), 0, c.stream, a);
+    else hipLaunchKernelGGL((adj_stage_kernel<P, NT, false>), dim3(grid), dim3(NT), 0, c.stream, a);
+}
+
+// Forward N steps from the current state recording the gauges after every step (geta[k-1] after
+// step k) and saving the state every `interval` steps into ckpt; the handle's state ends at U^N.
+template <int P>
+int adj_forward(Ctx& c, int it0, int nsteps, int interval, bool save) {
+    const size_t ssz = (size_t)3 * c.dof * c.ne;
+    const int B = 256, gg = (c.ng + B - 1) / B;
+    for (int s0 = 0; s0 < nsteps; s0 += kTableChunk) {
+        const int n = std::min(kTableChunk, nsteps - s0);
+        if (build_stage_tables(c, it0 + s0, n)) return 1;
+        hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(1), 0, c.stream, c.step.p, 1, 0);
+        for (int s = 0; s < n; s++) {
+            const int k = s0 + s;   // steps done before this one
+            if (save && k % interval == 0)
+                HIPCHK(hipMemcpyAsync(c.ckpt.p + (size_t)(k / interval) * ssz, c.u[c.ucur].p, ssz * sizeof(double),
+                                      hipMemcpyDeviceToDevice, c.stream));
+            if (launch_step<P>(c, false)) return 1;
+            hipLaunchKernelGGL((gauge_kernel<P>), dim3(gg), dim3(B), 0, c.stream, c.ng, c.ne, (const int*)c.gel.p,
+                               (const double*)c.gphi.p, (const double*)c.u[c.ucur].p, c.geta.p + (size_t)k * c.ng);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    return 0;
+}
+
+template <int P>
+int adj_backward(Ctx& c, int it0, int nsteps, int interval, double weight) {
+    const int nrk = c.rk.nrk, ne = c.ne;
+    const size_t ssz = (size_t)3 * c.dof * ne;
+    const int nseg = (nsteps + interval - 1) / interval;
+    HIPCHK(hipMemsetAsync(c.lam.p, 0, ssz * sizeof(double), c.stream));
+    HIPCHK(hipMemsetAsync(c.gdp.p, 0, (size_t)3 * ne * sizeof(double), c.stream));
+    HIPCHK(hipMemsetAsync(c.gnb.p, 0, (size_t)ne * sizeof(double), c.stream));
+    // seed: misfit at step N
+    hipLaunchKernelGGL((gauge_seed_kernel<P>), dim3(1), dim3(64), 0, c.stream, c.ng, ne, (const int*)c.gel.p,
+                       (const double*)c.gphi.p, (const double*)(c.geta.p + (size_t)(nsteps - 1) * c.ng),
+                       (const double*)(c.gdata.p + (size_t)(nsteps - 1) * c.ng), weight, c.lam.p);
+    double* lam = c.lam.p;
+    double* ub[KMAXRK + 1];
+    for (int i = 0; i < nrk; i++) ub[i] = c.ubar[i].p;
+    for (int sg = nseg - 1; sg >= 0; sg--) {
+        const int k0 = sg * interval, k1 = std::min(k0 + interval, nsteps);
+        const int n = k1 - k0;
+        // stage states of steps k0+1..k1: seg[(k-k0-1)*nrk + i] holds stage i of step k
+        auto stage = [&](int k, int i) { return c.seg.p + ((size_t)(k - k0 - 1) * nrk + i) * ssz; };
+        if (build_stage_tables(c, it0 + k0, n)) return 1;
+        HIPCHK(hipMemcpyAsync(stage(k0 + 1, 0), c.ckpt.p + (size_t)sg * ssz, ssz * sizeof(double),
+                              hipMemcpyDeviceToDevice, c.stream));
+        hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(1), 0, c.stream, c.step.p, 1, 0);
+        for (int k = k0 + 1; k < k1; k++) {
+            double* st[KMAXRK + 1];
+            for (int i = 0; i < nrk; i++) st[i] = stage(k, i);
+            st[nrk] = stage(k + 1, 0);
+            if (launch_step_explicit<P>(c, st)) return 1;
+        }
+        if (nrk > 1) {   // the last step's intermediate stages
+            double* st[KMAXRK + 1];
+            for (int i = 0; i < nrk; i++) st[i] = stage(k1, i);
+            st[nrk] = c.pout.p;   // output unused
+            if (launch_step_explicit<P>(c, st)) return 1;
+        }
+        for (int k = k1; k > k0; k--) {
+            hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(1), 0, c.stream, c.step.p, 1, k - k0 - 1);
+            ub[nrk] = lam;
+            for (int i = nrk - 1; i >= 0; i--) {
+                AdjArgs a{};
+                a.ed = elem_data(c); a.ph = c.phys; a.fo = forcing(c);
+                a.u = stage(k, i);
+                a.irk = i;
+                a.nsrc = nrk - i;
+                for (int j = 0; j < a.nsrc; j++) {
+                    const int ip = i + j;   // later stage I' whose output U(I'+1) carries a cotangent
+                    a.src[j] = ub[ip + 1];
+                    a.acoef[j] = c.rk.a[ip][i];
+                    a.bcoef[j] = c.rk.b[ip][i];
+                }
+                a.out = ub[i];
+                a.gdp = c.gdp.p;
+                a.gnbar = c.phys.load_manning ? c.gnb.p : nullptr;
+                if (P == 1) launch_adj<P, 128>(c, a);
+                else launch_adj<P, 64>(c, a);
+                HIPCHK(hipGetLastError());
+            }
+            std::swap(lam, ub[0]);   // cotangent of U^(k-1)
+            if (k - 1 >= 1)
+                hipLaunchKernelGGL((gauge_seed_kernel<P>), dim3(1), dim3(64), 0, c.stream, c.ng, ne,
+                                   (const int*)c.gel.p, (const double*)c.gphi.p,
+                                   (const double*)(c.geta.p + (size_t)(k - 2) * c.ng),
+                                   (const double*)(c.gdata.p + (size_t)(k - 2) * c.ng), weight, lam);
+        }
+    }
+    // keep the buffers' identities for the next call
+    if (lam != c.lam.p) {
+        for (int i = 0; i < nrk; i++)
+            if (ub[i] == c.lam.p) { std::swap(c.lam.p, c.ubar[i].p); break; }
+    }
+    return 0;
 }
 
 int validate(const dg_params* p) {
@@ -1089,6 +1240,124 @@ int dg_gpu_kernel_stats(int h, int kid, double* total_ms, long long* launches, c
     if (total_ms) *total_ms = c->ktime[kid];
     if (launches) *launches = c->kcount[kid];
     if (name && name_len > 0) { std::strncpy(name, kKernelNames[kid], name_len - 1); name[name_len - 1] = 0; }
+    return 0;
+}
+
+int dg_gpu_set_manning(int h, const double* manning) {
+    Ctx* c = get(h);
+    if (!c) return 1;
+    if (c->S != 1) return fail("parameters of an ensemble are set on its members");
+    if (!c->phys.load_manning) return fail("the run was set up without Manning's n");
+    Member& m = c->mem[0];
+    const int ne = m.ne;
+    for (int e = 0; e < ne; e++) {
+        const int n0 = m.nm[e], n1 = m.nm[(size_t)ne + e], n2 = m.nm[2 * (size_t)ne + e];
+        m.nbar[e] = (manning[n0] + manning[n1] + manning[n2]) / 3.0;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(c->nbar.upload(m.nbar.data(), ne, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dg_gpu_set_depth(int h, const double* dp) {
+    Ctx* c = get(h);
+    if (!c) return 1;
+    if (c->S != 1) return fail("parameters of an ensemble are set on its members");
+    Member& m = c->mem[0];
+    const int ne = m.ne;
+    for (int v = 0; v < 3; v++)
+        for (int e = 0; e < ne; e++) m.dpv[(size_t)v * ne + e] = dp[m.nm[(size_t)v * ne + e]];
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(c->dpv.upload(m.dpv.data(), m.dpv.size(), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dg_adj_set_gauges(int h, int ng, const int* elem, const double* r, const double* s) {
+    Ctx* c = get(h);
+    if (!c) return 1;
+    if (ng <= 0) return fail("ng must be positive");
+    const int dof = c->dof;
+    std::vector<int> ge(ng);
+    std::vector<double> ph((size_t)ng * dof);
+    for (int g = 0; g < ng; g++) {
+        if (elem[g] < 1 || elem[g] > c->ne) return fail("gauge element out of range");
+        if (!(r[g] >= -1.0 && s[g] >= -1.0 && r[g] + s[g] <= 0.0)) return fail("gauge outside its element");
+        ge[g] = elem[g] - 1;
+        basis_at(c->prm.p, r[g], s[g], &ph[(size_t)g * dof]);
+    }
+    c->ng = ng;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(c->gel.upload(ge.data(), ng, c->stream));
+    HIPCHK(c->gphi.upload(ph.data(), ph.size(), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dg_adj_gauges(int h, int it, int nsteps, double* eta) {
+    Ctx* c = get(h);
+    if (!c) return 1;
+    if (c->ng == 0) return fail("no gauges set (dg_adj_set_gauges)");
+    if (nsteps <= 0) return 0;
+    HIPCHK(c->geta.alloc((size_t)nsteps * c->ng));
+    const int rc = c->prm.p == 1 ? adj_forward<1>(*c, it, nsteps, 1, false) : adj_forward<2>(*c, it, nsteps, 1, false);
+    if (rc) return rc;
+    c->next_it = it + nsteps;
+    HIPCHK(hipMemcpyAsync(eta, c->geta.p, (size_t)nsteps * c->ng * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dg_adj_gradient(int h, int it, int nsteps, const double* data, double weight, int interval, double* loss,
+                    double* grad_manning, double* grad_dp) {
+    Ctx* c = get(h);
+    if (!c) return 1;
+    if (adj_check(*c)) return 1;
+    if (nsteps <= 0) return fail("nsteps must be positive");
+    if (interval <= 0) interval = std::max(1, (int)std::lround(std::sqrt((double)nsteps)));
+    interval = std::min(interval, nsteps);
+    const int nrk = c->rk.nrk, ne = c->ne, np = c->np;
+    const size_t ssz = (size_t)3 * c->dof * ne;
+    const int ncp = (nsteps + interval - 1) / interval;
+    HIPCHK(c->geta.alloc((size_t)nsteps * c->ng));
+    HIPCHK(c->gdata.upload(data, (size_t)nsteps * c->ng, c->stream));
+    HIPCHK(c->ckpt.alloc((size_t)ncp * ssz));
+    HIPCHK(c->seg.alloc((size_t)interval * nrk * ssz));
+    HIPCHK(c->lam.alloc(ssz));
+    for (int i = 0; i < nrk; i++) HIPCHK(c->ubar[i].alloc(ssz));
+    HIPCHK(c->gdp.alloc((size_t)3 * ne));
+    HIPCHK(c->gnb.alloc(ne));
+    HIPCHK(c->pout.alloc(std::max(ssz, (size_t)2 * np)));
+    const bool prof = c->prof;
+    c->prof = false;
+    int rc = c->prm.p == 1 ? adj_forward<1>(*c, it, nsteps, interval, true) : adj_forward<2>(*c, it, nsteps, interval, true);
+    if (!rc) rc = c->prm.p == 1 ? adj_backward<1>(*c, it, nsteps, interval, weight) : adj_backward<2>(*c, it, nsteps, interval, weight);
+    c->prof = prof;
+    if (rc) return rc;
+    // node gradients
+    double* gdp_n = c->pout.p;
+    double* gn_n = c->pout.p + np;
+    if (!c->phys.load_manning) HIPCHK(hipMemsetAsync(c->gnb.p, 0, (size_t)ne * sizeof(double), c->stream));
+    hipLaunchKernelGGL(param_gather_kernel, dim3((np + 255) / 256), dim3(256), 0, c->stream, np, ne,
+                       (const int*)c->n2e_off.p, (const int*)c->n2e.p, (const double*)c->gdp.p,
+                       (const double*)c->gnb.p, gdp_n, gn_n);
+    HIPCHK(hipGetLastError());
+    std::vector<double> eta((size_t)nsteps * c->ng);
+    HIPCHK(hipMemcpyAsync(eta.data(), c->geta.p, eta.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (grad_dp) HIPCHK(hipMemcpyAsync(grad_dp, gdp_n, (size_t)np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (grad_manning) HIPCHK(hipMemcpyAsync(grad_manning, gn_n, (size_t)np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    // the handle's state returns to where the call started
+    HIPCHK(hipMemcpyAsync(c->u[c->ucur].p, c->ckpt.p, ssz * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (loss) {
+        double J = 0.0;
+        for (size_t k = 0; k < eta.size(); k++) {
+            const double d = eta[k] - data[k];
+            J += d * d;
+        }
+        *loss = weight * J;
+    }
     return 0;
 }
 

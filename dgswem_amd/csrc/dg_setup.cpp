@@ -82,6 +82,11 @@ double legendre(int n, double x) {
 
 }  // namespace
 
+void basis_at(int p, double r, double s, double* phi) {
+    double dr[MAXDOF], ds[MAXDOF];
+    dubiner(p, r, s, phi, dr, ds);
+}
+
 bool build_ref_tables(int p, RefTables& t, std::string& err) {
     if (p < 1 || p > 2) { err = "polynomial order must be 1 or 2"; return false; }
     t = RefTables{};

@@ -28,6 +28,8 @@ struct RefTables {
     double psi[MAXAGP][3]{};                // linear nodal basis PSI1-3 at the area points
 };
 bool build_ref_tables(int p, RefTables& t, std::string& err);
+// Values of the p<=2 modes at reference coordinates (r, s) of the triangle (-1,-1),(1,-1),(-1,1).
+void basis_at(int p, double r, double s, double* phi);
 
 // SSP Runge-Kutta scheme in Shu-Osher form (src/prep_DG.F:1551-2078).
 struct RKScheme {

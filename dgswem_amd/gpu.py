@@ -28,7 +28,8 @@ K_STAGE, K_MINMAX, K_LIMIT, K_M2N, K_MET = 0, 1, 2, 3, 4
 EXPORTED = ["dg_create_edge_data", "dg_gpu_init", "dg_gpu_set_tide", "dg_gpu_set_flow",
             "dg_gpu_set_state", "dg_gpu_get_state", "dg_gpu_step", "dg_gpu_synchronize",
             "dg_gpu_modal2nodal", "dg_gpu_mass", "dg_gpu_profile", "dg_gpu_kernel_stats",
-            "dg_gpu_set_holland", "dg_holland_storm_at", "dg_ens_create", "dg_ens_layout", "dg_gpu_use_graph", "dg_gpu_autotune",
+            "dg_gpu_set_holland", "dg_holland_storm_at", "dg_gpu_set_manning", "dg_gpu_set_depth",
+            "dg_adj_set_gauges", "dg_adj_gauges", "dg_adj_gradient", "dg_ens_create", "dg_ens_layout", "dg_gpu_use_graph", "dg_gpu_autotune",
             "dg_gpu_set_block", "dg_gpu_last_error", "dg_gpu_finalize"]
 
 
@@ -74,6 +75,12 @@ def load_library(path: str = LIB_PATH):
     L.dg_gpu_set_holland.argtypes = [C.c_int, C.c_int, _ip, _ip, C.c_int, C.c_double, _dp, _dp]
     L.dg_holland_storm_at.argtypes = [C.c_int, _ip, _ip, C.c_int, C.c_double, C.c_double, C.c_int,
                                       _dp, _dp]
+    L.dg_gpu_set_manning.argtypes = [C.c_int, _dp]
+    L.dg_gpu_set_depth.argtypes = [C.c_int, _dp]
+    L.dg_adj_set_gauges.argtypes = [C.c_int, C.c_int, _ip, _dp, _dp]
+    L.dg_adj_gauges.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
+    L.dg_adj_gradient.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_double, C.c_int,
+                                  C.POINTER(C.c_double), _dp, _dp]
     L.dg_ens_create.argtypes = [C.c_int, _ip, C.POINTER(C.c_int)]
     L.dg_ens_layout.argtypes = [C.c_int, C.POINTER(C.c_int), C.c_void_p, C.c_void_p]
     L.dg_gpu_use_graph.argtypes = [C.c_int, C.c_int]
@@ -254,6 +261,40 @@ class _Handle:
 
     def set_block(self, block: int) -> None:
         _check(load_library().dg_gpu_set_block(self.h, block))
+
+    # ---- parameters and the discrete adjoint (single runs)
+    def set_manning(self, manning) -> None:
+        """Per-node Manning n (ManningsN); element means are rebuilt as in the forward set-up."""
+        _check(load_library().dg_gpu_set_manning(self.h, _f(manning)))
+
+    def set_depth(self, dp) -> None:
+        """Per-node depth DP (after any H0 floor)."""
+        _check(load_library().dg_gpu_set_depth(self.h, _f(dp)))
+
+    def set_gauges(self, elem, r, s) -> None:
+        """Gauges at reference coordinates (r, s) of 1-based elements."""
+        self.ng = len(elem)
+        _check(load_library().dg_adj_set_gauges(self.h, self.ng, _i(elem), _f(r), _f(s)))
+
+    def gauges(self, n: int) -> np.ndarray:
+        """Advance n steps recording the gauge elevations after each: array [n, ng]."""
+        eta = np.empty((n, self.ng))
+        _check(load_library().dg_adj_gauges(self.h, self.it + 1, n, eta.reshape(-1)))
+        self.it += n
+        return eta
+
+    def gradient(self, data, weight: float = 1.0, interval: int = 0):
+        """Loss J = weight * sum_k sum_g (eta_g(U^k) - data[k-1, g])^2 over the next len(data)
+        steps and its gradient with respect to the per-node Manning n and depth DP, by the
+        discrete adjoint with checkpoints every `interval` steps (0: about sqrt(N)).  The state
+        is left where it was."""
+        d = _f(data)
+        n = d.shape[0]
+        gn, gdp = np.empty(self.np_), np.empty(self.np_)
+        J = C.c_double()
+        _check(load_library().dg_adj_gradient(self.h, self.it + 1, n, d.reshape(-1), float(weight),
+                                              int(interval), C.byref(J), gn, gdp))
+        return J.value, gn, gdp
 
     def layout(self):
         """Element and node offsets of the members (two entries for a single run)."""

@@ -137,6 +137,31 @@ int dg_gpu_set_holland(int handle, int nrec, const int* rec, const int* ref, int
 int dg_holland_storm_at(int nrec, const int* rec, const int* ref, int storm_number, double bladj,
                         double g, int ntimes, const double* times, double* rows);
 
+/* ---- Parameters and the discrete adjoint.  None of these has a counterpart in the reference. */
+
+/* Replace the per-node Manning n (ManningsN; element means rebuilt as at set-up) or the per-node
+ * depth DP.  Single runs only. */
+int dg_gpu_set_manning(int handle, const double* manning);
+int dg_gpu_set_depth(int handle, const double* dp);
+
+/* Gauges for the misfit loss: ng points at reference coordinates (r, s) of 1-based elements
+ * (the reference triangle is (-1,-1), (1,-1), (-1,1)); the gauge elevation is the element's
+ * modal expansion of ZE there. */
+int dg_adj_set_gauges(int handle, int ng, const int* elem, const double* r, const double* s);
+
+/* Advance nsteps steps (IT = it .. it+nsteps-1) and write eta[k][g], the gauge elevations after
+ * step it+k. */
+int dg_adj_gauges(int handle, int it, int nsteps, double* eta);
+
+/* Loss J = weight * sum_{k<nsteps} sum_g (eta_g after step it+k - data[k][g])^2 from the current
+ * state, and its gradient with respect to the per-node Manning n and depth DP, by the discrete
+ * adjoint of the stage RHS and the SSP-RK stages.  States are checkpointed every `interval` steps
+ * (0: round(sqrt(nsteps))) and each segment is recomputed once on the way back.  Runs without
+ * slope limiter, wetting/drying and NWS forcing; single runs.  The handle's state is left as it
+ * was.  grad_manning / grad_dp: NP values each (either may be NULL). */
+int dg_adj_gradient(int handle, int it, int nsteps, const double* data, double weight,
+                    int interval, double* loss, double* grad_manning, double* grad_dp);
+
 /* ---- Batched ensembles, graph replay and launch tuning.  None of these has a counterpart in the
  * reference, which runs one scenario per process: an ensemble replaces N separate dgswem_serial
  * runs (one DG_HYDRO_TIMESTEP per member per step) with one stage kernel launch per RK stage over

@@ -1,0 +1,162 @@
+"""Discrete adjoint of the GPU step (through the C ABI and the torch front end).
+
+Gradients of the gauge-misfit loss with respect to per-node Manning n and depth DP are checked
+against central finite differences of the same GPU forward model in 20 seeded directions
+(relative agreement 1e-6), against central differences of the CPU oracle (the FP64 CPU reference
+gradient), and for independence from the checkpoint spacing (bit for bit).
+"""
+import numpy as np
+import pytest
+
+from dgswem_amd import adjoint, gpu, mesh
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+
+
+def small_case(p=1, n_elem=2000, seed=31, nsteps=60):
+    """All-wet seeded Delaunay basin, no limiter: the differentiated configuration."""
+    return mesh.delaunay_basin(n_elem=n_elem, seed=seed, lx=6.0e4, ly=3.6e4, depth_range=(10.0, 300.0),
+                               shelf=False, wetdry=False, slopeflag=0, p=p, nsteps=nsteps)
+
+
+def directions(case, k, seed):
+    rng = np.random.default_rng(seed)
+    for _ in range(k):
+        vn = rng.standard_normal(case.np_) * case.manning
+        vd = rng.standard_normal(case.np_) * case.dp
+        yield vn, vd
+
+
+def central_fd(problem, case, vn, vd, eps):
+    jp = problem.loss(case.manning + eps * vn, case.dp + eps * vd)
+    jm = problem.loss(case.manning - eps * vn, case.dp - eps * vd)
+    return (jp - jm) / (2.0 * eps)
+
+
+@pytest.fixture(scope="module")
+def p1():
+    case = small_case()
+    prob = adjoint.twin_problem(case, nsteps=60, ng=20, seed=5)
+    J, gn, gdp = prob.gradient(case.manning, case.dp)
+    return case, prob, J, gn, gdp
+
+
+def test_loss_matches_forward(p1):
+    case, prob, J, gn, gdp = p1
+    print(f"J = {J:.6e}", flush=True)
+    assert J > 0.0
+    assert J == pytest.approx(prob.loss(case.manning, case.dp), rel=1e-13)
+    assert np.all(np.isfinite(gn)) and np.all(np.isfinite(gdp))
+    assert np.abs(gn).max() > 0.0 and np.abs(gdp).max() > 0.0
+
+
+def test_gradient_matches_central_differences_20_directions(p1):
+    case, prob, J, gn, gdp = p1
+    worst = 0.0
+    for i, (vn, vd) in enumerate(directions(case, 20, seed=77)):
+        ad = float(gn @ vn + gdp @ vd)
+        fd = central_fd(prob, case, vn, vd, eps=1e-6)
+        rel = abs(fd - ad) / max(abs(ad), 1e-300)
+        worst = max(worst, rel)
+        print(f"direction {i}: adjoint {ad:.12e} fd {fd:.12e} rel {rel:.2e}", flush=True)
+        assert rel <= TOL
+    print(f"worst relative difference {worst:.2e}", flush=True)
+
+
+def test_manning_and_depth_parts_separately(p1):
+    case, prob, J, gn, gdp = p1
+    rng = np.random.default_rng(3)
+    vn = rng.standard_normal(case.np_) * case.manning
+    vd = rng.standard_normal(case.np_) * case.dp
+    z = np.zeros(case.np_)
+    for a, b, g in ((vn, z, gn @ vn), (z, vd, gdp @ vd)):
+        fd = central_fd(prob, case, a, b, eps=1e-6)
+        assert abs(fd - g) <= TOL * abs(g)
+
+
+def test_checkpoint_spacing_does_not_change_gradient(p1):
+    case, prob, J, gn, gdp = p1
+    for interval in (1, 7, 60):
+        prob.interval = interval
+        J2, gn2, gdp2 = prob.gradient(case.manning, case.dp)
+        assert J2 == J
+        np.testing.assert_array_equal(gn2, gn)
+        np.testing.assert_array_equal(gdp2, gdp)
+    prob.interval = 0
+
+
+def test_gradient_leaves_state_unchanged(p1):
+    case, prob, J, gn, gdp = p1
+    prob.reset()
+    s0 = prob.solver.state()
+    prob.solver.gradient(prob.data, prob.weight)
+    for a, b in zip(prob.solver.state(), s0):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_torch_autograd_front_end(p1):
+    torch = pytest.importorskip("torch")
+    case, prob, J, gn, gdp = p1
+    m = torch.tensor(case.manning, dtype=torch.float64, requires_grad=True)
+    d = torch.tensor(case.dp, dtype=torch.float64, requires_grad=True)
+    loss = adjoint.gauge_misfit(m, d, prob)
+    assert float(loss) == pytest.approx(J, rel=1e-13)
+    (2.0 * loss).backward()
+    np.testing.assert_array_equal(m.grad.numpy(), 2.0 * gn)
+    np.testing.assert_array_equal(d.grad.numpy(), 2.0 * gdp)
+
+
+def test_gradient_matches_cpu_oracle_differences(p1):
+    """FP64 CPU reference: central differences of the oracle's forward model (the restatement of
+    dgswem_serial's step) in three directions."""
+    from oracle.oracle import Oracle
+    case, prob, J, gn, gdp = p1
+    elem, r, s = adjoint.seeded_gauges(case, 20, 5)
+    phi = np.stack([np.ones_like(r), (1.0 + 3.0 * s) / 2.0, (1.0 + 2.0 * r + s) / 2.0], axis=1)
+
+    def oracle_loss(mn, dp):
+        import copy
+        c = copy.copy(case)
+        c.manning = mn; c.dp = dp
+        o = Oracle(c, threads=8)
+        J = 0.0
+        for k in range(prob.nsteps):
+            o.step(1)
+            ze = o.state()[0]
+            eta = np.sum(ze[elem - 1] * phi, axis=1)
+            J += np.sum((eta - prob.data[k]) ** 2)
+        return prob.weight * J
+
+    j0 = oracle_loss(case.manning, case.dp)
+    assert j0 == pytest.approx(J, rel=1e-9)
+    eps = 1e-6
+    for vn, vd in directions(case, 3, seed=11):
+        ad = float(gn @ vn + gdp @ vd)
+        fd = (oracle_loss(case.manning + eps * vn, case.dp + eps * vd) -
+              oracle_loss(case.manning - eps * vn, case.dp - eps * vd)) / (2 * eps)
+        print(f"oracle fd {fd:.12e} adjoint {ad:.12e}", flush=True)
+        assert abs(fd - ad) <= TOL * abs(ad)
+
+
+def test_p2_gradient_matches_central_differences():
+    case = small_case(p=2, n_elem=1200, seed=32, nsteps=30)
+    case.params.dtdp = case.params.dtdp * 0.5     # p=2 needs a smaller CFL number
+    prob = adjoint.twin_problem(case, nsteps=30, ng=10, seed=6)
+    J, gn, gdp = prob.gradient(case.manning, case.dp)
+    for vn, vd in directions(case, 4, seed=12):
+        ad = float(gn @ vn + gdp @ vd)
+        fd = central_fd(prob, case, vn, vd, eps=1e-6)
+        print(f"p2 adjoint {ad:.12e} fd {fd:.12e}", flush=True)
+        assert abs(fd - ad) <= TOL * abs(ad)
+
+
+def test_adjoint_rejects_unsupported_runs():
+    case = mesh.delaunay_basin(n_elem=1500, seed=33, lx=6.0e4, ly=3.6e4, depth_range=(2.0, 30.0),
+                               nsteps=10)           # limiter and wetting/drying on
+    s = gpu.GpuSolver(case)
+    s.set_gauges(np.array([1, 2], np.int32), np.array([-0.5, -0.5]), np.array([-0.5, -0.5]))
+    with pytest.raises(gpu.GpuError, match="limiter|wetting"):
+        s.gradient(np.zeros((5, 2)))
+    s.close()
