@@ -8,6 +8,7 @@ limiter and wetting/drying on, dt from CFL 0.3.  A "step" is one full time step 
 with edge fluxes, volume quadrature, RK update, limiter and wet/dry) over all elements.
 
     python bench.py [--gpus 1] [--steps 2000] [--warmup 50] [--p 1] [--elements 1000000]
+                    [--ens-steps 200] [--adj-steps 2000] [--no-ensemble] [--no-adjoint]
 
 Prints one JSON line.  `value` = NE * 3 * Np * NRK * steps / wall time of the timed steps, with the
 state resident in HBM.  `roofline` is for the stage kernel: algorithmic bytes per launch
@@ -15,6 +16,14 @@ state resident in HBM.  `roofline` is for the stage kernel: algorithmic bytes pe
 recorded on the library's stream during the timed region.  `cpu_baseline` times the CPU oracle
 (a bit-exact restatement of dgswem_serial's step) with OpenMP on this host's cores on a short
 sample of the same mesh.
+
+The same line carries the other two parts of the metric:
+* `ensemble` -- 64 uneven spherical meshes (NE log-uniform in 20k-300k) under their own Holland
+  hurricanes, limiter and wetting/drying on: wall time of the members run one after another over
+  the wall time of the batched run (one stage-kernel launch per RK stage for all members), same
+  number of steps on the same GPU, both replaying HIP graphs;
+* `adjoint` -- gradient of a 50-gauge misfit with respect to per-node Manning n and depth on a
+  200k-element all-wet basin over 2000 steps: wall(forward + checkpointed adjoint) / wall(forward).
 """
 from __future__ import annotations
 
@@ -58,6 +67,84 @@ def cpu_baseline(case, sample_steps: int, threads: int):
     return dof / dt, dt
 
 
+def ensemble_leg(gpu, mesh, n_members: int, steps: int, warmup: int, seed: int):
+    """Batched ensemble of uneven meshes under Holland hurricanes (NE log-uniform in 20k-300k,
+    limiter and wetting/drying on, p=1) against the same members run one after another on the
+    same GPU.  Both legs replay HIP graphs and use autotuned block sizes."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(20_000), np.log(300_000), n_members)).astype(int)
+    t0 = time.perf_counter()
+    cases = [mesh.storm_basin(int(n), seed=seed + 100 + i, nsteps=warmup + steps)
+             for i, n in enumerate(sizes)]
+    log(f"ensemble: {n_members} meshes, {sum(c.ne for c in cases)} elements "
+        f"({time.perf_counter() - t0:.1f} s to generate)")
+    members = [gpu.GpuSolver(c) for c in cases]
+    seq = 0.0
+    dof_stage = 0
+    for m, c in zip(members, cases):
+        m.use_graph(True)
+        m.step(warmup)
+        m.autotune()
+        m.synchronize()
+        t1 = time.perf_counter()
+        m.step(steps, sync=False)
+        m.synchronize()
+        seq += time.perf_counter() - t1
+        dof_stage += c.ne * 3 * c.dofh * c.dg.rk_stage * steps
+    fresh = [gpu.GpuSolver(c) for c in cases]
+    ens = gpu.Ensemble(fresh)
+    for f in fresh:
+        f.close()
+    ens.use_graph(True)
+    ens.step(warmup)
+    block = ens.autotune()
+    ens.synchronize()
+    t1 = time.perf_counter()
+    ens.step(steps, sync=False)
+    ens.synchronize()
+    bat = time.perf_counter() - t1
+    ze, qx, qy, _ = ens.state()
+    finite = bool(np.isfinite(ze).all() and np.isfinite(qx).all() and np.isfinite(qy).all())
+    ens.close()
+    for m in members:
+        m.close()
+    log(f"ensemble: sequential {seq:.3f} s, batched {bat:.3f} s, speedup {seq / bat:.2f}")
+    return {"speedup": seq / bat, "members": n_members, "elements": int(sum(c.ne for c in cases)),
+            "steps": steps, "sequential_s": seq, "batched_s": bat,
+            "batched_dof_stage_per_s": dof_stage / bat, "block": block, "state_finite": finite,
+            "workload": "64 spherical Delaunay basins, NE log-uniform 20k-300k, own Holland "
+                        "hurricane each (Pmin 920-990 hPa, Rmax 20-60 km, straight/curved track), "
+                        "p=1, limiter and wetting/drying on; both legs replay HIP graphs"}
+
+
+def adjoint_leg(gpu, mesh, n_elem: int, steps: int, seed: int):
+    """Gauge-misfit gradient with respect to per-node Manning n and depth on an all-wet Delaunay
+    basin without limiter: wall(forward + checkpointed adjoint) / wall(forward)."""
+    import numpy as np
+    from dgswem_amd import adjoint
+    case = mesh.delaunay_basin(n_elem=n_elem, seed=seed, depth_range=(10.0, 3000.0), shelf=False,
+                               wetdry=False, slopeflag=0, nsteps=steps)
+    prob = adjoint.twin_problem(case, nsteps=steps, ng=50, seed=seed)
+    s = prob.solver
+    prob.loss(case.manning, case.dp)            # warm-up
+    t0 = time.perf_counter()
+    J = prob.loss(case.manning, case.dp)
+    fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J2, gn, gdp = prob.gradient(case.manning, case.dp)
+    grad = time.perf_counter() - t0
+    interval = max(1, int(round(steps ** 0.5)))
+    s.close()
+    log(f"adjoint: forward {fwd:.3f} s, forward+adjoint {grad:.3f} s, ratio {grad / fwd:.2f}")
+    return {"cost_ratio": grad / fwd, "forward_s": fwd, "gradient_s": grad, "elements": case.ne,
+            "nodes": case.np_, "steps": steps, "checkpoint_interval": interval, "loss": J2,
+            "params": 2 * case.np_,
+            "workload": "all-wet seeded Delaunay basin (GRF depth 10-3000 m), p=1, no limiter, "
+                        "50 gauges, misfit against a perturbed-Manning twin; gradient w.r.t. "
+                        "per-node Manning n and depth"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +157,13 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0,
                     help="stage-kernel block size (0 = autotune on the warmed-up state)")
+    ap.add_argument("--ens-members", type=int, default=64)
+    ap.add_argument("--ens-steps", type=int, default=200,
+                    help="timed steps of each ensemble leg (the ratio is per step)")
+    ap.add_argument("--adj-elements", type=int, default=200_000)
+    ap.add_argument("--adj-steps", type=int, default=2000)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -129,6 +223,12 @@ def main() -> int:
                "sample": f"{steps_cpu} steps of the same {ne}-element mesh ({secs:.1f} s) with the "
                          f"OpenMP oracle (bit-exact restatement of dgswem_serial's step)"}
 
+    g.close()
+    ens = None if args.no_ensemble else ensemble_leg(gpu, mesh, args.ens_members, args.ens_steps,
+                                                     10, args.seed)
+    adj = None if args.no_adjoint else adjoint_leg(gpu, mesh, args.adj_elements, args.adj_steps,
+                                                   args.seed)
+
     out = {
         "metric": "DOF-stage updates/s FP64 p=1 1M elems",
         "value": value,
@@ -154,11 +254,12 @@ def main() -> int:
                      "algorithmic_bytes_per_launch": alg_bytes},
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "ensemble": ens,
+        "adjoint": adj,
     }
     if cpu:
         out["speedup_vs_cpu_baseline"] = value / cpu["value"]
     print(json.dumps(out), flush=True)
-    g.close()
     return 0
 
 

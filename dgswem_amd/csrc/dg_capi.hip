@@ -172,6 +172,8 @@ DevTables to_dev(const RefTables& t) {
     std::memcpy(d.m_inv, t.m_inv, sizeof d.m_inv);
     std::memcpy(d.wegp, t.wegp, sizeof d.wegp);
     std::memcpy(d.xegp, t.xegp, sizeof d.xegp);
+    static_assert(sizeof d.psi == sizeof t.psi, "psi table layout");
+    std::memcpy(d.psi, t.psi, sizeof d.psi);
     return d;
 }
 
