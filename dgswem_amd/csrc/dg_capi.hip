@@ -25,6 +25,7 @@
 
 // the kernels are compiled in this translation unit
 #include "dg_kernels.hip"
+#include "dg_stage3.hip"
 #include "dg_adjoint.hip"
 
 using namespace dg;
@@ -683,6 +684,25 @@ int build_stage_tables(Ctx& c, int it, int n) {
     return 0;
 }
 
+// Triplet variant: NB elements and 3*NB threads per block.
+template <int P, int NB, bool MET>
+void launch_stage3_met(Ctx& c, const StageArgs& a, bool sph, bool fuse) {
+    const int grid = (c.ne + NB - 1) / NB;
+    if (sph) {
+        if (fuse) hipLaunchKernelGGL((stage3_kernel<P, NB, true, true, MET>), dim3(grid), dim3(3 * NB), 0, c.stream, a);
+        else hipLaunchKernelGGL((stage3_kernel<P, NB, true, false, MET>), dim3(grid), dim3(3 * NB), 0, c.stream, a);
+    } else {
+        if (fuse) hipLaunchKernelGGL((stage3_kernel<P, NB, false, true, MET>), dim3(grid), dim3(3 * NB), 0, c.stream, a);
+        else hipLaunchKernelGGL((stage3_kernel<P, NB, false, false, MET>), dim3(grid), dim3(3 * NB), 0, c.stream, a);
+    }
+}
+
+template <int P, int NB>
+void launch_stage3(Ctx& c, const StageArgs& a, bool sph, bool fuse) {
+    if (c.met) launch_stage3_met<P, NB, true>(c, a, sph, fuse);
+    else launch_stage3_met<P, NB, false>(c, a, sph, fuse);
+}
+
 template <int P, int NT, bool MET>
 void launch_stage_met(Ctx& c, const StageArgs& a, bool sph, bool fuse) {
     const int grid = (c.ne + NT - 1) / NT;
@@ -701,9 +721,25 @@ void launch_stage(Ctx& c, const StageArgs& a, bool sph, bool fuse) {
     else launch_stage_met<P, NT, false>(c, a, sph, fuse);
 }
 
-// Block sizes: p=1 64/128/256 threads; p=2 64/128 (its LDS accumulators double per thread).
+// Launch shapes, by threads per block: 64, 128, 256 (p=1 only) run stage_kernel with one thread
+// per element; 192 and 384 (p=1 only) run stage3_kernel with three threads per element.
+bool valid_block(int p, int nt) {
+    if (p == 1) return nt == 64 || nt == 128 || nt == 256 || nt == 192 || nt == 384;
+    return nt == 64 || nt == 128 || nt == 192;
+}
+
 template <int P>
 void launch_stage_nt(Ctx& c, const StageArgs& a, bool sph, bool fuse, int nt) {
+    if (nt == 192) {
+        launch_stage3<P, 64>(c, a, sph, fuse);
+        return;
+    }
+    if constexpr (P == 1) {
+        if (nt == 384) {
+            launch_stage3<P, 128>(c, a, sph, fuse);
+            return;
+        }
+    }
     if (nt == 64) {
         launch_stage<P, 64>(c, a, sph, fuse);
     } else if constexpr (P == 1) {
@@ -1440,7 +1476,8 @@ int dg_gpu_autotune(int h, int* block) {
     StageArgs a = stage_args(*c, 0, slot);
     a.rout = nullptr;
     const bool sph = c->prm.ics == 2, fuse = c->prm.nolifa >= 2 && c->prm.slopeflag == 0;
-    const std::vector<int> cand = c->prm.p == 1 ? std::vector<int>{64, 128, 256} : std::vector<int>{64, 128};
+    const std::vector<int> cand = c->prm.p == 1 ? std::vector<int>{64, 128, 256, 192, 384}
+                                                : std::vector<int>{64, 128, 192};
     if (build_stage_tables(*c, c->next_it, 1)) return 1;
     hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->step.p, 1, 0);
     HIPCHK(hipGetLastError());
@@ -1485,8 +1522,8 @@ int dg_gpu_autotune(int h, int* block) {
 int dg_gpu_set_block(int h, int block) {
     Ctx* c = get(h);
     if (!c) return 1;
-    if (block != 64 && block != 128 && !(block == 256 && c->prm.p == 1))
-        return fail("block size must be 64, 128 or (p=1) 256");
+    if (!valid_block(c->prm.p, block))
+        return fail("block size must be 64, 128, 192 or, for p=1, 256 or 384 threads");
     HIPCHK(hipStreamSynchronize(c->stream));
     if (block != c->block) { c->block = block; c->drop_graphs(); }
     return 0;

@@ -186,11 +186,14 @@ int dg_ens_layout(int handle, int* nmembers, int* elem_offsets, int* node_offset
  * only the eager remainder of each call. */
 int dg_gpu_use_graph(int handle, int enable);
 
-/* Time the stage kernel at each supported block size on the handle's current state and keep the
- * fastest; writes it to *block.  Touches only scratch stage buffers: the state is unchanged. */
+/* Time the stage kernel at each supported launch shape (see dg_gpu_set_block) on the handle's
+ * current state and keep the fastest; writes it to *block.  Touches only scratch stage buffers:
+ * the state is unchanged. */
 int dg_gpu_autotune(int handle, int* block);
 
-/* Set the stage-kernel block size explicitly: 64, 128, or 256 (p=1 only). */
+/* Set the stage-kernel launch shape explicitly, as threads per block: 64, 128 or (p=1) 256 run
+ * one thread per element; 192 or (p=1) 384 run three threads per element (64 or 128 elements per
+ * block).  All shapes give bit-identical results. */
 int dg_gpu_set_block(int handle, int block);
 
 /* Message of the last failure on this thread ("" if none). */

@@ -135,6 +135,11 @@ def main() -> None:
     generated_case("delaunay4k", mesh.delaunay_basin(n_elem=4000, seed=1, lx=6.0e4, ly=3.6e4,
                                                      depth_range=(2.0, 30.0), cfl=0.3),
                    [100, 1000])
+    # wetting and drying without the slope limiter (WETDRY fused into the stage)
+    generated_case("delaunay4k_nolim", mesh.delaunay_basin(n_elem=4000, seed=2, lx=6.0e4, ly=3.6e4,
+                                                           depth_range=(2.0, 30.0), cfl=0.3,
+                                                           slopeflag=0),
+                   [300])
     generated_case("storm3k", storm_case(), [100, 500])
     reference_case("quarter_annular", [100, 2000])
     reference_case("wetdry", [50, 500])
