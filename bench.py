@@ -216,7 +216,7 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
-        steps_cpu = args.cpu_steps or max(2, int(30 * 1_000_000 / max(ne, 1)))
+        steps_cpu = args.cpu_steps or max(2, int(150 * 1_000_000 / max(ne, 1)))
         log(f"cpu baseline: oracle, {threads} threads, {steps_cpu} steps")
         rate, secs = cpu_baseline(case, steps_cpu, threads)
         cpu = {"value": rate, "unit": "DOF-stage updates/s", "cores": threads, "kind": "port",

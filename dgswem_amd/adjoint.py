@@ -61,14 +61,29 @@ def seeded_gauges(case: Case, ng: int, seed: int):
     return elem.astype(np.int32), r, s
 
 
+def hump_state(case: Case, amplitude: float = 0.5, width: float = 0.25):
+    """Initial state at rest with a Gaussian elevation hump centred in the basin (cell means only),
+    so that the whole domain moves from the first step."""
+    x = case.x[case.nm - 1].mean(axis=1); y = case.y[case.nm - 1].mean(axis=1)
+    cx, cy = 0.5 * (case.x.min() + case.x.max()), 0.5 * (case.y.min() + case.y.max())
+    L = width * max(np.ptp(case.x), np.ptp(case.y))
+    ze = np.zeros((case.ne, case.dofh))
+    ze[:, 0] = amplitude * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / L ** 2)
+    z = np.zeros_like(ze)
+    return ze, z, z.copy(), np.ones(case.ne, np.int32)
+
+
 def twin_problem(case: Case, nsteps: int, ng: int = 50, seed: int = 0, perturb: float = 0.2,
-                 weight: float | None = None, interval: int = 0) -> MisfitProblem:
+                 weight: float | None = None, interval: int = 0, state0=None) -> MisfitProblem:
     """Gauge data from a twin run whose Manning field is perturbed by a seeded smooth relative
-    factor (1 + perturb * field), then a problem set up at the unperturbed parameters."""
+    factor (1 + perturb * field), then a problem set up at the unperturbed parameters.  The runs
+    start from `state0` (default: a Gaussian hump at rest, see hump_state)."""
     s = gpu.GpuSolver(case)
     elem, r, sr = seeded_gauges(case, ng, seed)
     s.set_gauges(elem, r, sr)
-    state0 = s.state()
+    if state0 is None:
+        state0 = hump_state(case)
+    s.set_state(*state0)
     rng = np.random.default_rng(seed + 1)
     kx, ky, ph = rng.uniform(1.0, 3.0), rng.uniform(1.0, 3.0), rng.uniform(0.0, 2 * np.pi)
     x = (case.x - case.x.min()) / np.ptp(case.x); y = (case.y - case.y.min()) / np.ptp(case.y)

@@ -404,7 +404,7 @@ __global__ void __launch_bounds__(NT) adj_stage_kernel(AdjArgs a) {
             bool fric_active = false;
             double pw = 0.0;
             if (ph.load_manning) {
-                pw = pow(depth, 1.0 / 3.0);
+                pw = cbrt(depth);
                 fric = ph.g * (nbar * nbar) / pw;
                 if (fric < sc.cf) fric = sc.cf;
                 else fric_active = true;

@@ -25,7 +25,6 @@
 
 // the kernels are compiled in this translation unit
 #include "dg_kernels.hip"
-#include "dg_stage3.hip"
 #include "dg_adjoint.hip"
 
 using namespace dg;
@@ -684,62 +683,39 @@ int build_stage_tables(Ctx& c, int it, int n) {
     return 0;
 }
 
-// Triplet variant: NB elements and 3*NB threads per block.
-template <int P, int NB, bool MET>
-void launch_stage3_met(Ctx& c, const StageArgs& a, bool sph, bool fuse) {
-    const int grid = (c.ne + NB - 1) / NB;
-    if (sph) {
-        if (fuse) hipLaunchKernelGGL((stage3_kernel<P, NB, true, true, MET>), dim3(grid), dim3(3 * NB), 0, c.stream, a);
-        else hipLaunchKernelGGL((stage3_kernel<P, NB, true, false, MET>), dim3(grid), dim3(3 * NB), 0, c.stream, a);
-    } else {
-        if (fuse) hipLaunchKernelGGL((stage3_kernel<P, NB, false, true, MET>), dim3(grid), dim3(3 * NB), 0, c.stream, a);
-        else hipLaunchKernelGGL((stage3_kernel<P, NB, false, false, MET>), dim3(grid), dim3(3 * NB), 0, c.stream, a);
-    }
-}
-
-template <int P, int NB>
-void launch_stage3(Ctx& c, const StageArgs& a, bool sph, bool fuse) {
-    if (c.met) launch_stage3_met<P, NB, true>(c, a, sph, fuse);
-    else launch_stage3_met<P, NB, false>(c, a, sph, fuse);
-}
-
-template <int P, int NT, bool MET>
-void launch_stage_met(Ctx& c, const StageArgs& a, bool sph, bool fuse) {
+template <int P, int NT, bool MET, bool NL>
+void launch_stage_v(Ctx& c, const StageArgs& a, bool sph, bool fuse) {
     const int grid = (c.ne + NT - 1) / NT;
     if (sph) {
-        if (fuse) hipLaunchKernelGGL((stage_kernel<P, NT, true, true, MET>), dim3(grid), dim3(NT), 0, c.stream, a);
-        else hipLaunchKernelGGL((stage_kernel<P, NT, true, false, MET>), dim3(grid), dim3(NT), 0, c.stream, a);
+        if (fuse) hipLaunchKernelGGL((stage_kernel<P, NT, true, true, MET, NL>), dim3(grid), dim3(NT), 0, c.stream, a);
+        else hipLaunchKernelGGL((stage_kernel<P, NT, true, false, MET, NL>), dim3(grid), dim3(NT), 0, c.stream, a);
     } else {
-        if (fuse) hipLaunchKernelGGL((stage_kernel<P, NT, false, true, MET>), dim3(grid), dim3(NT), 0, c.stream, a);
-        else hipLaunchKernelGGL((stage_kernel<P, NT, false, false, MET>), dim3(grid), dim3(NT), 0, c.stream, a);
+        if (fuse) hipLaunchKernelGGL((stage_kernel<P, NT, false, true, MET, NL>), dim3(grid), dim3(NT), 0, c.stream, a);
+        else hipLaunchKernelGGL((stage_kernel<P, NT, false, false, MET, NL>), dim3(grid), dim3(NT), 0, c.stream, a);
     }
 }
 
+// Nonlinear runs (NOLICAT != 0) use the kernels with NLEQ/LEQ folded at compile time; linearised
+// runs use the general kernels with 64-thread blocks.
 template <int P, int NT>
 void launch_stage(Ctx& c, const StageArgs& a, bool sph, bool fuse) {
-    if (c.met) launch_stage_met<P, NT, true>(c, a, sph, fuse);
-    else launch_stage_met<P, NT, false>(c, a, sph, fuse);
+    if (c.prm.nolicat != 0) {
+        if (c.met) launch_stage_v<P, NT, true, true>(c, a, sph, fuse);
+        else launch_stage_v<P, NT, false, true>(c, a, sph, fuse);
+    } else {
+        if (c.met) launch_stage_v<P, 64, true, false>(c, a, sph, fuse);
+        else launch_stage_v<P, 64, false, false>(c, a, sph, fuse);
+    }
 }
 
-// Launch shapes, by threads per block: 64, 128, 256 (p=1 only) run stage_kernel with one thread
-// per element; 192 and 384 (p=1 only) run stage3_kernel with three threads per element.
+// Launch shapes, by threads per block (one thread per element): 64, 128, and 256 for p=1.
 bool valid_block(int p, int nt) {
-    if (p == 1) return nt == 64 || nt == 128 || nt == 256 || nt == 192 || nt == 384;
-    return nt == 64 || nt == 128 || nt == 192;
+    if (p == 1) return nt == 64 || nt == 128 || nt == 256;
+    return nt == 64 || nt == 128;
 }
 
 template <int P>
 void launch_stage_nt(Ctx& c, const StageArgs& a, bool sph, bool fuse, int nt) {
-    if (nt == 192) {
-        launch_stage3<P, 64>(c, a, sph, fuse);
-        return;
-    }
-    if constexpr (P == 1) {
-        if (nt == 384) {
-            launch_stage3<P, 128>(c, a, sph, fuse);
-            return;
-        }
-    }
     if (nt == 64) {
         launch_stage<P, 64>(c, a, sph, fuse);
     } else if constexpr (P == 1) {
@@ -1476,8 +1452,7 @@ int dg_gpu_autotune(int h, int* block) {
     StageArgs a = stage_args(*c, 0, slot);
     a.rout = nullptr;
     const bool sph = c->prm.ics == 2, fuse = c->prm.nolifa >= 2 && c->prm.slopeflag == 0;
-    const std::vector<int> cand = c->prm.p == 1 ? std::vector<int>{64, 128, 256, 192, 384}
-                                                : std::vector<int>{64, 128, 192};
+    const std::vector<int> cand = c->prm.p == 1 ? std::vector<int>{64, 128, 256} : std::vector<int>{64, 128};
     if (build_stage_tables(*c, c->next_it, 1)) return 1;
     hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->step.p, 1, 0);
     HIPCHK(hipGetLastError());
@@ -1523,7 +1498,7 @@ int dg_gpu_set_block(int h, int block) {
     Ctx* c = get(h);
     if (!c) return 1;
     if (!valid_block(c->prm.p, block))
-        return fail("block size must be 64, 128, 192 or, for p=1, 256 or 384 threads");
+        return fail("block size must be 64, 128 or, for p=1, 256 threads");
     HIPCHK(hipStreamSynchronize(c->stream));
     if (block != c->block) { c->block = block; c->drop_graphs(); }
     return 0;

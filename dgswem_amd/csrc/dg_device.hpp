@@ -160,4 +160,61 @@ __device__ __forceinline__ void llf_flux(const Phys& P, double ze_in, double ze_
     f2 = a3 - 0.5 * em * (j3 * ny2 + LEQ * j2 * nxy);
 }
 
+// llf_flux for the nonlinear equations (NOLICAT != 0: NLEQ = 1, LEQ = 0) with those constants
+// folded by hand: x*NLEQ = x, NLEQ + LEQ*h = 1 and y + LEQ*z = y hold exactly for finite operands,
+// so the values equal llf_flux's bit for bit (up to the sign of an exact zero).
+__device__ __forceinline__ void llf_flux_nl(double ze_in, double ze_ex, double qx_in, double qx_ex,
+                                            double qy_in, double qy_ex, double hb_in, double hb_ex,
+                                            double sf_in, double sf_ex, double nx, double ny, double G,
+                                            double& f0, double& f1, double& f2) {
+    const double j1 = ze_ex - ze_in, j2 = qx_ex - qx_in, j3 = qy_ex - qy_in;
+    const double ht_in = ze_in + hb_in;
+    const double fx1_in = qx_in * sf_in, fy1_in = qy_in;
+    double fg = G * ze_in;
+    double fh = 1.0 / ht_in;
+    const double u_in = qx_in * fh, v_in = qy_in * fh;
+    double huu = qx_in * u_in, hvv = qy_in * v_in, huv = qx_in * v_in;
+    double gh2 = fg * (0.5 * ze_in + hb_in);
+    const double fx2_in = (huu + gh2) * sf_in, fy2_in = huv;
+    const double fx3_in = huv * sf_in, fy3_in = hvv + gh2;
+    const double ht_ex = ze_ex + hb_ex;
+    const double fx1_ex = qx_ex * sf_ex, fy1_ex = qy_ex;
+    fg = G * ze_ex;
+    fh = 1.0 / ht_ex;
+    const double u_ex = qx_ex * fh, v_ex = qy_ex * fh;
+    huu = qx_ex * u_ex; hvv = qy_ex * v_ex; huv = qx_ex * v_ex;
+    gh2 = fg * (0.5 * ze_ex + hb_ex);
+    const double fx2_ex = (huu + gh2) * sf_ex, fy2_ex = huv;
+    const double fx3_ex = huv * sf_ex, fy3_ex = hvv + gh2;
+    const double a1 = 0.5 * ((fx1_in + fx1_ex) * nx + (fy1_in + fy1_ex) * ny);
+    const double a2 = 0.5 * ((fx2_in + fx2_ex) * nx + (fy2_in + fy2_ex) * ny);
+    const double a3 = 0.5 * ((fx3_in + fx3_ex) * nx + (fy3_in + fy3_ex) * ny);
+    const double un_in = u_in * nx + v_in * ny;
+    double nxs = nx * sf_in;
+    const double c_in = sqrt(G * ht_in * (ny * ny + nxs * nxs));
+    const double un_ex = u_ex * nx + v_ex * ny;
+    nxs = nx * sf_ex;
+    const double c_ex = sqrt(G * ht_ex * (ny * ny + nxs * nxs));
+    double em = fabs(un_in + c_in);
+    em = fmax(em, fabs(un_in));
+    em = fmax(em, fabs(un_in - c_in));
+    em = fmax(em, fabs(un_ex + c_ex));
+    em = fmax(em, fabs(un_ex));
+    em = fmax(em, fabs(un_ex - c_ex));
+    f0 = a1 - 0.5 * em * (j1);
+    f1 = a2 - 0.5 * em * (j2);
+    f2 = a3 - 0.5 * em * (j3);
+}
+
+template <bool NL>
+__device__ __forceinline__ void llf(const Phys& P, double ze_in, double ze_ex, double qx_in, double qx_ex,
+                                    double qy_in, double qy_ex, double hb_in, double hb_ex, double sf_in,
+                                    double sf_ex, double nx, double ny, double G, double& f0, double& f1,
+                                    double& f2) {
+    if constexpr (NL)
+        llf_flux_nl(ze_in, ze_ex, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_ex, sf_in, sf_ex, nx, ny, G, f0, f1, f2);
+    else
+        llf_flux(P, ze_in, ze_ex, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_ex, sf_in, sf_ex, nx, ny, G, f0, f1, f2);
+}
+
 }  // namespace dg

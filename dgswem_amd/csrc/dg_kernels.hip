@@ -178,7 +178,7 @@ __device__ __forceinline__ double trace(const double (&u)[DOF], const double* ph
     return s;
 }
 
-template <int P, int NT, bool SPH, bool WDFUSED, bool MET>
+template <int P, int NT, bool SPH, bool WDFUSED, bool MET, bool NL>
 __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
     constexpr int DOF = (P + 1) * (P + 2) / 2;
     constexpr int NAGP = (P == 1) ? 3 : 6;
@@ -271,7 +271,7 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
                 const double hb_in = role ? hn : ho, hb_ex = role ? ho : hn;
                 const double sf_in = role ? sn : so, sf_ex = role ? so : sn;
                 double f0, f1, f2;
-                llf_flux(ph, ze_in, ze_ex, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_ex, sf_in, sf_ex, nx, ny, ph.g, f0, f1, f2);
+                llf<NL>(ph, ze_in, ze_ex, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_ex, sf_in, sf_ex, nx, ny, ph.g, f0, f1, f2);
                 double fo = f0, gm = role ? f1 : f1, hm = f2;
                 // dry-side treatment (src/internal_edge_hydro.F:212-302): 1 reflect into a dry
                 // "in" element, 2 zero-gravity flux for a dry "in", 3 reflect into a dry "ex",
@@ -298,7 +298,7 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
                         G = 0.0;
                     }
                     double r0, r1, r2;
-                    llf_flux(ph, a_zi, a_ze, a_xi, a_xe, a_yi, a_ye, a_hi, a_he, a_si, a_se, nx, ny, G, r0, r1, r2);
+                    llf<NL>(ph, a_zi, a_ze, a_xi, a_xe, a_yi, a_ye, a_hi, a_he, a_si, a_se, nx, ny, G, r0, r1, r2);
                     // mode 1: fo, ex momentum from the reflected flux, zero for the dry "in"
                     // mode 3: fo, in momentum from the reflected flux, zero for the dry "ex"
                     if (mode == 1) { fo = r0; gm = role ? r1 : 0.0; hm = role ? r2 : 0.0; }
@@ -335,7 +335,7 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
                     const double den = 1.0 / (nx * ty - ny * tx);
                     const double qx_ex = (ty * qn_ext - ny * qt_ext) * den;
                     const double qy_ex = (-tx * qn_ext + nx * qt_ext) * den;
-                    llf_flux(ph, ze_in, ze_in, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_in, sf, sf, nx, ny, ph.g, f0, f1, f2);
+                    llf<NL>(ph, ze_in, ze_in, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_in, sf, sf, nx, ny, ph.g, f0, f1, f2);
                 } else if (kind == SIDE_OCEAN) {
                     double ze_ex = 0.0;
                     const double xe = T.xegp[I];
@@ -349,7 +349,7 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
                         ze_ex = ze_ex + (0.5 * (z1 + z2) + 0.5 * (z2 - z1) * xe);
                     }
                     if ((ze_ex * ph.ifnlfa + hb_in) <= 0) ze_ex = fabs(hb_in) + sc.h0;
-                    llf_flux(ph, ze_in, ze_ex, qx_in, qx_in, qy_in, qy_in, hb_in, hb_in, sf, sf, nx, ny, ph.g, f0, f1, f2);
+                    llf<NL>(ph, ze_in, ze_ex, qx_in, qx_in, qy_in, qy_in, hb_in, hb_in, sf, sf, nx, ny, ph.g, f0, f1, f2);
                 } else {
                     double qn_ext = 0.0, qx_ex = 0.0, qy_ex = 0.0;
                     const double xe = T.xegp[I];
@@ -367,7 +367,7 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
                         qx_ex = -(ty * qn_ext - ny * qt_ext) / (nx * ty - ny * tx);
                         qy_ex = -(-tx * qn_ext + nx * qt_ext) / (nx * ty - ny * tx);
                     }
-                    llf_flux(ph, ze_in, ze_in, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_in, sf, sf, nx, ny, ph.g, f0, f1, f2);
+                    llf<NL>(ph, ze_in, ze_in, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_in, sf, sf, nx, ny, ph.g, f0, f1, f2);
                 }
 #pragma unroll
                 for (int k = 0; k < DOF; k++) {
@@ -427,7 +427,7 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
             met_lds[7][tid] = pdy;
         }
         const double depth_c = hb[0] + U[0][0];
-        const double fh_nl_c = 1.0 / (ph.nleq * depth_c + ph.leq);
+        const double fh_nl_c = NL ? 1.0 / depth_c : 1.0 / (ph.nleq * depth_c + ph.leq);
         const double cori = E.cori_el ? E.cori_el[e] : sc.cori;
         const double nbar = ph.load_manning ? E.nbar[e] : 0.0;
 #pragma unroll 1
@@ -451,19 +451,21 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
             }
             const double sfq = SPH ? E.sfacq[(size_t)I * ne + e] : 1.0;
             const double depth = ze + hbq;
-            const double f1 = ph.nleq + ph.leq * hbq;
-            const double fx = qx * f1 * sfq, fy = qy * f1;
-            const double fu = ph.nleq * qx, fv = ph.nleq * qy;
+            // with NL (NLEQ = 1, LEQ = 0) the constant factors fold exactly
+            const double f1 = NL ? 1.0 : ph.nleq + ph.leq * hbq;
+            const double fx = NL ? qx * sfq : qx * f1 * sfq, fy = NL ? qy : qy * f1;
+            const double fu = NL ? qx : ph.nleq * qx, fv = NL ? qy : ph.nleq * qy;
             const double fg = ph.nleqg * ze * wd;
-            const double fh = 1.0 / (ph.nleq * depth + ph.leq);
+            const double fh = NL ? 1.0 / depth : 1.0 / (ph.nleq * depth + ph.leq);
             const double u = qx * fh, v = qy * fh;
             const double huu = fu * u, hvv = fv * v, huv = fu * v;
-            const double gh2 = fg * (0.5 * ze + hbq) + ph.fg_l * ze;
+            const double gh2 = NL ? fg * (0.5 * ze + hbq) : fg * (0.5 * ze + hbq) + ph.fg_l * ze;
             const double gx = (huu + gh2) * sfq, gy = huv;
             const double hx = huv * sfq, hy = hvv + gh2;
             double fric = sc.fric;
             if (ph.load_manning) {
-                fric = ph.g * (nbar * nbar) / pow(depth, 1.0 / 3.0);
+                // DEPTH**(1/3) (src/rhs_dg_hydro.F:302); cbrt is within an ulp of pow(x, 1/3)
+                fric = ph.g * (nbar * nbar) / cbrt(depth);
                 if (fric < sc.cf) fric = sc.cf;
             }
             double tau = fric;
@@ -478,12 +480,12 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
             double sy = -(tau * qy) + fg * dhby - cori * qx;
             if (MET) {
                 // wind stress and atmospheric pressure (src/rhs_dg_hydro.F:362-373)
-                const double fw = 1.0 / f1;
                 const double* ps = T.psi[I];
-                sx = sx + fw * (met_lds[0][tid] * ps[0] + met_lds[1][tid] * ps[1] + met_lds[2][tid] * ps[2]) -
-                     ph.g * sfq * depth * met_lds[6][tid];
-                sy = sy + fw * (met_lds[3][tid] * ps[0] + met_lds[4][tid] * ps[1] + met_lds[5][tid] * ps[2]) -
-                     ph.g * depth * met_lds[7][tid];
+                const double wsx = met_lds[0][tid] * ps[0] + met_lds[1][tid] * ps[1] + met_lds[2][tid] * ps[2];
+                const double wsy = met_lds[3][tid] * ps[0] + met_lds[4][tid] * ps[1] + met_lds[5][tid] * ps[2];
+                const double fw = 1.0 / f1;
+                sx = sx + (NL ? wsx : fw * wsx) - ph.g * sfq * depth * met_lds[6][tid];
+                sy = sy + (NL ? wsy : fw * wsy) - ph.g * depth * met_lds[7][tid];
             }
             R[1][0] = R[1][0] + T.srfac[I][0] * sx;
             R[2][0] = R[2][0] + T.srfac[I][0] * sy;

@@ -191,9 +191,8 @@ int dg_gpu_use_graph(int handle, int enable);
  * the state is unchanged. */
 int dg_gpu_autotune(int handle, int* block);
 
-/* Set the stage-kernel launch shape explicitly, as threads per block: 64, 128 or (p=1) 256 run
- * one thread per element; 192 or (p=1) 384 run three threads per element (64 or 128 elements per
- * block).  All shapes give bit-identical results. */
+/* Set the stage-kernel block size explicitly (one thread per element): 64, 128 or (p=1) 256.
+ * All sizes give bit-identical results. */
 int dg_gpu_set_block(int handle, int block);
 
 /* Message of the last failure on this thread ("" if none). */

@@ -57,7 +57,7 @@ def test_gradient_matches_central_differences_20_directions(p1):
     worst = 0.0
     for i, (vn, vd) in enumerate(directions(case, 20, seed=77)):
         ad = float(gn @ vn + gdp @ vd)
-        fd = central_fd(prob, case, vn, vd, eps=1e-6)
+        fd = central_fd(prob, case, vn, vd, eps=1e-5)
         rel = abs(fd - ad) / max(abs(ad), 1e-300)
         worst = max(worst, rel)
         print(f"direction {i}: adjoint {ad:.12e} fd {fd:.12e} rel {rel:.2e}", flush=True)
@@ -72,7 +72,7 @@ def test_manning_and_depth_parts_separately(p1):
     vd = rng.standard_normal(case.np_) * case.dp
     z = np.zeros(case.np_)
     for a, b, g in ((vn, z, gn @ vn), (z, vd, gdp @ vd)):
-        fd = central_fd(prob, case, a, b, eps=1e-6)
+        fd = central_fd(prob, case, a, b, eps=1e-5)
         assert abs(fd - g) <= TOL * abs(g)
 
 
@@ -102,7 +102,7 @@ def test_torch_autograd_front_end(p1):
     m = torch.tensor(case.manning, dtype=torch.float64, requires_grad=True)
     d = torch.tensor(case.dp, dtype=torch.float64, requires_grad=True)
     loss = adjoint.gauge_misfit(m, d, prob)
-    assert float(loss) == pytest.approx(J, rel=1e-13)
+    assert float(loss.detach()) == pytest.approx(J, rel=1e-13)
     (2.0 * loss).backward()
     np.testing.assert_array_equal(m.grad.numpy(), 2.0 * gn)
     np.testing.assert_array_equal(d.grad.numpy(), 2.0 * gdp)
@@ -116,11 +116,14 @@ def test_gradient_matches_cpu_oracle_differences(p1):
     elem, r, s = adjoint.seeded_gauges(case, 20, 5)
     phi = np.stack([np.ones_like(r), (1.0 + 3.0 * s) / 2.0, (1.0 + 2.0 * r + s) / 2.0], axis=1)
 
+    ze0, qx0, qy0, wd0 = prob.state0
+
     def oracle_loss(mn, dp):
         import copy
         c = copy.copy(case)
         c.manning = mn; c.dp = dp
         o = Oracle(c, threads=8)
+        o.set_state(ze0, qx0, qy0, wd0)
         J = 0.0
         for k in range(prob.nsteps):
             o.step(1)
@@ -131,7 +134,7 @@ def test_gradient_matches_cpu_oracle_differences(p1):
 
     j0 = oracle_loss(case.manning, case.dp)
     assert j0 == pytest.approx(J, rel=1e-9)
-    eps = 1e-6
+    eps = 1e-5
     for vn, vd in directions(case, 3, seed=11):
         ad = float(gn @ vn + gdp @ vd)
         fd = (oracle_loss(case.manning + eps * vn, case.dp + eps * vd) -
@@ -147,7 +150,7 @@ def test_p2_gradient_matches_central_differences():
     J, gn, gdp = prob.gradient(case.manning, case.dp)
     for vn, vd in directions(case, 4, seed=12):
         ad = float(gn @ vn + gdp @ vd)
-        fd = central_fd(prob, case, vn, vd, eps=1e-6)
+        fd = central_fd(prob, case, vn, vd, eps=1e-5)
         print(f"p2 adjoint {ad:.12e} fd {fd:.12e}", flush=True)
         assert abs(fd - ad) <= TOL * abs(ad)
 
