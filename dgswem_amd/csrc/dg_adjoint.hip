@@ -19,12 +19,14 @@
 namespace dg {
 
 // Reverse of llf_flux: given fb = d(loss)/d(f0,f1,f2), returns the cotangents of
-// (ze_in, ze_ex, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_ex) in bar[0..7].
+// (ze_in, ze_ex, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_ex) in bar[0..7].  NL: the nonlinear
+// equations (NLEQ = 1, LEQ = 0) with the vanishing terms left out.
+template <bool NL>
 __device__ __forceinline__ void llf_flux_adj(const Phys& P, double zi, double ze, double xi, double xe,
                                              double yi, double ye, double hi, double he, double sfi,
                                              double sfe, double nx, double ny, double G,
                                              const double fb[3], double bar[8]) {
-    const double NLEQ = P.nleq, LEQ = P.leq, FG_L = P.fg_l;
+    const double NLEQ = NL ? 1.0 : P.nleq, LEQ = NL ? 0.0 : P.leq, FG_L = NL ? 0.0 : P.fg_l;
     const double NLEQG = G * NLEQ;
     // ---- forward values needed by the reverse sweep
     const double j1 = ze - zi, j2 = xe - xi, j3 = ye - yi;
@@ -34,7 +36,7 @@ __device__ __forceinline__ void llf_flux_adj(const Phys& P, double zi, double ze
 #pragma unroll
     for (int s = 0; s < 2; s++) {
         ht[s] = side_z[s] * NLEQ + side_h[s];
-        fh[s] = 1.0 / (NLEQ * ht[s] + LEQ);
+        fh[s] = NL ? 1.0 / ht[s] : 1.0 / (NLEQ * ht[s] + LEQ);
         u[s] = side_x[s] * fh[s];
         v[s] = side_y[s] * fh[s];
         un[s] = (u[s] * nx + v[s] * ny) * NLEQ;
@@ -50,14 +52,22 @@ __device__ __forceinline__ void llf_flux_adj(const Phys& P, double zi, double ze
 #pragma unroll
     for (int q = 1; q < 6; q++)
         if (cand[q] > em) { em = cand[q]; arg = q; }
-    const double nx2 = NLEQ + LEQ * nx * nx, ny2 = NLEQ + LEQ * ny * ny, nxy = nx * ny;
     // ---- reverse
     const double a1b = fb[0], a2b = fb[1], a3b = fb[2];
-    const double emb = -0.5 * j1 * fb[0] - 0.5 * (j2 * nx2 + LEQ * j3 * nxy) * fb[1] -
-                       0.5 * (j3 * ny2 + LEQ * j2 * nxy) * fb[2];
-    const double j1b = -0.5 * em * fb[0];
-    const double j2b = -0.5 * em * (nx2 * fb[1] + LEQ * nxy * fb[2]);
-    const double j3b = -0.5 * em * (LEQ * nxy * fb[1] + ny2 * fb[2]);
+    double emb, j1b, j2b, j3b;
+    if constexpr (NL) {
+        emb = -0.5 * (j1 * fb[0] + j2 * fb[1] + j3 * fb[2]);
+        j1b = -0.5 * em * fb[0];
+        j2b = -0.5 * em * fb[1];
+        j3b = -0.5 * em * fb[2];
+    } else {
+        const double nx2 = NLEQ + LEQ * nx * nx, ny2 = NLEQ + LEQ * ny * ny, nxy = nx * ny;
+        emb = -0.5 * j1 * fb[0] - 0.5 * (j2 * nx2 + LEQ * j3 * nxy) * fb[1] -
+              0.5 * (j3 * ny2 + LEQ * j2 * nxy) * fb[2];
+        j1b = -0.5 * em * fb[0];
+        j2b = -0.5 * em * (nx2 * fb[1] + LEQ * nxy * fb[2]);
+        j3b = -0.5 * em * (LEQ * nxy * fb[1] + ny2 * fb[2]);
+    }
     double unb[2] = {0.0, 0.0}, cb[2] = {0.0, 0.0};
     {
         const int s = arg / 3, q = arg % 3;
@@ -75,7 +85,7 @@ __device__ __forceinline__ void llf_flux_adj(const Phys& P, double zi, double ze
 #pragma unroll
     for (int s = 0; s < 2; s++) {
         const double z = side_z[s], x = side_x[s], y = side_y[s], h = side_h[s], sf = side_s[s];
-        const double fnl = NLEQ + LEQ * h;
+        const double fnl = NL ? 1.0 : NLEQ + LEQ * h;
         const double fu = NLEQ * x, fv = NLEQ * y, fg = NLEQG * z;
         double zbar = 0.0, xbar = 0.0, ybar = 0.0, hbar = 0.0;
         // fy3 = hvv + gh2; fx3 = huv*sf; fy2 = huv; fx2 = (huu + gh2)*sf
@@ -85,7 +95,7 @@ __device__ __forceinline__ void llf_flux_adj(const Phys& P, double zi, double ze
         const double huub = sf * fx2b;
         // gh2 = fg*(0.5*z + h) + FG_L*z
         double fgb = (0.5 * z + h) * gh2b;
-        zbar += (0.5 * fg + FG_L) * gh2b;
+        zbar += (NL ? 0.5 * fg : 0.5 * fg + FG_L) * gh2b;
         hbar += fg * gh2b;
         // huu = fu*u, hvv = fv*v, huv = fu*v
         double fub = u[s] * huub + v[s] * huvb;
@@ -110,8 +120,7 @@ __device__ __forceinline__ void llf_flux_adj(const Phys& P, double zi, double ze
         // fx1 = x*fnl*sf, fy1 = y*fnl
         xbar += fnl * sf * fx1b;
         ybar += fnl * fy1b;
-        const double fnlb = x * sf * fx1b + y * fy1b;
-        hbar += LEQ * fnlb;
+        if constexpr (!NL) hbar += LEQ * (x * sf * fx1b + y * fy1b);
         // ht = z*NLEQ + h
         zbar += NLEQ * htb;
         hbar += htb;
@@ -161,7 +170,7 @@ __device__ __forceinline__ void load_lambda(const AdjArgs& a, int ne, int el, do
     }
 }
 
-template <int P, int NT, bool SPH>
+template <int P, int NT, bool SPH, bool NL>
 __global__ void __launch_bounds__(NT) adj_stage_kernel(AdjArgs a) {
     constexpr int DOF = (P + 1) * (P + 2) / 2;
     constexpr int NAGP = (P == 1) ? 3 : 6;
@@ -255,8 +264,8 @@ __global__ void __launch_bounds__(NT) adj_stage_kernel(AdjArgs a) {
                     fb[f] = role ? (so_ - sn_) : (sn_ - so_);
                 }
                 double bar[8];
-                if (role) llf_flux_adj(ph, zn, zo, xn, xo, yn, yo, hn, ho, sn, so, nx, ny, ph.g, fb, bar);
-                else llf_flux_adj(ph, zo, zn, xo, xn, yo, yn, ho, hn, so, sn, nx, ny, ph.g, fb, bar);
+                if (role) llf_flux_adj<NL>(ph, zn, zo, xn, xo, yn, yo, hn, ho, sn, so, nx, ny, ph.g, fb, bar);
+                else llf_flux_adj<NL>(ph, zo, zn, xo, xn, yo, yn, ho, hn, so, sn, nx, ny, ph.g, fb, bar);
                 const int s = role ? 1 : 0;   // this element's side of the flux arguments
                 const double zb = bar[0 + s], xb = bar[2 + s], yb = bar[4 + s], hbar = bar[6 + s];
 #pragma unroll
@@ -292,7 +301,7 @@ __global__ void __launch_bounds__(NT) adj_stage_kernel(AdjArgs a) {
                     const double den = 1.0 / (nx * ty - ny * tx);
                     const double qx_ex = (ty * qn_ext - ny * qt_ext) * den;
                     const double qy_ex = (-tx * qn_ext + nx * qt_ext) * den;
-                    llf_flux_adj(ph, ze_in, ze_in, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_in, sf, sf, nx, ny, ph.g, fb, bar);
+                    llf_flux_adj<NL>(ph, ze_in, ze_in, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_in, sf, sf, nx, ny, ph.g, fb, bar);
                     zb = bar[0] + bar[1];
                     hbar = bar[6] + bar[7];
                     // mirror: (qx_ex, qy_ex) from (qx_in, qy_in)
@@ -315,7 +324,7 @@ __global__ void __launch_bounds__(NT) adj_stage_kernel(AdjArgs a) {
                     }
                     const bool eirik = (ze_ex * ph.ifnlfa + hb_in) <= 0;
                     if (eirik) ze_ex = fabs(hb_in) + sc.h0;
-                    llf_flux_adj(ph, ze_in, ze_ex, qx_in, qx_in, qy_in, qy_in, hb_in, hb_in, sf, sf, nx, ny, ph.g, fb, bar);
+                    llf_flux_adj<NL>(ph, ze_in, ze_ex, qx_in, qx_in, qy_in, qy_in, hb_in, hb_in, sf, sf, nx, ny, ph.g, fb, bar);
                     zb = bar[0];
                     xb = bar[2] + bar[3];
                     yb = bar[4] + bar[5];
@@ -337,7 +346,7 @@ __global__ void __launch_bounds__(NT) adj_stage_kernel(AdjArgs a) {
                         qx_ex = -(ty * qn_ext) / (nx * ty - ny * tx);
                         qy_ex = -(-tx * qn_ext) / (nx * ty - ny * tx);
                     }
-                    llf_flux_adj(ph, ze_in, ze_in, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_in, sf, sf, nx, ny, ph.g, fb, bar);
+                    llf_flux_adj<NL>(ph, ze_in, ze_in, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_in, sf, sf, nx, ny, ph.g, fb, bar);
                     zb = bar[0] + bar[1];
                     xb = bar[2];
                     yb = bar[4];
@@ -370,7 +379,7 @@ __global__ void __launch_bounds__(NT) adj_stage_kernel(AdjArgs a) {
             dsdy = 1.0 / AREA * (X1 - X0);
         }
         const double depth_c = hb[0] + U[0][0];
-        const double fh_nl_c = 1.0 / (ph.nleq * depth_c + ph.leq);
+        const double fh_nl_c = NL ? 1.0 / depth_c : 1.0 / (ph.nleq * depth_c + ph.leq);
         const double cori = E.cori_el ? E.cori_el[e] : sc.cori;
         const double nbar = ph.load_manning ? E.nbar[e] : 0.0;
         double depth_cb = 0.0;
@@ -395,10 +404,10 @@ __global__ void __launch_bounds__(NT) adj_stage_kernel(AdjArgs a) {
             }
             const double sfq = SPH ? E.sfacq[(size_t)I * ne + e] : 1.0;
             const double depth = ze + hbq;
-            const double f1 = ph.nleq + ph.leq * hbq;
-            const double fu = ph.nleq * qx, fv = ph.nleq * qy;
+            const double f1 = NL ? 1.0 : ph.nleq + ph.leq * hbq;
+            const double fu = NL ? qx : ph.nleq * qx, fv = NL ? qy : ph.nleq * qy;
             const double fg = ph.nleqg * ze;
-            const double fh = 1.0 / (ph.nleq * depth + ph.leq);
+            const double fh = NL ? 1.0 / depth : 1.0 / (ph.nleq * depth + ph.leq);
             const double u = qx * fh, v = qy * fh;
             double fric = sc.fric;
             bool fric_active = false;

@@ -880,11 +880,17 @@ int adj_check(Ctx& c) {
     return 0;
 }
 
+template <int P, int NT, bool NL>
+void launch_adj_nl(Ctx& c, const AdjArgs& a) {
+    const int grid = (c.ne + NT - 1) / NT;
+    if (c.prm.ics == 2) hipLaunchKernelGGL((adj_stage_kernel<P, NT, true, NL>), dim3(grid), dim3(NT), 0, c.stream, a);
+    else hipLaunchKernelGGL((adj_stage_kernel<P, NT, false, NL>), dim3(grid), dim3(NT), 0, c.stream, a);
+}
+
 template <int P, int NT>
 void launch_adj(Ctx& c, const AdjArgs& a) {
-    const int grid = (c.ne + NT - 1) / NT;
-    if (c.prm.ics == 2) hipLaunchKernelGGL((adj_stage_kernel<P, NT, true>), dim3(grid), dim3(NT), 0, c.stream, a);
-    else hipLaunchKernelGGL((adj_stage_kernel<P, NT, false>), dim3(grid), dim3(NT), 0, c.stream, a);
+    if (c.prm.nolicat != 0) launch_adj_nl<P, NT, true>(c, a);
+    else launch_adj_nl<P, NT, false>(c, a);
 }
 
 // Forward N steps from the current state recording the gauges after every step (geta[k-1] after
@@ -965,7 +971,7 @@ int adj_backward(Ctx& c, int it0, int nsteps, int interval, double weight) {
                 a.out = ub[i];
                 a.gdp = c.gdp.p;
                 a.gnbar = c.phys.load_manning ? c.gnb.p : nullptr;
-                if (P == 1) launch_adj<P, 128>(c, a);
+                if constexpr (P == 1) launch_adj<P, 128>(c, a);
                 else launch_adj<P, 64>(c, a);
                 HIPCHK(hipGetLastError());
             }
