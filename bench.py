@@ -123,10 +123,11 @@ def adjoint_leg(gpu, mesh, n_elem: int, steps: int, seed: int):
     basin without limiter: wall(forward + checkpointed adjoint) / wall(forward)."""
     import numpy as np
     from dgswem_amd import adjoint
-    case = mesh.delaunay_basin(n_elem=n_elem, seed=seed, depth_range=(10.0, 3000.0), shelf=False,
-                               wetdry=False, slopeflag=0, nsteps=steps)
+    case = mesh.delaunay_basin(n_elem=n_elem, seed=seed, depth_range=(5.0, 300.0), shelf=False,
+                               wetdry=False, slopeflag=0, nsteps=steps, cf=1e-4)
     prob = adjoint.twin_problem(case, nsteps=steps, ng=50, seed=seed)
     s = prob.solver
+    s.autotune()
     prob.loss(case.manning, case.dp)            # warm-up
     t0 = time.perf_counter()
     J = prob.loss(case.manning, case.dp)
@@ -140,7 +141,7 @@ def adjoint_leg(gpu, mesh, n_elem: int, steps: int, seed: int):
     return {"cost_ratio": grad / fwd, "forward_s": fwd, "gradient_s": grad, "elements": case.ne,
             "nodes": case.np_, "steps": steps, "checkpoint_interval": interval, "loss": J2,
             "params": 2 * case.np_,
-            "workload": "all-wet seeded Delaunay basin (GRF depth 10-3000 m), p=1, no limiter, "
+            "workload": "all-wet seeded Delaunay basin (GRF depth 5-300 m, CF 1e-4), p=1, no limiter, "
                         "50 gauges, misfit against a perturbed-Manning twin; gradient w.r.t. "
                         "per-node Manning n and depth"}
 

@@ -26,6 +26,8 @@ __device__ __forceinline__ void llf_flux_adj(const Phys& P, double zi, double ze
                                              double yi, double ye, double hi, double he, double sfi,
                                              double sfe, double nx, double ny, double G,
                                              const double fb[3], double bar[8]) {
+    // derivatives need not round like the forward step: let products and sums fuse
+#pragma clang fp contract(fast)
     const double NLEQ = NL ? 1.0 : P.nleq, LEQ = NL ? 0.0 : P.leq, FG_L = NL ? 0.0 : P.fg_l;
     const double NLEQG = G * NLEQ;
     // ---- forward values needed by the reverse sweep
@@ -171,11 +173,13 @@ __device__ __forceinline__ void load_lambda(const AdjArgs& a, int ne, int el, do
 }
 
 template <int P, int NT, bool SPH, bool NL>
-__global__ void __launch_bounds__(NT) adj_stage_kernel(AdjArgs a) {
+__global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a) {
+#pragma clang fp contract(fast)
     constexpr int DOF = (P + 1) * (P + 2) / 2;
     constexpr int NAGP = (P == 1) ? 3 : 6;
     constexpr int NEGP = P + 2;
     __shared__ double ub_lds[3 * DOF][NT];     // cotangent of this element's modes
+    __shared__ double lm_lds[3 * DOF][NT];     // this element's RHS cotangent (kept out of registers)
     const DevTables& T = tab<P>();
     const ElemData& E = a.ed;
     const Phys& ph = a.ph;
@@ -185,15 +189,23 @@ __global__ void __launch_bounds__(NT) adj_stage_kernel(AdjArgs a) {
     if (e >= ne) return;
     const int sid = E.sid ? (int)E.sid[e] : 0;
     const Scen sc = E.scen[sid];
-    const int row = *a.fo.step * a.fo.nrk + a.irk;
+    const int row = table_step(a.fo) * a.fo.nrk + a.irk;
     const double c13 = 1.0 / 3.0, c16 = 1.0 / 6.0;
 
-    double U[3][DOF], Lm[3][DOF];
+    double U[3][DOF];
 #pragma unroll
     for (int f = 0; f < 3; f++)
 #pragma unroll
         for (int k = 0; k < DOF; k++) U[f][k] = a.u[(size_t)(f * DOF + k) * ne + e];
-    load_lambda<DOF>(a, ne, e, sc.dtdp, Lm);
+    {
+        double L0[3][DOF];
+        load_lambda<DOF>(a, ne, e, sc.dtdp, L0);
+#pragma unroll
+        for (int f = 0; f < 3; f++)
+#pragma unroll
+            for (int k = 0; k < DOF; k++) lm_lds[f * DOF + k][tid] = L0[f][k];
+    }
+#define Lm(f, k) lm_lds[(f) * DOF + (k)][tid]
 #pragma unroll
     for (int j = 0; j < 3 * DOF; j++) ub_lds[j][tid] = 0.0;
     double hbb[3] = {0.0, 0.0, 0.0};
@@ -258,7 +270,7 @@ __global__ void __launch_bounds__(NT) adj_stage_kernel(AdjArgs a) {
                     double so_ = 0.0, sn_ = 0.0;
 #pragma unroll
                     for (int k = 0; k < DOF; k++) {
-                        so_ = so_ + edfac * eqo[k] * Lm[f][k];
+                        so_ = so_ + edfac * eqo[k] * Lm(f, k);
                         sn_ = sn_ + edfac_n * eqn[k] * Ln[f][k];
                     }
                     fb[f] = role ? (so_ - sn_) : (sn_ - so_);
@@ -289,9 +301,9 @@ __global__ void __launch_bounds__(NT) adj_stage_kernel(AdjArgs a) {
 #pragma unroll
                 for (int k = 0; k < DOF; k++) {
                     const double w = 2.0 * T.m_inv[k] / AREAS * xlen * pv[k] * T.wegp[I];
-                    fb[0] = fb[0] - w * Lm[0][k];
-                    fb[1] = fb[1] - w * Lm[1][k];
-                    fb[2] = fb[2] - w * Lm[2][k];
+                    fb[0] = fb[0] - w * Lm(0, k);
+                    fb[1] = fb[1] - w * Lm(1, k);
+                    fb[2] = fb[2] - w * Lm(2, k);
                 }
                 double bar[8];
                 double zb = 0.0, xb = 0.0, yb = 0.0, hbar = 0.0;
@@ -433,15 +445,15 @@ __global__ void __launch_bounds__(NT) adj_stage_kernel(AdjArgs a) {
             }
             // cotangents of the quadrature-point fluxes and sources from RHS(K)
             double fxb = 0.0, fyb = 0.0, gxb = 0.0, gyb = 0.0, hxb = 0.0, hyb_ = 0.0;
-            double sxb = T.srfac[I][0] * Lm[1][0], syb = T.srfac[I][0] * Lm[2][0];
+            double sxb = T.srfac[I][0] * Lm(1, 0), syb = T.srfac[I][0] * Lm(2, 0);
 #pragma unroll
             for (int k = 1; k < DOF; k++) {
                 const double dpxk = T.drphi[I][k] * drdx + T.dsphi[I][k] * dsdx;
                 const double dpyk = T.drphi[I][k] * drdy + T.dsphi[I][k] * dsdy;
                 const double xf = T.mw[I][k] * dpxk, yf = T.mw[I][k] * dpyk, sr = T.srfac[I][k];
-                fxb += xf * Lm[0][k]; fyb += yf * Lm[0][k];
-                gxb += xf * Lm[1][k]; gyb += yf * Lm[1][k]; sxb += sr * Lm[1][k];
-                hxb += xf * Lm[2][k]; hyb_ += yf * Lm[2][k]; syb += sr * Lm[2][k];
+                fxb += xf * Lm(0, k); fyb += yf * Lm(0, k);
+                gxb += xf * Lm(1, k); gyb += yf * Lm(1, k); sxb += sr * Lm(1, k);
+                hxb += xf * Lm(2, k); hyb_ += yf * Lm(2, k); syb += sr * Lm(2, k);
             }
             double zeb = 0.0, qxb = 0.0, qyb = 0.0, hbqb = 0.0, dhbxb = 0.0, dhbyb = 0.0, depthb = 0.0;
             // sx = -(tau*qx) + fg*dhbx*sfq + cori*qy ; sy = -(tau*qy) + fg*dhby - cori*qx
@@ -543,6 +555,7 @@ __global__ void __launch_bounds__(NT) adj_stage_kernel(AdjArgs a) {
         a.gdp[(size_t)ne + e] += D1b;
         a.gdp[2 * (size_t)ne + e] += D2b;
     }
+#undef Lm
 }
 
 // Elevation at the gauges: eta_g = sum_K ZE(K, elem_g) * phi_K(gauge), written to out[g].
@@ -559,20 +572,32 @@ __global__ void gauge_kernel(int ng, int ne, const int* __restrict__ gel, const 
     out[g] = s;
 }
 
-// Cotangent seed of the gauge misfit sum_g w (eta_g - d_g)^2 into lam (one thread, gauges in
-// order, so gauges sharing an element add up deterministically).
+// Cotangent seed of the gauge misfit sum_g w (eta_g - d_g)^2 into lam.  One thread per gauge
+// element; gord lists the gauges grouped by element (ascending gauge index within a group, group u
+// at gord[goff[u]..goff[u+1])), so gauges sharing an element add up in gauge order.
 template <int P>
-__global__ void gauge_seed_kernel(int ng, int ne, const int* __restrict__ gel, const double* __restrict__ gphi,
-                                  const double* __restrict__ eta, const double* __restrict__ data, double w,
-                                  double* __restrict__ lam) {
+__global__ void __launch_bounds__(64) gauge_seed_kernel(int nu, int ne, const int* __restrict__ goff,
+                                                        const int* __restrict__ gord, const int* __restrict__ gel,
+                                                        const double* __restrict__ gphi,
+                                                        const double* __restrict__ eta,
+                                                        const double* __restrict__ data, double w,
+                                                        double* __restrict__ lam) {
     constexpr int DOF = (P + 1) * (P + 2) / 2;
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    for (int g = 0; g < ng; g++) {
-        const double r = 2.0 * w * (eta[g] - data[g]);
-        const int e = gel[g];
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nu) return;
+    const int j0 = goff[u], j1 = goff[u + 1];
+    const int e = gel[gord[j0]];
+    double t[DOF];
 #pragma unroll
-        for (int k = 0; k < DOF; k++) lam[(size_t)k * ne + e] += r * gphi[(size_t)g * DOF + k];
+    for (int k = 0; k < DOF; k++) t[k] = lam[(size_t)k * ne + e];
+    for (int j = j0; j < j1; j++) {
+        const int g = gord[j];
+        const double r = 2.0 * w * (eta[g] - data[g]);
+#pragma unroll
+        for (int k = 0; k < DOF; k++) t[k] += r * gphi[(size_t)g * DOF + k];
     }
+#pragma unroll
+    for (int k = 0; k < DOF; k++) lam[(size_t)k * ne + e] = t[k];
 }
 
 // Node gradients from the element accumulators: dJ/dDP(n) = sum over incident elements of the

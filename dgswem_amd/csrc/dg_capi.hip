@@ -115,6 +115,7 @@ struct Ctx {
     // forcing
     DevBuf<double> emo, efa, qnam, qnph, argj, rff, fargj, frff;
     DevBuf<int> step;                            // device step counter within the current tables
+    int hstep = 0;                               // the same index for eager (uncaptured) launches
     int rows = 0;                                // table rows per scenario
     // NWS=8 meteorological forcing
     bool met = false;
@@ -124,8 +125,8 @@ struct Ctx {
     // launch configuration
     int block = 256;
     // adjoint: gauges, checkpoints, segment stage states, cotangents, parameter accumulators
-    int ng = 0;
-    DevBuf<int> gel;
+    int ng = 0, ngu = 0;                         // gauges, distinct gauge elements
+    DevBuf<int> gel, goff, gord;
     DevBuf<double> gphi, geta, gdata, ckpt, seg, lam, gdp, gnb, pout;
     DevBuf<double> ubar[KMAXRK];
     // graphs of one slot-rotation period, keyed by (ucur, wdcur) at capture
@@ -220,10 +221,13 @@ ElemData elem_data(Ctx& c) {
     return E;
 }
 
-Forcing forcing(Ctx& c) {
+// Forcing-table arguments.  Captured launches read the step index from the device counter (graph
+// replays advance it); eager launches take it as an argument.
+Forcing forcing(Ctx& c, bool captured = false) {
     Forcing F{};
     F.nbfr = c.nbfr; F.nffr = c.nffr; F.rows = c.rows; F.nrk = c.rk.nrk;
-    F.step = c.step.p;
+    F.step = captured ? c.step.p : nullptr;
+    F.step_imm = c.hstep;
     F.emo = c.emo.p; F.efa = c.efa.p; F.qnam = c.qnam.p; F.qnph = c.qnph.p;
     F.argj = c.argj.p; F.rff = c.rff.p; F.fargj = c.fargj.p; F.frff = c.frff.p;
     F.ramp = c.ramp.p;
@@ -582,7 +586,7 @@ int init_ctx_common(Ctx& c, const dg_params* prm, bool manning) {
     if (!build_ref_tables(prm->p, c.T, err)) return fail(err);
     if (!build_rk(prm->rk_stage, prm->rk_order, c.rk, err)) return fail(err);
     c.dof = c.T.dof; c.nagp = c.T.nagp; c.negp = c.T.negp;
-    c.block = prm->p == 1 ? 256 : 128;
+    c.block = 128;
     HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     {
         std::lock_guard<std::mutex> lk(g_mu);
@@ -726,9 +730,9 @@ void launch_stage_nt(Ctx& c, const StageArgs& a, bool sph, bool fuse, int nt) {
     }
 }
 
-StageArgs stage_args(Ctx& c, int irk, const int slot[]) {
+StageArgs stage_args(Ctx& c, int irk, const int slot[], bool captured = false) {
     StageArgs a{};
-    a.ed = elem_data(c); a.ph = c.phys; a.fo = forcing(c);
+    a.ed = elem_data(c); a.ph = c.phys; a.fo = forcing(c, captured);
     if (c.met) a.mt = met_args(c);
     a.uin = c.u[slot[irk]].p;
     a.uout = c.u[slot[irk + 1]].p;
@@ -760,12 +764,13 @@ int launch_step(Ctx& c, bool capturing) {
     for (int i = 0; i <= nrk; i++) slot[i] = (c.ucur + i) % (nrk + 1);
     if (c.met) {
         int ev = prof_begin(c, K_MET, capturing);
-        hipLaunchKernelGGL(met_kernel, dim3(gn), dim3(B), 0, c.stream, met_args(c), (const int*)c.step.p);
+        hipLaunchKernelGGL(met_kernel, dim3(gn), dim3(B), 0, c.stream, met_args(c),
+                           capturing ? (const int*)c.step.p : nullptr, c.hstep);
         prof_end(c, ev);
         HIPCHK(hipGetLastError());
     }
     for (int irk = 0; irk < nrk; irk++) {
-        StageArgs a = stage_args(c, irk, slot);
+        StageArgs a = stage_args(c, irk, slot, capturing);
         for (int i = 0; i < irk; i++)
             if (a.bprev[i] != 0.0 && !c.keep_r[i]) return fail("internal: RHS of an earlier stage was not kept");
         const bool fuse = wdon && !lim;
@@ -793,8 +798,12 @@ int launch_step(Ctx& c, bool capturing) {
             HIPCHK(hipGetLastError());
         }
     }
-    hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(1), 0, c.stream, c.step.p, 0, 0);
-    HIPCHK(hipGetLastError());
+    if (capturing) {
+        hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(1), 0, c.stream, c.step.p, 0, 0);
+        HIPCHK(hipGetLastError());
+    } else {
+        c.hstep++;
+    }
     c.ucur = slot[nrk];
     return 0;
 }
@@ -832,8 +841,7 @@ int launch_step_explicit(Ctx& c, double* const* st) {
         launch_stage_nt<P>(c, a, sph, false, c.block);
         HIPCHK(hipGetLastError());
     }
-    hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(1), 0, c.stream, c.step.p, 0, 0);
-    HIPCHK(hipGetLastError());
+    c.hstep++;
     return 0;
 }
 
@@ -841,11 +849,12 @@ int launch_step_explicit(Ctx& c, double* const* st) {
 // steps, after which the stage slots and wet/dry buffers are back in place), then eager launches
 // for the remainder.
 int run_chunk(Ctx& c, int n) {
-    hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(1), 0, c.stream, c.step.p, 1, 0);
-    HIPCHK(hipGetLastError());
+    c.hstep = 0;
     const int period = c.rk.nrk + 1;
     int done = 0;
     if (c.use_graph && n >= period) {
+        hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(1), 0, c.stream, c.step.p, 1, 0);
+        HIPCHK(hipGetLastError());
         const int key = c.ucur * 2 + c.wdcur;
         auto it = c.graphs.find(key);
         if (it == c.graphs.end()) {
@@ -866,6 +875,7 @@ int run_chunk(Ctx& c, int n) {
         const int reps = n / period;
         for (int r = 0; r < reps; r++) HIPCHK(hipGraphLaunch(it->second, c.stream));
         done = reps * period;
+        c.hstep = done;
     }
     return launch_steps(c, n - done, false);
 }
@@ -902,7 +912,7 @@ int adj_forward(Ctx& c, int it0, int nsteps, int interval, bool save) {
     for (int s0 = 0; s0 < nsteps; s0 += kTableChunk) {
         const int n = std::min(kTableChunk, nsteps - s0);
         if (build_stage_tables(c, it0 + s0, n)) return 1;
-        hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(1), 0, c.stream, c.step.p, 1, 0);
+        c.hstep = 0;
         for (int s = 0; s < n; s++) {
             const int k = s0 + s;   // steps done before this one
             if (save && k % interval == 0)
@@ -926,9 +936,14 @@ int adj_backward(Ctx& c, int it0, int nsteps, int interval, double weight) {
     HIPCHK(hipMemsetAsync(c.gdp.p, 0, (size_t)3 * ne * sizeof(double), c.stream));
     HIPCHK(hipMemsetAsync(c.gnb.p, 0, (size_t)ne * sizeof(double), c.stream));
     // seed: misfit at step N
-    hipLaunchKernelGGL((gauge_seed_kernel<P>), dim3(1), dim3(64), 0, c.stream, c.ng, ne, (const int*)c.gel.p,
-                       (const double*)c.gphi.p, (const double*)(c.geta.p + (size_t)(nsteps - 1) * c.ng),
-                       (const double*)(c.gdata.p + (size_t)(nsteps - 1) * c.ng), weight, c.lam.p);
+    const int gs = (c.ngu + 63) / 64;
+    auto seed = [&](int k, double* l) {   // misfit of the state after step k (1-based)
+        hipLaunchKernelGGL((gauge_seed_kernel<P>), dim3(gs), dim3(64), 0, c.stream, c.ngu, ne, (const int*)c.goff.p,
+                           (const int*)c.gord.p, (const int*)c.gel.p, (const double*)c.gphi.p,
+                           (const double*)(c.geta.p + (size_t)(k - 1) * c.ng),
+                           (const double*)(c.gdata.p + (size_t)(k - 1) * c.ng), weight, l);
+    };
+    seed(nsteps, c.lam.p);
     double* lam = c.lam.p;
     double* ub[KMAXRK + 1];
     for (int i = 0; i < nrk; i++) ub[i] = c.ubar[i].p;
@@ -940,7 +955,7 @@ int adj_backward(Ctx& c, int it0, int nsteps, int interval, double weight) {
         if (build_stage_tables(c, it0 + k0, n)) return 1;
         HIPCHK(hipMemcpyAsync(stage(k0 + 1, 0), c.ckpt.p + (size_t)sg * ssz, ssz * sizeof(double),
                               hipMemcpyDeviceToDevice, c.stream));
-        hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(1), 0, c.stream, c.step.p, 1, 0);
+        c.hstep = 0;
         for (int k = k0 + 1; k < k1; k++) {
             double* st[KMAXRK + 1];
             for (int i = 0; i < nrk; i++) st[i] = stage(k, i);
@@ -954,7 +969,7 @@ int adj_backward(Ctx& c, int it0, int nsteps, int interval, double weight) {
             if (launch_step_explicit<P>(c, st)) return 1;
         }
         for (int k = k1; k > k0; k--) {
-            hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(1), 0, c.stream, c.step.p, 1, k - k0 - 1);
+            c.hstep = k - k0 - 1;
             ub[nrk] = lam;
             for (int i = nrk - 1; i >= 0; i--) {
                 AdjArgs a{};
@@ -976,11 +991,7 @@ int adj_backward(Ctx& c, int it0, int nsteps, int interval, double weight) {
                 HIPCHK(hipGetLastError());
             }
             std::swap(lam, ub[0]);   // cotangent of U^(k-1)
-            if (k - 1 >= 1)
-                hipLaunchKernelGGL((gauge_seed_kernel<P>), dim3(1), dim3(64), 0, c.stream, c.ng, ne,
-                                   (const int*)c.gel.p, (const double*)c.gphi.p,
-                                   (const double*)(c.geta.p + (size_t)(k - 2) * c.ng),
-                                   (const double*)(c.gdata.p + (size_t)(k - 2) * c.ng), weight, lam);
+            if (k - 1 >= 1) seed(k - 1, lam);
         }
     }
     // keep the buffers' identities for the next call
@@ -1307,9 +1318,19 @@ int dg_adj_set_gauges(int h, int ng, const int* elem, const double* r, const dou
         ge[g] = elem[g] - 1;
         basis_at(c->prm.p, r[g], s[g], &ph[(size_t)g * dof]);
     }
+    // gauges grouped by element, ascending gauge index within a group
+    std::vector<int> ord(ng), off;
+    for (int g = 0; g < ng; g++) ord[g] = g;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return ge[a] < ge[b]; });
+    for (int j = 0; j < ng; j++)
+        if (j == 0 || ge[ord[j]] != ge[ord[j - 1]]) off.push_back(j);
+    off.push_back(ng);
     c->ng = ng;
+    c->ngu = (int)off.size() - 1;
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(c->gel.upload(ge.data(), ng, c->stream));
+    HIPCHK(c->gord.upload(ord.data(), ng, c->stream));
+    HIPCHK(c->goff.upload(off.data(), off.size(), c->stream));
     HIPCHK(c->gphi.upload(ph.data(), ph.size(), c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
@@ -1455,16 +1476,15 @@ int dg_gpu_autotune(int h, int* block) {
     const int nrk = c->rk.nrk;
     int slot[KMAXRK + 1];
     for (int i = 0; i <= nrk; i++) slot[i] = (c->ucur + i) % (nrk + 1);
+    c->hstep = 0;
     StageArgs a = stage_args(*c, 0, slot);
     a.rout = nullptr;
     const bool sph = c->prm.ics == 2, fuse = c->prm.nolifa >= 2 && c->prm.slopeflag == 0;
     const std::vector<int> cand = c->prm.p == 1 ? std::vector<int>{64, 128, 256} : std::vector<int>{64, 128};
     if (build_stage_tables(*c, c->next_it, 1)) return 1;
-    hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->step.p, 1, 0);
-    HIPCHK(hipGetLastError());
     if (c->met) {
         hipLaunchKernelGGL(met_kernel, dim3((c->np + 255) / 256), dim3(256), 0, c->stream, met_args(*c),
-                           (const int*)c->step.p);
+                           nullptr, 0);
         HIPCHK(hipGetLastError());
     }
     hipEvent_t e0, e1;

@@ -80,7 +80,9 @@ struct ElemData {
 // that a captured graph of several steps can be replayed.
 struct Forcing {
     int nbfr, nffr, rows, nrk;
-    const int* __restrict__ step;       // device step counter within the current tables
+    const int* __restrict__ step;       // device step counter within the current tables (graph
+                                        // replays), or nullptr: the step is step_imm
+    int step_imm;
     const double* __restrict__ emo;     // [needs][nbfr][2]
     const double* __restrict__ efa;     // [needs][nbfr][2]
     const double* __restrict__ qnam;    // [nfeds][nffr][2]
@@ -91,6 +93,10 @@ struct Forcing {
     const double* __restrict__ frff;    // [nstage][nffr]
     const double* __restrict__ ramp;    // [scenario][row] RAMPDG of each stage (meteorological forcing)
 };
+
+// Step index within the current forcing tables.
+__device__ __forceinline__ int table_step(const Forcing& f) { return f.step ? *f.step : f.step_imm; }
+
 
 // NWS=8 meteorological forcing at the nodes.  The node kernel evaluates the Holland profile once
 // per step at TIME_A (the reference evaluates it at every stage with the same TIME_A) and keeps the

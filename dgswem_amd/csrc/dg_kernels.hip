@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
     if (e >= ne) return;
     const int sid = E.sid ? (int)E.sid[e] : 0;
     const Scen sc = E.scen[sid];
-    const int row = *a.fo.step * a.fo.nrk + a.irk;
+    const int row = table_step(a.fo) * a.fo.nrk + a.irk;
 
     double U[3][DOF];
 #pragma unroll
@@ -701,18 +701,16 @@ __global__ void __launch_bounds__(256) modal2nodal_kernel(int np, ElemData E, co
     nodecode[n] = nc;
 }
 
-// Device step counter used by the forcing-table lookups (a kernel, not a memset, so that the
-// sequence can be captured into a HIP graph).
 // ------------------------------------------------------------------------------------------
 // NWS=8 node kernel: HollandGet's per-node profile (src/wind.F:2062-2131) and the wind speed and
 // Garratt drag of MET_FORCING (src/met_forcing.F:488-500, src/wind.F:135-141), at TIME_A of the
 // current step.  The stage's ramp is applied in the stage kernel.
 // ------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) met_kernel(Met m, const int* __restrict__ step) {
+__global__ void __launch_bounds__(256) met_kernel(Met m, const int* __restrict__ step, int step_imm) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= m.np) return;
     const int sid = m.nsid ? (int)m.nsid[n] : 0;
-    const double* r = m.row + ((size_t)sid * m.rows + *step) * MET_ROW;
+    const double* r = m.row + ((size_t)sid * m.rows + (step ? *step : step_imm)) * MET_ROW;
     const double re = 6378206.4;
     const double deg2rad = 3.141592653589793 / 180.0;
     const double lat = r[MR_LAT], lon = r[MR_LON], spd = r[MR_SPD], B = r[MR_B];
@@ -736,6 +734,8 @@ __global__ void __launch_bounds__(256) met_kernel(Met m, const int* __restrict__
     m.wx[n] = wx; m.wy[n] = wy; m.wm[n] = wmag; m.cd[n] = cd; m.prn[n] = press;
 }
 
+// Device step counter used by the forcing-table lookups of captured steps (a kernel, not a memset,
+// so that the sequence can be captured into a HIP graph).
 __global__ void step_counter_kernel(int* __restrict__ step, int set, int value) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *step = set ? value : *step + 1;
 }

@@ -148,14 +148,16 @@ def delaunay_basin(n_elem: int = 1_000_000, seed: int = 0, lx: float = 1.0e6, ly
                    land_height: float = 2.0, n_const: int = 5, p: int = 1, cfl: float = 0.3,
                    slopeflag: int = 5, wetdry: bool = True, nsteps: int = 2000,
                    manning_range=(0.02, 0.03), refine: float = 3.0, ics: int = 1,
-                   lon_c: float = -75.03, lat_c: float = 27.97, storm=None) -> Case:
+                   lon_c: float = -75.03, lat_c: float = 27.97, storm=None,
+                   cf: float = 0.0025) -> Case:
     """Seeded Delaunay basin of about `n_elem` triangles (see module docstring).
 
     The open boundary is x=0 (west); the coast x=lx (east) is land with a shelf that rises from the
     random-field depth to `land_height` metres above datum (DP=-land_height) over the last
     `shelf_frac` of the domain; north and south sides are land.  Point density grows by a factor
     `1+refine` toward the coast.  The time step is `cfl` times the smallest element inradius over
-    the gravity wave speed of its deepest node.
+    the gravity wave speed of its deepest node.  `cf` is fort.15's CF, the floor of the Manning
+    friction coefficient g n^2 / H^(1/3).
 
     With ics=2 the basin is placed on the sphere, centred at (lon_c, lat_c) degrees, which is also
     the CPP projection centre; nodes are stored as fort.14 lon/lat and projected exactly as the
@@ -261,7 +263,7 @@ def delaunay_basin(n_elem: int = 1_000_000, seed: int = 0, lx: float = 1.0e6, ly
     dt = float(cfl * np.min(rin / np.sqrt(9.81 * hmax)))
     prm = _params(ics=ics, nolibf=1, nolifa=2 if wetdry else 1, nolica=1, nolicat=1, nwp=1,
                   ncor=1 if ics == 2 else 0, ntip=0, nws=8 if storm is not None else 0, nramp=1, g=9.81, tau0=0.005, dtdp=dt, statim=0.0, reftim=0.0,
-                  rnday=nsteps * dt / 86400.0, dramp=0.5, h0=0.1, cf=0.0025, cori=0.0,
+                  rnday=nsteps * dt / 86400.0, dramp=0.5, h0=0.1, cf=cf, cori=0.0,
                   nt=nsteps, nbfr=n_const, amig=np.array([c[1] for c in const]),
                   ff=np.ones(n_const), face=np.zeros(n_const),
                   per=np.array([2.0 * PI / c[1] for c in const]), emo=emo, efa=efa,
