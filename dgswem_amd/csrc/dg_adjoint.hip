@@ -107,8 +107,10 @@ __device__ __forceinline__ void llf_flux_adj(const Phys& P, double zi, double ze
         // un = (u*nx + v*ny)*NLEQ
         ub += nx * NLEQ * unb[s];
         vb += ny * NLEQ * unb[s];
-        // c = sqrt(G*ht*K)
-        double htb = c[s] > 0.0 ? G * K[s] / (2.0 * c[s]) * cb[s] : 0.0;
+        // c = sqrt(G*ht*K): dc/dht = G*K/(2c) = c/(2*ht), and fh = 1/ht for the nonlinear equations
+        double htb;
+        if constexpr (NL) htb = c[s] > 0.0 ? 0.5 * c[s] * fh[s] * cb[s] : 0.0;
+        else htb = c[s] > 0.0 ? G * K[s] / (2.0 * c[s]) * cb[s] : 0.0;
         // u = x*fh, v = y*fh
         xbar += fh[s] * ub;
         ybar += fh[s] * vb;
@@ -172,7 +174,8 @@ __device__ __forceinline__ void load_lambda(const AdjArgs& a, int ne, int el, do
     }
 }
 
-template <int P, int NT, bool SPH, bool NL>
+// QF: quadratic bottom friction without the hybrid factor (NOLIBF=1), fixed at compile time.
+template <int P, int NT, bool SPH, bool NL, bool QF>
 __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a) {
 #pragma clang fp contract(fast)
     constexpr int DOF = (P + 1) * (P + 2) / 2;
@@ -191,6 +194,8 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
     const Scen sc = E.scen[sid];
     const int row = table_step(a.fo) * a.fo.nrk + a.irk;
     const double c13 = 1.0 / 3.0, c16 = 1.0 / 6.0;
+    const bool quad = QF || ph.iflinbf == 0;
+    const bool hybrid = !QF && ph.ifhybf == 1;
 
     double U[3][DOF];
 #pragma unroll
@@ -298,9 +303,10 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
                 const double sf = SPH ? E.sfaced[((size_t)L * NEGP + I) * ne + e] : 1.0;
                 // flux cotangent: RHS(K) -= w_K*F with w_K = 2 M_INV(K)/AREAS*XLEN*PHI*WEGP
                 double fb[3] = {0.0, 0.0, 0.0};
+                const double wq = 2.0 * xlen / AREAS * T.wegp[I];
 #pragma unroll
                 for (int k = 0; k < DOF; k++) {
-                    const double w = 2.0 * T.m_inv[k] / AREAS * xlen * pv[k] * T.wegp[I];
+                    const double w = wq * T.m_inv[k] * pv[k];
                     fb[0] = fb[0] - w * Lm(0, k);
                     fb[1] = fb[1] - w * Lm(1, k);
                     fb[2] = fb[2] - w * Lm(2, k);
@@ -432,10 +438,10 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
             }
             double tau = fric, umag = 0.0, hyb = 1.0, hyb_in = 0.0;
             bool capped = false;
-            if (ph.iflinbf == 0) {
+            if (quad) {
                 umag = sqrt(u * u + v * v);
                 tau = tau * umag * fh;
-                if (ph.ifhybf == 1) {
+                if (hybrid) {
                     hyb_in = 1.0 + pow(ph.hbreak * fh_nl_c, ph.ftheta);
                     hyb = pow(hyb_in, ph.fgamma / ph.ftheta);
                     tau = tau * hyb;
@@ -465,13 +471,13 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
             dhbyb += fg * syb;
             // friction
             double fricb = 0.0, ub = 0.0, vb = 0.0, fhb = 0.0;
-            if (ph.iflinbf == 0) {
+            if (quad) {
                 if (!capped) {
                     // tau = fric*umag*fh*hyb
                     fricb += umag * fh * hyb * taub;
                     const double umagb = fric * fh * hyb * taub;
                     fhb += fric * umag * hyb * taub;
-                    if (ph.ifhybf == 1) {
+                    if (hybrid) {
                         const double hybb = fric * umag * fh * taub;
                         // hyb = (1 + (hbreak*fh_nl_c)^ftheta)^(fgamma/ftheta)
                         const double dh_in = (ph.fgamma / ph.ftheta) * pow(hyb_in, ph.fgamma / ph.ftheta - 1.0);
@@ -480,15 +486,19 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
                         const double fhcb = hybb * dh_in * dx_ * ph.hbreak;
                         depth_cb += -ph.nleq * fh_nl_c * fh_nl_c * fhcb;
                     }
-                    if (umag > 0.0) { ub += u / umag * umagb; vb += v / umag * umagb; }
+                    if (umag > 0.0) {
+                        const double r = umagb / umag;
+                        ub += u * r; vb += v * r;
+                    }
                 }
             } else {
                 fricb += taub;
             }
             if (ph.load_manning && fric_active) {
-                // fric = g*nbar^2/depth^(1/3)
-                nbarb += ph.g * 2.0 * nbar / pw * fricb;
-                depthb += -ph.g * (nbar * nbar) / (3.0 * pw * depth) * fricb;
+                // fric = g*nbar^2/depth^(1/3): d/dnbar = 2*fric/nbar, d/ddepth = -fric/(3*depth)
+                const double gp = ph.g / pw;
+                nbarb += 2.0 * gp * nbar * fricb;
+                depthb += -(1.0 / 3.0) * gp * (nbar * nbar) * (NL ? fh : 1.0 / depth) * fricb;
             }
             // hy = hvv + gh2 ; hx = huv*sfq ; gy = huv ; gx = (huu + gh2)*sfq ; fx = qx*f1*sfq ; fy = qy*f1
             const double hvvb = hyb_;

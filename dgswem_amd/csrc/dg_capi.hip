@@ -893,8 +893,16 @@ int adj_check(Ctx& c) {
 template <int P, int NT, bool NL>
 void launch_adj_nl(Ctx& c, const AdjArgs& a) {
     const int grid = (c.ne + NT - 1) / NT;
-    if (c.prm.ics == 2) hipLaunchKernelGGL((adj_stage_kernel<P, NT, true, NL>), dim3(grid), dim3(NT), 0, c.stream, a);
-    else hipLaunchKernelGGL((adj_stage_kernel<P, NT, false, NL>), dim3(grid), dim3(NT), 0, c.stream, a);
+    // NOLIBF=1 (quadratic friction, no hybrid factor) with the nonlinear equations: friction fixed
+    // at compile time
+    const bool qf = NL && c.phys.iflinbf == 0 && c.phys.ifhybf == 0;
+    if (c.prm.ics == 2) {
+        if (qf) hipLaunchKernelGGL((adj_stage_kernel<P, NT, true, NL, NL>), dim3(grid), dim3(NT), 0, c.stream, a);
+        else hipLaunchKernelGGL((adj_stage_kernel<P, NT, true, NL, false>), dim3(grid), dim3(NT), 0, c.stream, a);
+    } else {
+        if (qf) hipLaunchKernelGGL((adj_stage_kernel<P, NT, false, NL, NL>), dim3(grid), dim3(NT), 0, c.stream, a);
+        else hipLaunchKernelGGL((adj_stage_kernel<P, NT, false, NL, false>), dim3(grid), dim3(NT), 0, c.stream, a);
+    }
 }
 
 template <int P, int NT>
