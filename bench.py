@@ -146,6 +146,19 @@ def adjoint_leg(gpu, mesh, n_elem: int, steps: int, seed: int):
                         "per-node Manning n and depth"}
 
 
+def copy_calibration() -> int:
+    """2 GiB (larger than the 256 MiB last-level cache) copied device to device three times."""
+    import torch
+    n = 2 * 1024 ** 3 // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": a.numel() * 8, "copies": 3}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,7 +178,12 @@ def main() -> int:
     ap.add_argument("--adj-steps", type=int, default=2000)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 2 GiB device buffer three times (a plain copy kernel of known "
+                         "size, to scale the FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(one GPU per run; a multi-GPU run would shard the mesh by element partition)", file=sys.stderr)
@@ -231,7 +249,7 @@ def main() -> int:
                                                    args.seed)
 
     out = {
-        "metric": "DOF-stage updates/s FP64 p=1 1M elems",
+        "metric": "DOF-stage updates/s FP64 p=1 1M elems; ensemble speedup; adjoint/forward cost",
         "value": value,
         "unit": "DOF-stage updates/s",
         "n_gpus": 1,
