@@ -183,12 +183,19 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
     constexpr int NEGP = P + 2;
     __shared__ double ub_lds[3 * DOF][NT];     // cotangent of this element's modes
     __shared__ double lm_lds[3 * DOF][NT];     // this element's RHS cotangent (kept out of registers)
+    __shared__ double tpe[3][NEGP][DOF], teq[3][NEGP][DOF];   // edge tables (lane-divergent index)
     const DevTables& T = tab<P>();
     const ElemData& E = a.ed;
     const Phys& ph = a.ph;
     const int ne = E.ne;
     const int tid = threadIdx.x;
     const int e = xcd_tile(blockIdx.x, gridDim.x) * NT + tid;
+    for (int j = tid; j < 3 * NEGP * DOF; j += NT) {
+        const int l = j / (NEGP * DOF), g = (j / DOF) % NEGP, k = j % DOF;
+        tpe[l][g][k] = T.phi_edge[l][g][k];
+        teq[l][g][k] = T.edgeq[l][g][k];
+    }
+    __syncthreads();
     if (e >= ne) return;
     const int sid = E.sid ? (int)E.sid[e] : 0;
     const Scen sc = E.scen[sid];
@@ -223,11 +230,12 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
         hb[0] = c13 * (D0 + D1 + D2); hb[1] = -(c16 * (D0 + D1)) + c13 * D2; hb[2] = -(0.5 * D0) + 0.5 * D1;
     }
     const int info = E.sinfo[e];
+    const int nb0 = E.snbr[e], nb1 = E.snbr[(size_t)ne + e], nb2 = E.snbr[2 * (size_t)ne + e];
 #pragma unroll 1
     for (int slot = 0; slot < 3; slot++) {
         const int bits = side_bits(info, slot);
         const int L = bits & 3, role = (bits >> 2) & 1, kind = (bits >> 3) & 3, nled = (bits >> 5) & 3;
-        const int nb = E.snbr[(size_t)slot * ne + e];
+        const int nb = slot == 0 ? nb0 : (slot == 1 ? nb1 : nb2);
         int va = (L + 1) % 3, vb = (L + 2) % 3;
         if (role) { const int t = va; va = vb; vb = t; }
         const double dy = E.yv[(size_t)vb * ne + e] - E.yv[(size_t)va * ne + e];
@@ -256,8 +264,8 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
             for (int I = 0; I < NEGP; I++) {
                 const int gi = I, gx = NEGP - 1 - I;
                 const int gown = role ? gx : gi, gnb = role ? gi : gx;
-                const double* po = T.phi_edge[L][gown];
-                const double* pn = T.phi_edge[nled][gnb];
+                const double* po = tpe[L][gown];
+                const double* pn = tpe[nled][gnb];
                 const double zo = trace<DOF>(U[0], po), xo = trace<DOF>(U[1], po), yo = trace<DOF>(U[2], po);
                 const double zn = trace<DOF>(V[0], pn), xn = trace<DOF>(V[1], pn), yn = trace<DOF>(V[2], pn);
                 const double ho = trace<3>(hb, po), hn = trace<3>(hbn, pn);
@@ -267,8 +275,8 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
                     sn = E.sfaced[((size_t)nled * NEGP + gnb) * ne + nb];
                 }
                 // flux cotangent: the "in" element's RHS took -w_in*F, the "ex" element's +w_ex*F
-                const double* eqo = T.edgeq[L][gown];
-                const double* eqn = T.edgeq[nled][gnb];
+                const double* eqo = teq[L][gown];
+                const double* eqn = teq[nled][gnb];
                 double fb[3];
 #pragma unroll
                 for (int f = 0; f < 3; f++) {
@@ -297,7 +305,7 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
         } else {
 #pragma unroll 1
             for (int I = 0; I < NEGP; I++) {
-                const double* pv = T.phi_edge[L][I];
+                const double* pv = tpe[L][I];
                 const double ze_in = trace<DOF>(U[0], pv), qx_in = trace<DOF>(U[1], pv), qy_in = trace<DOF>(U[2], pv);
                 const double hb_in = trace<3>(hb, pv);
                 const double sf = SPH ? E.sfaced[((size_t)L * NEGP + I) * ne + e] : 1.0;

@@ -109,7 +109,17 @@ struct Ctx {
     DevBuf<Scen> scen;
     DevBuf<double> xv, yv, dpv, nbar, cori_el, sfacq, sfaced;
     DevBuf<int> sinfo, snbr, nm, n2e_off, n2e;
+    std::vector<int> h_sinfo, h_snbr;            // host copies (staging lists)
     bool has_cori_el = false;
+    // block-local staging of the stage kernel, per block size: each block copies the states of its
+    // own elements and of its halo (out-of-block internal neighbours) into LDS
+    struct Staging {
+        int ns = 0;                              // staged elements per block: block size + largest halo
+        size_t lds = 0;                          // dynamic LDS bytes per block
+        DevBuf<int> hoff, hidx, snbl;            // halo offsets per tile, halo elements, packed local indices
+    };
+    std::map<int, Staging> stg;
+    int max_lds = 65536;                         // LDS bytes per block the device allows
     DevBuf<double> nmm, outbuf, partial;
     DevBuf<int> nodecode;
     // forcing
@@ -546,6 +556,7 @@ int upload_mesh(Ctx& c) {
     HIPCHK(c.nm.upload(nm.data(), nm.size(), c.stream));
     HIPCHK(c.sinfo.upload(sinfo.data(), sinfo.size(), c.stream));
     HIPCHK(c.snbr.upload(snbr.data(), snbr.size(), c.stream));
+    c.stg.clear();
     HIPCHK(c.n2e_off.upload(off.data(), off.size(), c.stream));
     HIPCHK(c.n2e.upload(n2e.data(), n2e.size(), c.stream));
     HIPCHK(c.scen.upload(scen.data(), scen.size(), c.stream));
@@ -574,6 +585,8 @@ int upload_mesh(Ctx& c) {
     for (int i = 0; i < 2; i++) HIPCHK(c.wd[i].alloc(ne));
     // the host vectors go out of scope: wait for the copies
     HIPCHK(hipStreamSynchronize(c.stream));
+    c.h_sinfo = std::move(sinfo);
+    c.h_snbr = std::move(snbr);
     if (upload_met(c)) return 1;
     return upload_forcing(c);
 }
@@ -587,6 +600,12 @@ int init_ctx_common(Ctx& c, const dg_params* prm, bool manning) {
     if (!build_rk(prm->rk_stage, prm->rk_order, c.rk, err)) return fail(err);
     c.dof = c.T.dof; c.nagp = c.T.nagp; c.negp = c.T.negp;
     c.block = 128;
+    {
+        int dev = 0, lds = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        c.max_lds = lds;
+    }
     HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     {
         std::lock_guard<std::mutex> lk(g_mu);
@@ -687,29 +706,94 @@ int build_stage_tables(Ctx& c, int it, int n) {
     return 0;
 }
 
-template <int P, int NT, bool MET, bool NL>
-void launch_stage_v(Ctx& c, const StageArgs& a, bool sph, bool fuse) {
-    const int grid = (c.ne + NT - 1) / NT;
-    if (sph) {
-        if (fuse) hipLaunchKernelGGL((stage_kernel<P, NT, true, true, MET, NL>), dim3(grid), dim3(NT), 0, c.stream, a);
-        else hipLaunchKernelGGL((stage_kernel<P, NT, true, false, MET, NL>), dim3(grid), dim3(NT), 0, c.stream, a);
-    } else {
-        if (fuse) hipLaunchKernelGGL((stage_kernel<P, NT, false, true, MET, NL>), dim3(grid), dim3(NT), 0, c.stream, a);
-        else hipLaunchKernelGGL((stage_kernel<P, NT, false, false, MET, NL>), dim3(grid), dim3(NT), 0, c.stream, a);
+// Static LDS bytes of stage_kernel<P, NT, ..., MET, ...> (its __shared__ arrays).
+size_t stage_static_lds(int dof, int negp, int nt, bool met) {
+    return (size_t)8 * (2 * 3 * dof * nt + (met ? 8 : 1) * nt + 2 * 3 * negp * dof);
+}
+
+// Block size the stage kernel runs with: the chosen one for nonlinear runs, 64 for linearised runs.
+int stage_nt(const Ctx& c, int nt) { return c.prm.nolicat != 0 ? nt : 64; }
+
+// Halo lists of the stage kernel's blocks of `nt` elements (tile t = elements [t*nt, t*nt+nt)):
+// the out-of-tile internal neighbours of each tile in ascending order, and per element the local
+// staging index of each internal side's neighbour (tile element j -> j, halo entry i -> nt + i),
+// packed 10 bits per side.  Built once per block size; rejected when it does not fit in LDS.
+int ensure_staging(Ctx& c, int nt) {
+    if (c.stg.count(nt)) return 0;
+    const int ne = c.ne, ntiles = (ne + nt - 1) / nt;
+    std::vector<int> hoff(ntiles + 1, 0), hidx, snbl(ne, 0), halo;
+    int hmax = 0;
+    for (int t = 0; t < ntiles; t++) {
+        const int b = t * nt, en = std::min(ne, b + nt);
+        halo.clear();
+        for (int e = b; e < en; e++)
+            for (int sl = 0; sl < 3; sl++) {
+                if (((side_bits(c.h_sinfo[e], sl) >> 3) & 3) != SIDE_INTERNAL) continue;
+                const int nb = c.h_snbr[(size_t)sl * ne + e];
+                if (nb < b || nb >= en) halo.push_back(nb);
+            }
+        std::sort(halo.begin(), halo.end());
+        halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+        hmax = std::max(hmax, (int)halo.size());
+        for (int e = b; e < en; e++) {
+            int packed = 0;
+            for (int sl = 0; sl < 3; sl++) {
+                if (((side_bits(c.h_sinfo[e], sl) >> 3) & 3) != SIDE_INTERNAL) continue;
+                const int nb = c.h_snbr[(size_t)sl * ne + e];
+                int ln;
+                if (nb >= b && nb < en) ln = nb - b;
+                else ln = nt + (int)(std::lower_bound(halo.begin(), halo.end(), nb) - halo.begin());
+                packed |= ln << (10 * sl);
+            }
+            snbl[e] = packed;
+        }
+        hidx.insert(hidx.end(), halo.begin(), halo.end());
+        hoff[t + 1] = (int)hidx.size();
     }
+    const int ns = nt + hmax;
+    if (ns > 1023) return fail("stage kernel staging: halo too large for block size " + std::to_string(nt));
+    const size_t dyn = ((size_t)8 * (3 * c.dof + 3) * ns + ns + 7) / 8 * 8;
+    if (dyn + stage_static_lds(c.dof, c.negp, nt, c.met) > (size_t)c.max_lds)
+        return fail("stage kernel staging: block size " + std::to_string(nt) + " needs more LDS than a block may use");
+    Ctx::Staging& S = c.stg[nt];
+    S.ns = ns;
+    S.lds = dyn;
+    if (hidx.empty()) hidx.push_back(0);
+    HIPCHK(S.hoff.upload(hoff.data(), hoff.size(), c.stream));
+    HIPCHK(S.hidx.upload(hidx.data(), hidx.size(), c.stream));
+    HIPCHK(S.snbl.upload(snbl.data(), snbl.size(), c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return 0;
+}
+
+template <int P, int NT, bool MET, bool NL>
+int launch_stage_v(Ctx& c, const StageArgs& a0, bool sph, bool fuse) {
+    const int grid = (c.ne + NT - 1) / NT;
+    auto it = c.stg.find(NT);
+    if (it == c.stg.end()) return fail("internal: stage kernel staging not built for this block size");
+    StageArgs a = a0;
+    a.hoff = it->second.hoff.p; a.hidx = it->second.hidx.p; a.snbl = it->second.snbl.p; a.ns = it->second.ns;
+    const size_t lds = it->second.lds;
+    if (sph) {
+        if (fuse) hipLaunchKernelGGL((stage_kernel<P, NT, true, true, MET, NL>), dim3(grid), dim3(NT), lds, c.stream, a);
+        else hipLaunchKernelGGL((stage_kernel<P, NT, true, false, MET, NL>), dim3(grid), dim3(NT), lds, c.stream, a);
+    } else {
+        if (fuse) hipLaunchKernelGGL((stage_kernel<P, NT, false, true, MET, NL>), dim3(grid), dim3(NT), lds, c.stream, a);
+        else hipLaunchKernelGGL((stage_kernel<P, NT, false, false, MET, NL>), dim3(grid), dim3(NT), lds, c.stream, a);
+    }
+    return 0;
 }
 
 // Nonlinear runs (NOLICAT != 0) use the kernels with NLEQ/LEQ folded at compile time; linearised
 // runs use the general kernels with 64-thread blocks.
 template <int P, int NT>
-void launch_stage(Ctx& c, const StageArgs& a, bool sph, bool fuse) {
+int launch_stage(Ctx& c, const StageArgs& a, bool sph, bool fuse) {
     if (c.prm.nolicat != 0) {
-        if (c.met) launch_stage_v<P, NT, true, true>(c, a, sph, fuse);
-        else launch_stage_v<P, NT, false, true>(c, a, sph, fuse);
-    } else {
-        if (c.met) launch_stage_v<P, 64, true, false>(c, a, sph, fuse);
-        else launch_stage_v<P, 64, false, false>(c, a, sph, fuse);
+        if (c.met) return launch_stage_v<P, NT, true, true>(c, a, sph, fuse);
+        return launch_stage_v<P, NT, false, true>(c, a, sph, fuse);
     }
+    if (c.met) return launch_stage_v<P, 64, true, false>(c, a, sph, fuse);
+    return launch_stage_v<P, 64, false, false>(c, a, sph, fuse);
 }
 
 // Launch shapes, by threads per block (one thread per element): 64, 128, and 256 for p=1.
@@ -719,15 +803,13 @@ bool valid_block(int p, int nt) {
 }
 
 template <int P>
-void launch_stage_nt(Ctx& c, const StageArgs& a, bool sph, bool fuse, int nt) {
-    if (nt == 64) {
-        launch_stage<P, 64>(c, a, sph, fuse);
-    } else if constexpr (P == 1) {
-        if (nt == 128) launch_stage<P, 128>(c, a, sph, fuse);
-        else launch_stage<P, 256>(c, a, sph, fuse);
-    } else {
-        launch_stage<P, 128>(c, a, sph, fuse);
+int launch_stage_nt(Ctx& c, const StageArgs& a, bool sph, bool fuse, int nt) {
+    if (nt == 64) return launch_stage<P, 64>(c, a, sph, fuse);
+    if constexpr (P == 1) {
+        if (nt == 128) return launch_stage<P, 128>(c, a, sph, fuse);
+        return launch_stage<P, 256>(c, a, sph, fuse);
     }
+    return launch_stage<P, 128>(c, a, sph, fuse);
 }
 
 StageArgs stage_args(Ctx& c, int irk, const int slot[], bool captured = false) {
@@ -775,7 +857,7 @@ int launch_step(Ctx& c, bool capturing) {
             if (a.bprev[i] != 0.0 && !c.keep_r[i]) return fail("internal: RHS of an earlier stage was not kept");
         const bool fuse = wdon && !lim;
         int ev = prof_begin(c, K_STAGE, capturing);
-        launch_stage_nt<P>(c, a, sph, fuse, c.block);
+        if (launch_stage_nt<P>(c, a, sph, fuse, c.block)) return 1;
         prof_end(c, ev);
         HIPCHK(hipGetLastError());
         if (fuse) c.wdcur = 1 - c.wdcur;
@@ -838,7 +920,7 @@ int launch_step_explicit(Ctx& c, double* const* st) {
         a.bcur = c.rk.b[irk][irk];
         a.rout = c.keep_r[irk] ? c.r[irk].p : nullptr;
         a.irk = irk;
-        launch_stage_nt<P>(c, a, sph, false, c.block);
+        if (launch_stage_nt<P>(c, a, sph, false, c.block)) return 1;
         HIPCHK(hipGetLastError());
     }
     c.hstep++;
@@ -1198,6 +1280,7 @@ int dg_gpu_step(int h, int it, int nsteps) {
         if (m.needs > 0 && m.nbfr == 0) return fail("tidal forcing not set (dg_gpu_set_tide)");
         if (m.nfeds > 0 && m.nffr == 0) return fail("flux forcing not set (dg_gpu_set_flow)");
     }
+    if (ensure_staging(*c, stage_nt(*c, c->block))) return 1;
     for (int s0 = 0; s0 < nsteps; s0 += kTableChunk) {
         const int n = std::min(kTableChunk, nsteps - s0);
         if (build_stage_tables(*c, it + s0, n)) return 1;
@@ -1349,6 +1432,7 @@ int dg_adj_gauges(int h, int it, int nsteps, double* eta) {
     if (!c) return 1;
     if (c->ng == 0) return fail("no gauges set (dg_adj_set_gauges)");
     if (nsteps <= 0) return 0;
+    if (ensure_staging(*c, stage_nt(*c, c->block))) return 1;
     HIPCHK(c->geta.alloc((size_t)nsteps * c->ng));
     const int rc = c->prm.p == 1 ? adj_forward<1>(*c, it, nsteps, 1, false) : adj_forward<2>(*c, it, nsteps, 1, false);
     if (rc) return rc;
@@ -1364,6 +1448,7 @@ int dg_adj_gradient(int h, int it, int nsteps, const double* data, double weight
     if (!c) return 1;
     if (adj_check(*c)) return 1;
     if (nsteps <= 0) return fail("nsteps must be positive");
+    if (ensure_staging(*c, stage_nt(*c, c->block))) return 1;
     if (interval <= 0) interval = std::max(1, (int)std::lround(std::sqrt((double)nsteps)));
     interval = std::min(interval, nsteps);
     const int nrk = c->rk.nrk, ne = c->ne, np = c->np;
@@ -1501,13 +1586,15 @@ int dg_gpu_autotune(int h, int* block) {
     float best = 1e30f;
     int best_nt = c->block, rc = 0;
     for (int nt : cand) {
+        if (ensure_staging(*c, stage_nt(*c, nt))) continue;   // does not fit: not a candidate
         float ms = 0.f;
         for (int rep = 0; rep < 2 && !rc; rep++) {   // the first pass warms caches and code
             if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail("hipEventRecord failed"); break; }
-            for (int k = 0; k < 3; k++) {
-                if (c->prm.p == 1) launch_stage_nt<1>(*c, a, sph, fuse, nt);
-                else launch_stage_nt<2>(*c, a, sph, fuse, nt);
+            for (int k = 0; k < 3 && !rc; k++) {
+                if (c->prm.p == 1) rc = launch_stage_nt<1>(*c, a, sph, fuse, nt);
+                else rc = launch_stage_nt<2>(*c, a, sph, fuse, nt);
             }
+            if (rc) break;
             hipError_t e = hipGetLastError();
             if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -1534,6 +1621,7 @@ int dg_gpu_set_block(int h, int block) {
     if (!valid_block(c->prm.p, block))
         return fail("block size must be 64, 128 or, for p=1, 256 threads");
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (ensure_staging(*c, stage_nt(*c, block))) return 1;
     if (block != c->block) { c->block = block; c->drop_graphs(); }
     return 0;
 }

@@ -48,6 +48,12 @@ struct StageArgs {
     double acur, bcur;                   // ATVD(IRK,IRK), BTVD(IRK,IRK)
     double* __restrict__ rout;           // RHS(IRK) kept for later stages, or nullptr
     int irk;                             // stage index
+    // block-local staging: tile t (elements [t*NT, t*NT+NT)) stages its elements and its halo
+    // hidx[hoff[t]..hoff[t+1]) in LDS; snbl packs each side's local neighbour index (10 bits each)
+    const int* __restrict__ hoff;
+    const int* __restrict__ hidx;
+    const int* __restrict__ snbl;
+    int ns;                              // staged elements per block (LDS row length)
 };
 
 // ------------------------------------------------------------------------------------------
@@ -189,12 +195,44 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
     __shared__ double acc_lds[2][3 * DOF][NT];
     // wind stress at the element's vertices and the pressure gradient sums (MET)
     __shared__ double met_lds[MET ? 8 : 1][NT];
+    // edge basis and edge quadrature tables: indexed by each lane's own local edge number, so
+    // read from LDS (short latency) rather than as divergent constant-memory loads
+    __shared__ double tpe[3][NEGP][DOF], teq[3][NEGP][DOF];
+    // staged states: rows [field*DOF + mode] then the three bathymetry modes, ns entries each, then
+    // the wet/dry flags (bytes)
+    extern __shared__ double stg[];
     const DevTables& T = tab<P>();
     const ElemData& E = a.ed;
     const Phys& ph = a.ph;
     const int ne = E.ne;
     const int tid = threadIdx.x;
-    const int e = xcd_tile(blockIdx.x, gridDim.x) * NT + tid;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int e = tile * NT + tid;
+    const double c13 = 1.0 / 3.0, c16 = 1.0 / 6.0;
+    const int ns = a.ns;
+    uint8_t* const stwd = reinterpret_cast<uint8_t*>(stg + (size_t)(3 * DOF + 3) * ns);
+    for (int j = tid; j < 3 * NEGP * DOF; j += NT) {
+        const int l = j / (NEGP * DOF), g = (j / DOF) % NEGP, k = j % DOF;
+        tpe[l][g][k] = T.phi_edge[l][g][k];
+        teq[l][g][k] = T.edgeq[l][g][k];
+    }
+    {
+        // this tile's elements and its halo: all loads issued together, one latency
+        const int h0 = a.hoff[tile], nh = a.hoff[tile + 1] - h0;
+        for (int j = tid; j < NT + nh; j += NT) {
+            const int g = j < NT ? tile * NT + j : a.hidx[h0 + j - NT];
+            if (g < ne) {
+#pragma unroll
+                for (int f = 0; f < 3 * DOF; f++) stg[f * ns + j] = a.uin[(size_t)f * ne + g];
+                const double d0 = E.dpv[g], d1 = E.dpv[(size_t)ne + g], d2 = E.dpv[2 * (size_t)ne + g];
+                stg[(3 * DOF) * ns + j] = c13 * (d0 + d1 + d2);
+                stg[(3 * DOF + 1) * ns + j] = -(c16 * (d0 + d1)) + c13 * d2;
+                stg[(3 * DOF + 2) * ns + j] = -(0.5 * d0) + 0.5 * d1;
+                stwd[j] = a.wd_in[g];
+            }
+        }
+    }
+    __syncthreads();
     if (e >= ne) return;
     const int sid = E.sid ? (int)E.sid[e] : 0;
     const Scen sc = E.scen[sid];
@@ -204,26 +242,28 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
 #pragma unroll
     for (int f = 0; f < 3; f++)
 #pragma unroll
-        for (int k = 0; k < DOF; k++) U[f][k] = a.uin[(size_t)(f * DOF + k) * ne + e];
-    const int wd = a.wd_in[e];
-    const double c13 = 1.0 / 3.0, c16 = 1.0 / 6.0;
+        for (int k = 0; k < DOF; k++) U[f][k] = stg[(f * DOF + k) * ns + tid];
+    const int wd = stwd[tid];
     double AREAS, hb[3];
     {
         const double X0 = E.xv[e], X1 = E.xv[(size_t)ne + e], X2 = E.xv[2 * (size_t)ne + e];
         const double Y0 = E.yv[e], Y1 = E.yv[(size_t)ne + e], Y2 = E.yv[2 * (size_t)ne + e];
-        const double D0 = E.dpv[e], D1 = E.dpv[(size_t)ne + e], D2 = E.dpv[2 * (size_t)ne + e];
         AREAS = (X0 - X2) * (Y1 - Y2) + (X2 - X1) * (Y0 - Y2);
-        hb[0] = c13 * (D0 + D1 + D2); hb[1] = -(c16 * (D0 + D1)) + c13 * D2; hb[2] = -(0.5 * D0) + 0.5 * D1;
+#pragma unroll
+        for (int m = 0; m < 3; m++) hb[m] = stg[(3 * DOF + m) * ns + tid];
     }
 #pragma unroll
     for (int j = 0; j < 3 * DOF; j++) { acc_lds[0][j][tid] = 0.0; acc_lds[1][j][tid] = 0.0; }
 
+    // the three sides' neighbours (global indices, and staged indices of internal neighbours)
     const int info = E.sinfo[e];
+    const int nb0 = E.snbr[e], nb1 = E.snbr[(size_t)ne + e], nb2 = E.snbr[2 * (size_t)ne + e];
+    const int sl = a.snbl[e];
 #pragma unroll 1
     for (int slot = 0; slot < 3; slot++) {
         const int bits = side_bits(info, slot);
         const int L = bits & 3, role = (bits >> 2) & 1, kind = (bits >> 3) & 3, nled = (bits >> 5) & 3;
-        const int nb = E.snbr[(size_t)slot * ne + e];
+        const int nb = slot == 0 ? nb0 : (slot == 1 ? nb1 : nb2);
         // edge geometry in the reference's node order (the counter-clockwise order of NEDEL(1))
         int va = (L + 1) % 3, vb = (L + 2) % 3;
         if (role) { const int t = va; va = vb; vb = t; }
@@ -233,18 +273,17 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
         const double nx = dy / xlen, ny = -dx / xlen;
         const double tx = -ny, ty = nx;
         if (kind == SIDE_INTERNAL) {
-            const int wdn = a.wd_in[nb];
+            const int ln = (sl >> (10 * slot)) & 1023;
+            const int wdn = stwd[ln];
             if (wd == 0 && wdn == 0) continue;
             double V[3][DOF];
 #pragma unroll
             for (int f = 0; f < 3; f++)
 #pragma unroll
-                for (int k = 0; k < DOF; k++) V[f][k] = a.uin[(size_t)(f * DOF + k) * ne + nb];
+                for (int k = 0; k < DOF; k++) V[f][k] = stg[(f * DOF + k) * ns + ln];
             double hbn[3];
-            {
-                const double d0 = E.dpv[nb], d1 = E.dpv[(size_t)ne + nb], d2 = E.dpv[2 * (size_t)ne + nb];
-                hbn[0] = c13 * (d0 + d1 + d2); hbn[1] = -(c16 * (d0 + d1)) + c13 * d2; hbn[2] = -(0.5 * d0) + 0.5 * d1;
-            }
+#pragma unroll
+            for (int m = 0; m < 3; m++) hbn[m] = stg[(3 * DOF + m) * ns + ln];
             const int wd_in_el = role ? wdn : wd, wd_ex_el = role ? wd : wdn;
             const double edfac = xlen / AREAS;
             double acc[3][DOF];
@@ -252,12 +291,12 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
             for (int f = 0; f < 3; f++)
 #pragma unroll
                 for (int k = 0; k < DOF; k++) acc[f][k] = 0.0;
-#pragma unroll 1
+#pragma unroll
             for (int I = 0; I < NEGP; I++) {
                 const int gi = I, gx = NEGP - 1 - I;
                 const int gown = role ? gx : gi, gnb = role ? gi : gx;
-                const double* po = T.phi_edge[L][gown];
-                const double* pn = T.phi_edge[nled][gnb];
+                const double* po = tpe[L][gown];
+                const double* pn = tpe[nled][gnb];
                 const double zo = trace<DOF>(U[0], po), xo = trace<DOF>(U[1], po), yo = trace<DOF>(U[2], po);
                 const double zn = trace<DOF>(V[0], pn), xn = trace<DOF>(V[1], pn), yn = trace<DOF>(V[2], pn);
                 const double ho = trace<3>(hb, po), hn = trace<3>(hbn, pn);
@@ -305,7 +344,7 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
                     else if (mode == 3) { fo = r0; gm = role ? 0.0 : r1; hm = role ? 0.0 : r2; }
                     else { gm = r1; hm = r2; }
                 }
-                const double* eq = T.edgeq[L][gown];
+                const double* eq = teq[L][gown];
 #pragma unroll
                 for (int k = 0; k < DOF; k++) {
                     const double w = edfac * eq[k];
@@ -324,7 +363,7 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
             if (wd == 0) continue;
 #pragma unroll 1
             for (int I = 0; I < NEGP; I++) {
-                const double* pv = T.phi_edge[L][I];
+                const double* pv = tpe[L][I];
                 const double ze_in = trace<DOF>(U[0], pv), qx_in = trace<DOF>(U[1], pv), qy_in = trace<DOF>(U[2], pv);
                 const double hb_in = trace<3>(hb, pv);
                 const double sf = SPH ? E.sfaced[((size_t)L * NEGP + I) * ne + e] : 1.0;
