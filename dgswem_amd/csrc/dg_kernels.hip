@@ -465,8 +465,15 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
             met_lds[6][tid] = pdx;
             met_lds[7][tid] = pdy;
         }
-        const double depth_c = hb[0] + U[0][0];
-        const double fh_nl_c = NL ? 1.0 / depth_c : 1.0 / (ph.nleq * depth_c + ph.leq);
+        // hybrid friction factor (1 + (HBREAK/H_c)^FTHETA)^(FGAMMA/FTHETA) from the element-mean
+        // depth (src/rhs_dg_hydro.F:306-309): the same at every area point, evaluated once and only
+        // when the hybrid law is on
+        double hybf = 1.0;
+        if (ph.ifhybf == 1) {
+            const double depth_c = hb[0] + U[0][0];
+            const double fh_nl_c = NL ? 1.0 / depth_c : 1.0 / (ph.nleq * depth_c + ph.leq);
+            hybf = pow(1.0 + pow(ph.hbreak * fh_nl_c, ph.ftheta), ph.fgamma / ph.ftheta);
+        }
         const double cori = E.cori_el ? E.cori_el[e] : sc.cori;
         const double nbar = ph.load_manning ? E.nbar[e] : 0.0;
 #pragma unroll 1
@@ -511,7 +518,7 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
             if (ph.iflinbf == 0) {
                 const double umag = sqrt(u * u + v * v);
                 tau = tau * umag * fh;
-                if (ph.ifhybf == 1) tau = tau * pow(1.0 + pow(ph.hbreak * fh_nl_c, ph.ftheta), ph.fgamma / ph.ftheta);
+                if (ph.ifhybf == 1) tau = tau * hybf;
                 const double cap = 0.9 * sc.dtdph;
                 if (cap < tau) tau = cap;
             }
