@@ -78,6 +78,19 @@ def test_p2_closed_basin_bitwise():
         np.testing.assert_array_equal(x, y)
 
 
+def test_p2_limiter_wetdry_matches_oracle():
+    """The bench configuration at p=2 in small: limiter 5, wetting and drying over a shelf that rises
+    onshore, five tidal constituents, Manning n per node."""
+    case = mesh.delaunay_basin(n_elem=4000, seed=43, lx=2.0e5, ly=1.2e5, p=2, cfl=0.15, nsteps=300)
+    a, b = run_both(case, 300)
+    assert (b[3] == 0).sum() > 0                  # some elements are dry
+    np.testing.assert_array_equal(a[3], b[3])
+    for x, y, name in zip(a[:3], b[:3], ("ze", "qx", "qy")):
+        r = rel_l2(x, y)
+        print(f"{name} rel L2 {r:.3e}", flush=True)
+        assert r <= TOL
+
+
 def test_p2_block_sizes_bitwise():
     case = tidal_p2(nsteps=100)
     st = adjoint.hump_state(case, amplitude=0.3)
