@@ -141,6 +141,110 @@ __device__ __forceinline__ void llf_flux_adj(const Phys& P, double zi, double ze
     bar[7] = hbb[1];
 }
 
+// The part of llf_flux_adj that one side of an internal edge keeps: the cotangents of side `own`'s
+// arguments (ze, qx, qy, hb) only.  Both elements of an edge run it for their own side, with the
+// same canonical (in, ex) argument order, so no lane evaluates the other side's reverse.
+template <bool NL>
+__device__ __forceinline__ void llf_flux_adj_side(const Phys& P, double zi, double ze, double xi, double xe,
+                                                  double yi, double ye, double hi, double he, double sfi,
+                                                  double sfe, double nx, double ny, double G,
+                                                  const double fb[3], int own, double& zbar,
+                                                  double& xbar, double& ybar, double& hbar) {
+#pragma clang fp contract(fast)
+    const double NLEQ = NL ? 1.0 : P.nleq, LEQ = NL ? 0.0 : P.leq, FG_L = NL ? 0.0 : P.fg_l;
+    const double NLEQG = G * NLEQ;
+    const double j1 = ze - zi, j2 = xe - xi, j3 = ye - yi;
+    // forward values of both sides (the wave-speed bound needs both)
+    const double side_z[2] = {zi, ze}, side_x[2] = {xi, xe}, side_y[2] = {yi, ye}, side_h[2] = {hi, he};
+    const double side_s[2] = {sfi, sfe};
+    double fh[2], u[2], v[2], un[2], c[2], K[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const double ht = side_z[q] * NLEQ + side_h[q];
+        fh[q] = NL ? 1.0 / ht : 1.0 / (NLEQ * ht + LEQ);
+        u[q] = side_x[q] * fh[q];
+        v[q] = side_y[q] * fh[q];
+        un[q] = (u[q] * nx + v[q] * ny) * NLEQ;
+        const double nxs = nx * side_s[q];
+        K[q] = ny * ny + nxs * nxs;
+        c[q] = sqrt(G * ht * K[q]);
+    }
+    const double cand[6] = {fabs(un[0] + c[0]), fabs(un[0]), fabs(un[0] - c[0]),
+                            fabs(un[1] + c[1]), fabs(un[1]), fabs(un[1] - c[1])};
+    int arg = 0;
+    double em = cand[0];
+#pragma unroll
+    for (int q = 1; q < 6; q++)
+        if (cand[q] > em) { em = cand[q]; arg = q; }
+    double emb, j1b, j2b, j3b;
+    if constexpr (NL) {
+        emb = -0.5 * (j1 * fb[0] + j2 * fb[1] + j3 * fb[2]);
+        j1b = -0.5 * em * fb[0];
+        j2b = -0.5 * em * fb[1];
+        j3b = -0.5 * em * fb[2];
+    } else {
+        const double nx2 = NLEQ + LEQ * nx * nx, ny2 = NLEQ + LEQ * ny * ny, nxy = nx * ny;
+        emb = -0.5 * j1 * fb[0] - 0.5 * (j2 * nx2 + LEQ * j3 * nxy) * fb[1] -
+              0.5 * (j3 * ny2 + LEQ * j2 * nxy) * fb[2];
+        j1b = -0.5 * em * fb[0];
+        j2b = -0.5 * em * (nx2 * fb[1] + LEQ * nxy * fb[2]);
+        j3b = -0.5 * em * (LEQ * nxy * fb[1] + ny2 * fb[2]);
+    }
+    // this side's forward values
+    const double z = own ? ze : zi, x = own ? xe : xi, y = own ? ye : yi, h = own ? he : hi;
+    const double sf = own ? sfe : sfi;
+    const double fhs = own ? fh[1] : fh[0], us = own ? u[1] : u[0], vs = own ? v[1] : v[0];
+    const double cs = own ? c[1] : c[0], Ks = own ? K[1] : K[0];
+    double unb = 0.0, cb = 0.0;
+    if (arg / 3 == own) {
+        const int q = arg % 3;
+        const double uns = own ? un[1] : un[0];
+        const double val = q == 0 ? uns + cs : (q == 1 ? uns : uns - cs);
+        const double sg = val > 0.0 ? 1.0 : (val < 0.0 ? -1.0 : 0.0);
+        unb = sg * emb;
+        if (q == 0) cb = sg * emb;
+        else if (q == 2) cb = -sg * emb;
+    }
+    const double fx1b = 0.5 * nx * fb[0], fy1b = 0.5 * ny * fb[0];
+    const double fx2b = 0.5 * nx * fb[1], fy2b = 0.5 * ny * fb[1];
+    const double fx3b = 0.5 * nx * fb[2], fy3b = 0.5 * ny * fb[2];
+    const double fnl = NL ? 1.0 : NLEQ + LEQ * h;
+    const double fu = NLEQ * x, fv = NLEQ * y, fg = NLEQG * z;
+    double zb = 0.0, xb = 0.0, yb = 0.0, hb = 0.0;
+    const double hvvb = fy3b;
+    const double gh2b = fy3b + sf * fx2b;
+    const double huvb = sf * fx3b + fy2b;
+    const double huub = sf * fx2b;
+    const double fgb = (0.5 * z + h) * gh2b;
+    zb += (NL ? 0.5 * fg : 0.5 * fg + FG_L) * gh2b;
+    hb += fg * gh2b;
+    const double fub = us * huub + vs * huvb;
+    double ub = fu * huub;
+    const double fvb = vs * hvvb;
+    double vb = fv * hvvb + fu * huvb;
+    ub += nx * NLEQ * unb;
+    vb += ny * NLEQ * unb;
+    double htb;
+    if constexpr (NL) htb = cs > 0.0 ? 0.5 * cs * fhs * cb : 0.0;
+    else htb = cs > 0.0 ? G * Ks / (2.0 * cs) * cb : 0.0;
+    xb += fhs * ub;
+    yb += fhs * vb;
+    const double fhb = x * ub + y * vb;
+    htb += -NLEQ * fhs * fhs * fhb;
+    zb += NLEQG * fgb;
+    xb += NLEQ * fub;
+    yb += NLEQ * fvb;
+    xb += fnl * sf * fx1b;
+    yb += fnl * fy1b;
+    if constexpr (!NL) hb += LEQ * (x * sf * fx1b + y * fy1b);
+    zb += NLEQ * htb;
+    hb += htb;
+    zbar = own ? zb + j1b : zb - j1b;
+    xbar = own ? xb + j2b : xb - j2b;
+    ybar = own ? yb + j3b : yb - j3b;
+    hbar = hb;
+}
+
 struct AdjArgs {
     ElemData ed;
     Phys ph;
@@ -154,6 +258,12 @@ struct AdjArgs {
     double* __restrict__ out;            // cotangent of U(IRK)
     double* __restrict__ gdp;            // [3][ne] accumulated cotangent of the vertex depths
     double* __restrict__ gnbar;          // [ne] accumulated cotangent of the element mean Manning n
+    // block-local staging, as in the forward stage kernel: tile t stages its elements and its halo
+    // hidx[hoff[t]..hoff[t+1]); snbl packs each side's local neighbour index (10 bits each)
+    const int* __restrict__ hoff;
+    const int* __restrict__ hidx;
+    const int* __restrict__ snbl;
+    int ns;                              // staged elements per block (LDS row length)
 };
 
 // RHS cotangent of element `el` (its own stage contribution): sum_j b_j DT src_j.
@@ -182,25 +292,54 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
     constexpr int NAGP = (P == 1) ? 3 : 6;
     constexpr int NEGP = P + 2;
     __shared__ double ub_lds[3 * DOF][NT];     // cotangent of this element's modes
-    __shared__ double lm_lds[3 * DOF][NT];     // this element's RHS cotangent (kept out of registers)
     __shared__ double tpe[3][NEGP][DOF], teq[3][NEGP][DOF];   // edge tables (lane-divergent index)
+    // staged rows (ns entries each): state [3*DOF], RHS cotangent [3*DOF], bathymetry modes [3], AREAS
+    extern __shared__ double stg[];
+    constexpr int RL = 3 * DOF, RH = 6 * DOF, RA = 6 * DOF + 3;
     const DevTables& T = tab<P>();
     const ElemData& E = a.ed;
     const Phys& ph = a.ph;
     const int ne = E.ne;
     const int tid = threadIdx.x;
-    const int e = xcd_tile(blockIdx.x, gridDim.x) * NT + tid;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int e = tile * NT + tid;
+    const double c13 = 1.0 / 3.0, c16 = 1.0 / 6.0;
+    const int ns = a.ns;
     for (int j = tid; j < 3 * NEGP * DOF; j += NT) {
         const int l = j / (NEGP * DOF), g = (j / DOF) % NEGP, k = j % DOF;
         tpe[l][g][k] = T.phi_edge[l][g][k];
         teq[l][g][k] = T.edgeq[l][g][k];
+    }
+    {
+        // this tile's elements and its halo, all loads in flight together
+        const int h0 = a.hoff[tile], nh = a.hoff[tile + 1] - h0;
+        for (int j = tid; j < NT + nh; j += NT) {
+            const int g = j < NT ? tile * NT + j : a.hidx[h0 + j - NT];
+            if (g < ne) {
+#pragma unroll
+                for (int f = 0; f < 3 * DOF; f++) stg[f * ns + j] = a.u[(size_t)f * ne + g];
+                double L0[3][DOF];
+                const int sg = E.sid ? (int)E.sid[g] : 0;
+                load_lambda<DOF>(a, ne, g, E.scen[sg].dtdp, L0);
+#pragma unroll
+                for (int f = 0; f < 3; f++)
+#pragma unroll
+                    for (int k = 0; k < DOF; k++) stg[(RL + f * DOF + k) * ns + j] = L0[f][k];
+                const double d0 = E.dpv[g], d1 = E.dpv[(size_t)ne + g], d2 = E.dpv[2 * (size_t)ne + g];
+                stg[RH * ns + j] = c13 * (d0 + d1 + d2);
+                stg[(RH + 1) * ns + j] = -(c16 * (d0 + d1)) + c13 * d2;
+                stg[(RH + 2) * ns + j] = -(0.5 * d0) + 0.5 * d1;
+                const double X0 = E.xv[g], X1 = E.xv[(size_t)ne + g], X2 = E.xv[2 * (size_t)ne + g];
+                const double Y0 = E.yv[g], Y1 = E.yv[(size_t)ne + g], Y2 = E.yv[2 * (size_t)ne + g];
+                stg[RA * ns + j] = (X0 - X2) * (Y1 - Y2) + (X2 - X1) * (Y0 - Y2);
+            }
+        }
     }
     __syncthreads();
     if (e >= ne) return;
     const int sid = E.sid ? (int)E.sid[e] : 0;
     const Scen sc = E.scen[sid];
     const int row = table_step(a.fo) * a.fo.nrk + a.irk;
-    const double c13 = 1.0 / 3.0, c16 = 1.0 / 6.0;
     const bool quad = QF || ph.iflinbf == 0;
     const bool hybrid = !QF && ph.ifhybf == 1;
 
@@ -208,29 +347,18 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
 #pragma unroll
     for (int f = 0; f < 3; f++)
 #pragma unroll
-        for (int k = 0; k < DOF; k++) U[f][k] = a.u[(size_t)(f * DOF + k) * ne + e];
-    {
-        double L0[3][DOF];
-        load_lambda<DOF>(a, ne, e, sc.dtdp, L0);
-#pragma unroll
-        for (int f = 0; f < 3; f++)
-#pragma unroll
-            for (int k = 0; k < DOF; k++) lm_lds[f * DOF + k][tid] = L0[f][k];
-    }
-#define Lm(f, k) lm_lds[(f) * DOF + (k)][tid]
+        for (int k = 0; k < DOF; k++) U[f][k] = stg[(f * DOF + k) * ns + tid];
+#define Lm(f, k) stg[(RL + (f) * DOF + (k)) * ns + tid]
 #pragma unroll
     for (int j = 0; j < 3 * DOF; j++) ub_lds[j][tid] = 0.0;
     double hbb[3] = {0.0, 0.0, 0.0};
     double AREAS, hb[3];
-    {
-        const double X0 = E.xv[e], X1 = E.xv[(size_t)ne + e], X2 = E.xv[2 * (size_t)ne + e];
-        const double Y0 = E.yv[e], Y1 = E.yv[(size_t)ne + e], Y2 = E.yv[2 * (size_t)ne + e];
-        const double D0 = E.dpv[e], D1 = E.dpv[(size_t)ne + e], D2 = E.dpv[2 * (size_t)ne + e];
-        AREAS = (X0 - X2) * (Y1 - Y2) + (X2 - X1) * (Y0 - Y2);
-        hb[0] = c13 * (D0 + D1 + D2); hb[1] = -(c16 * (D0 + D1)) + c13 * D2; hb[2] = -(0.5 * D0) + 0.5 * D1;
-    }
+    AREAS = stg[RA * ns + tid];
+#pragma unroll
+    for (int m = 0; m < 3; m++) hb[m] = stg[(RH + m) * ns + tid];
     const int info = E.sinfo[e];
     const int nb0 = E.snbr[e], nb1 = E.snbr[(size_t)ne + e], nb2 = E.snbr[2 * (size_t)ne + e];
+    const int sl = a.snbl[e];
 #pragma unroll 1
     for (int slot = 0; slot < 3; slot++) {
         const int bits = side_bits(info, slot);
@@ -244,21 +372,19 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
         const double nx = dy / xlen, ny = -dx / xlen;
         const double tx = -ny, ty = nx;
         if (kind == SIDE_INTERNAL) {
+            const int ln = (sl >> (10 * slot)) & 1023;
             double V[3][DOF], Ln[3][DOF];
 #pragma unroll
             for (int f = 0; f < 3; f++)
 #pragma unroll
-                for (int k = 0; k < DOF; k++) V[f][k] = a.u[(size_t)(f * DOF + k) * ne + nb];
-            const int sidn = E.sid ? (int)E.sid[nb] : 0;
-            load_lambda<DOF>(a, ne, nb, E.scen[sidn].dtdp, Ln);
-            double hbn[3], areas_n;
-            {
-                const double d0 = E.dpv[nb], d1 = E.dpv[(size_t)ne + nb], d2 = E.dpv[2 * (size_t)ne + nb];
-                hbn[0] = c13 * (d0 + d1 + d2); hbn[1] = -(c16 * (d0 + d1)) + c13 * d2; hbn[2] = -(0.5 * d0) + 0.5 * d1;
-                const double X0 = E.xv[nb], X1 = E.xv[(size_t)ne + nb], X2 = E.xv[2 * (size_t)ne + nb];
-                const double Y0 = E.yv[nb], Y1 = E.yv[(size_t)ne + nb], Y2 = E.yv[2 * (size_t)ne + nb];
-                areas_n = (X0 - X2) * (Y1 - Y2) + (X2 - X1) * (Y0 - Y2);
-            }
+                for (int k = 0; k < DOF; k++) {
+                    V[f][k] = stg[(f * DOF + k) * ns + ln];
+                    Ln[f][k] = stg[(RL + f * DOF + k) * ns + ln];
+                }
+            double hbn[3];
+#pragma unroll
+            for (int m = 0; m < 3; m++) hbn[m] = stg[(RH + m) * ns + ln];
+            const double areas_n = stg[RA * ns + ln];
             const double edfac = xlen / AREAS, edfac_n = xlen / areas_n;
 #pragma unroll 1
             for (int I = 0; I < NEGP; I++) {
@@ -288,11 +414,11 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
                     }
                     fb[f] = role ? (so_ - sn_) : (sn_ - so_);
                 }
-                double bar[8];
-                if (role) llf_flux_adj<NL>(ph, zn, zo, xn, xo, yn, yo, hn, ho, sn, so, nx, ny, ph.g, fb, bar);
-                else llf_flux_adj<NL>(ph, zo, zn, xo, xn, yo, yn, ho, hn, so, sn, nx, ny, ph.g, fb, bar);
-                const int s = role ? 1 : 0;   // this element's side of the flux arguments
-                const double zb = bar[0 + s], xb = bar[2 + s], yb = bar[4 + s], hbar = bar[6 + s];
+                // canonical (in, ex) order; this element is side `role`
+                double zb, xb, yb, hbar;
+                llf_flux_adj_side<NL>(ph, role ? zn : zo, role ? zo : zn, role ? xn : xo, role ? xo : xn,
+                                      role ? yn : yo, role ? yo : yn, role ? hn : ho, role ? ho : hn,
+                                      role ? sn : so, role ? so : sn, nx, ny, ph.g, fb, role, zb, xb, yb, hbar);
 #pragma unroll
                 for (int k = 0; k < DOF; k++) {
                     ub_lds[k][tid] += zb * po[k];

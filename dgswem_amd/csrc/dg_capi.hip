@@ -711,6 +711,9 @@ size_t stage_static_lds(int dof, int negp, int nt, bool met) {
     return (size_t)8 * (2 * 3 * dof * nt + (met ? 8 : 1) * nt + 2 * 3 * negp * dof);
 }
 
+// Dynamic LDS bytes of adj_stage_kernel: 6*dof + 4 staged rows of ns doubles.
+size_t adj_dynamic_lds(int dof, int ns) { return (size_t)8 * (6 * dof + 4) * ns; }
+
 // Block size the stage kernel runs with: the chosen one for nonlinear runs, 64 for linearised runs.
 int stage_nt(const Ctx& c, int nt) { return c.prm.nolicat != 0 ? nt : 64; }
 
@@ -973,24 +976,44 @@ int adj_check(Ctx& c) {
 }
 
 template <int P, int NT, bool NL>
-void launch_adj_nl(Ctx& c, const AdjArgs& a) {
+int launch_adj_nl(Ctx& c, const AdjArgs& a0) {
     const int grid = (c.ne + NT - 1) / NT;
+    auto it = c.stg.find(NT);
+    if (it == c.stg.end()) return fail("internal: adjoint staging not built for this block size");
+    AdjArgs a = a0;
+    a.hoff = it->second.hoff.p; a.hidx = it->second.hidx.p; a.snbl = it->second.snbl.p; a.ns = it->second.ns;
+    const size_t lds = adj_dynamic_lds(c.dof, it->second.ns);
     // NOLIBF=1 (quadratic friction, no hybrid factor) with the nonlinear equations: friction fixed
     // at compile time
     const bool qf = NL && c.phys.iflinbf == 0 && c.phys.ifhybf == 0;
     if (c.prm.ics == 2) {
-        if (qf) hipLaunchKernelGGL((adj_stage_kernel<P, NT, true, NL, NL>), dim3(grid), dim3(NT), 0, c.stream, a);
-        else hipLaunchKernelGGL((adj_stage_kernel<P, NT, true, NL, false>), dim3(grid), dim3(NT), 0, c.stream, a);
+        if (qf) hipLaunchKernelGGL((adj_stage_kernel<P, NT, true, NL, NL>), dim3(grid), dim3(NT), lds, c.stream, a);
+        else hipLaunchKernelGGL((adj_stage_kernel<P, NT, true, NL, false>), dim3(grid), dim3(NT), lds, c.stream, a);
     } else {
-        if (qf) hipLaunchKernelGGL((adj_stage_kernel<P, NT, false, NL, NL>), dim3(grid), dim3(NT), 0, c.stream, a);
-        else hipLaunchKernelGGL((adj_stage_kernel<P, NT, false, NL, false>), dim3(grid), dim3(NT), 0, c.stream, a);
+        if (qf) hipLaunchKernelGGL((adj_stage_kernel<P, NT, false, NL, NL>), dim3(grid), dim3(NT), lds, c.stream, a);
+        else hipLaunchKernelGGL((adj_stage_kernel<P, NT, false, NL, false>), dim3(grid), dim3(NT), lds, c.stream, a);
     }
+    return 0;
 }
 
 template <int P, int NT>
-void launch_adj(Ctx& c, const AdjArgs& a) {
-    if (c.prm.nolicat != 0) launch_adj_nl<P, NT, true>(c, a);
-    else launch_adj_nl<P, NT, false>(c, a);
+int launch_adj(Ctx& c, const AdjArgs& a) {
+    if (c.prm.nolicat != 0) return launch_adj_nl<P, NT, true>(c, a);
+    return launch_adj_nl<P, NT, false>(c, a);
+}
+
+// Block size of the adjoint stage kernel.
+int adj_nt(int p) { return p == 1 ? 128 : 64; }
+
+// The adjoint kernel's block-local staging: the forward's halo lists for its block size, with its
+// own row count (state, RHS cotangent, bathymetry modes, AREAS per staged element).
+int ensure_adj_staging(Ctx& c) {
+    const int nt = adj_nt(c.prm.p);
+    if (ensure_staging(c, nt)) return 1;
+    const size_t stat = (size_t)8 * (3 * c.dof * nt + 2 * 3 * c.negp * c.dof);
+    if (adj_dynamic_lds(c.dof, c.stg[nt].ns) + stat > (size_t)c.max_lds)
+        return fail("adjoint staging needs more LDS than a block may use");
+    return 0;
 }
 
 // Forward N steps from the current state recording the gauges after every step (geta[k-1] after
@@ -1076,8 +1099,10 @@ int adj_backward(Ctx& c, int it0, int nsteps, int interval, double weight) {
                 a.out = ub[i];
                 a.gdp = c.gdp.p;
                 a.gnbar = c.phys.load_manning ? c.gnb.p : nullptr;
-                if constexpr (P == 1) launch_adj<P, 128>(c, a);
-                else launch_adj<P, 64>(c, a);
+                int rc;
+                if constexpr (P == 1) rc = launch_adj<P, 128>(c, a);
+                else rc = launch_adj<P, 64>(c, a);
+                if (rc) return rc;
                 HIPCHK(hipGetLastError());
             }
             std::swap(lam, ub[0]);   // cotangent of U^(k-1)
@@ -1448,7 +1473,7 @@ int dg_adj_gradient(int h, int it, int nsteps, const double* data, double weight
     if (!c) return 1;
     if (adj_check(*c)) return 1;
     if (nsteps <= 0) return fail("nsteps must be positive");
-    if (ensure_staging(*c, stage_nt(*c, c->block))) return 1;
+    if (ensure_staging(*c, stage_nt(*c, c->block)) || ensure_adj_staging(*c)) return 1;
     if (interval <= 0) interval = std::max(1, (int)std::lround(std::sqrt((double)nsteps)));
     interval = std::min(interval, nsteps);
     const int nrk = c->rk.nrk, ne = c->ne, np = c->np;
