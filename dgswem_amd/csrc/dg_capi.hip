@@ -144,6 +144,9 @@ struct Ctx {
     std::map<int, hipGraphExec_t> graphs;
     // profiling
     bool prof = false;
+    int prof_every = 1;                          // time the launches of every prof_every-th step
+    long long prof_step = 0;
+    bool prof_this = true;                       // the current step is sampled
     double ktime[K_COUNT] = {};
     long long kcount[K_COUNT] = {};
     std::vector<std::pair<hipEvent_t, hipEvent_t>> evpool;
@@ -190,7 +193,7 @@ DevTables to_dev(const RefTables& t) {
 
 // ---- profiling helpers (no events are recorded into a captured graph)
 int prof_begin(Ctx& c, int kid, bool capturing) {
-    if (!c.prof || capturing) return -1;
+    if (!c.prof || capturing || !c.prof_this) return -1;
     const int idx = (int)c.pending.size();
     if (idx >= (int)c.evpool.size()) {
         hipEvent_t a, b;
@@ -847,6 +850,7 @@ int launch_step(Ctx& c, bool capturing) {
     // stage slots: slot(i) holds U(i+1); slot(0) = current stage-1 state
     int slot[KMAXRK + 1];
     for (int i = 0; i <= nrk; i++) slot[i] = (c.ucur + i) % (nrk + 1);
+    if (c.prof && !capturing) c.prof_this = (c.prof_step++ % c.prof_every) == 0;
     if (c.met) {
         int ev = prof_begin(c, K_MET, capturing);
         hipLaunchKernelGGL(met_kernel, dim3(gn), dim3(B), 0, c.stream, met_args(c),
@@ -889,6 +893,7 @@ int launch_step(Ctx& c, bool capturing) {
     } else {
         c.hstep++;
     }
+    c.prof_this = true;
     c.ucur = slot[nrk];
     return 0;
 }
@@ -1374,7 +1379,10 @@ int dg_gpu_profile(int h, int enable) {
     Ctx* c = get(h);
     if (!c) return 1;
     if (prof_flush(*c)) return 1;
-    c->prof = enable != 0;
+    c->prof = enable > 0;
+    c->prof_every = std::max(1, enable);
+    c->prof_step = 0;
+    c->prof_this = true;
     for (int k = 0; k < K_COUNT; k++) { c->ktime[k] = 0.0; c->kcount[k] = 0; }
     return 0;
 }

@@ -344,15 +344,14 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
                     else if (mode == 3) { fo = r0; gm = role ? 0.0 : r1; hm = role ? 0.0 : r2; }
                     else { gm = r1; hm = r2; }
                 }
+                // the "in" side subtracts w*F and the "ex" side adds it: a - b = a + (-b) exactly,
+                // so one branch-free update serves both roles
                 const double* eq = teq[L][gown];
 #pragma unroll
                 for (int k = 0; k < DOF; k++) {
                     const double w = edfac * eq[k];
-                    if (!role) {
-                        acc[0][k] = acc[0][k] - w * fo; acc[1][k] = acc[1][k] - w * gm; acc[2][k] = acc[2][k] - w * hm;
-                    } else {
-                        acc[0][k] = acc[0][k] + w * fo; acc[1][k] = acc[1][k] + w * gm; acc[2][k] = acc[2][k] + w * hm;
-                    }
+                    const double ws = role ? w : -w;
+                    acc[0][k] = acc[0][k] + ws * fo; acc[1][k] = acc[1][k] + ws * gm; acc[2][k] = acc[2][k] + ws * hm;
                 }
             }
 #pragma unroll

@@ -243,8 +243,9 @@ class _Handle:
         _check(load_library().dg_gpu_mass(self.h, C.byref(m)))
         return m.value
 
-    def profile(self, enable: bool) -> None:
-        _check(load_library().dg_gpu_profile(self.h, 1 if enable else 0))
+    def profile(self, enable: bool, every: int = 1) -> None:
+        """Time kernel launches with HIP events: those of every `every`-th step."""
+        _check(load_library().dg_gpu_profile(self.h, max(1, int(every)) if enable else 0))
 
     def kernel_stats(self, kid: int):
         t = C.c_double(); n = C.c_longlong(); name = C.create_string_buffer(64)

@@ -108,8 +108,10 @@ int dg_gpu_modal2nodal(int handle, double* eta2, double* uu2, double* vv2, int* 
 /* Total water volume sum_e A_e*(ze_1 + hb_1) of the stage-1 state (mode 1 is the cell mean). */
 int dg_gpu_mass(int handle, double* mass);
 
-/* Kernel timing with HIP events on the handle's stream.  With enable=1 every kernel launch of
- * dg_gpu_step is bracketed by events; dg_gpu_kernel_stats returns the summed milliseconds and
+/* Kernel timing with HIP events on the handle's stream.  With enable=n>0 every kernel launch of
+ * every n-th step of dg_gpu_step is bracketed by events (n=1: all steps; a larger n samples a
+ * long run with little event overhead); enable=0 switches timing off.  dg_gpu_kernel_stats
+ * returns the summed milliseconds and
  * launch count of kernel `kernel_id` (0 stage kernel, 1 node min/max, 2 limiter+wet/dry,
  * 3 modal2nodal, 4 NWS=8 node forcing) since the last reset.  No counterpart in the reference
  * (it has no timers). */
