@@ -107,7 +107,7 @@ struct Ctx {
     // mesh
     DevBuf<uint16_t> sid;
     DevBuf<Scen> scen;
-    DevBuf<double> xv, yv, dpv, nbar, cori_el, sfacq, sfaced;
+    DevBuf<double> xv, yv, dpv, nbar, cori_el, sfacq, sfaced, sfs;
     DevBuf<int> sinfo, snbr, nm, n2e_off, n2e;
     std::vector<int> h_sinfo, h_snbr;            // host copies (staging lists)
     bool has_cori_el = false;
@@ -230,6 +230,7 @@ ElemData elem_data(Ctx& c) {
     E.cori_el = c.has_cori_el ? c.cori_el.p : nullptr;
     E.sfacq = c.prm.ics == 2 ? c.sfacq.p : nullptr;
     E.sfaced = c.prm.ics == 2 ? c.sfaced.p : nullptr;
+    E.sfs = c.prm.ics == 2 ? c.sfs.p : nullptr;
     E.sinfo = c.sinfo.p; E.snbr = c.snbr.p; E.nm = c.nm.p;
     return E;
 }
@@ -569,6 +570,26 @@ int upload_mesh(Ctx& c) {
     if (c.prm.ics == 2) {
         HIPCHK(c.sfacq.upload(sfq.data(), sfq.size(), c.stream));
         HIPCHK(c.sfaced.upload(sfe.data(), sfe.size(), c.stream));
+        // slot order, so that each lane reads its own element's column (coalesced) instead of a
+        // row chosen by its local edge number or a neighbour's column
+        std::vector<double> sfs((size_t)2 * 3 * negp * ne, 1.0);
+        for (int e = 0; e < ne; e++)
+            for (int sl = 0; sl < 3; sl++) {
+                const int bits = side_bits(sinfo[e], sl);
+                const int L = bits & 3, role = (bits >> 2) & 1, kind = (bits >> 3) & 3, nled = (bits >> 5) & 3;
+                for (int I = 0; I < negp; I++) {
+                    if (kind == SIDE_INTERNAL) {
+                        const int gown = role ? negp - 1 - I : I, gnb = role ? I : negp - 1 - I;
+                        const int nb = snbr[(size_t)sl * ne + e];
+                        sfs[((size_t)sl * negp + I) * ne + e] = sfe[((size_t)L * negp + gown) * ne + e];
+                        sfs[((size_t)(3 + sl) * negp + I) * ne + e] = sfe[((size_t)nled * negp + gnb) * ne + nb];
+                    } else {
+                        sfs[((size_t)sl * negp + I) * ne + e] = sfe[((size_t)L * negp + I) * ne + e];
+                    }
+                }
+            }
+        HIPCHK(c.sfs.upload(sfs.data(), sfs.size(), c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
     }
     HIPCHK(c.nmm.alloc((size_t)6 * np));
     HIPCHK(c.outbuf.alloc((size_t)3 * np));
@@ -711,7 +732,8 @@ int build_stage_tables(Ctx& c, int it, int n) {
 
 // Static LDS bytes of stage_kernel<P, NT, ..., MET, ...> (its __shared__ arrays).
 size_t stage_static_lds(int dof, int negp, int nt, bool met) {
-    return (size_t)8 * (2 * 3 * dof * nt + (met ? 8 : 1) * nt + 2 * 3 * negp * dof);
+    (void)met;
+    return (size_t)8 * (2 * 3 * dof * nt + 2 * 3 * negp * dof);
 }
 
 // Dynamic LDS bytes of adj_stage_kernel: 6*dof + 4 staged rows of ns doubles.

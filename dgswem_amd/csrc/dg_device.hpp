@@ -70,6 +70,8 @@ struct ElemData {
     const double* __restrict__ cori_el; // [ne] or nullptr
     const double* __restrict__ sfacq;   // [nagp][ne] spherical factor at area points, or nullptr
     const double* __restrict__ sfaced;  // [3][negp][ne] spherical factor at edge points, or nullptr
+    const double* __restrict__ sfs;     // [2][3 slots][negp][ne] the same in slot order: [0] own side
+                                        // at the edge's point I, [1] the internal neighbour's side
     const int* __restrict__ sinfo;      // [ne] packed side descriptors
     const int* __restrict__ snbr;       // [3][ne] neighbour element / boundary-list index per slot
     const int* __restrict__ nm;         // [3][ne] 0-based vertex nodes

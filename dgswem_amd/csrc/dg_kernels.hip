@@ -193,8 +193,6 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
     // RHS (accumulated first, like the reference's boundary edge routines), [1] internal-edge
     // slot sum (add_internal_flux)
     __shared__ double acc_lds[2][3 * DOF][NT];
-    // wind stress at the element's vertices and the pressure gradient sums (MET)
-    __shared__ double met_lds[MET ? 8 : 1][NT];
     // edge basis and edge quadrature tables: indexed by each lane's own local edge number, so
     // read from LDS (short latency) rather than as divergent constant-memory loads
     __shared__ double tpe[3][NEGP][DOF], teq[3][NEGP][DOF];
@@ -302,8 +300,8 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
                 const double ho = trace<3>(hb, po), hn = trace<3>(hbn, pn);
                 double so = 1.0, sn = 1.0;
                 if (SPH) {
-                    so = E.sfaced[((size_t)L * NEGP + gown) * ne + e];
-                    sn = E.sfaced[((size_t)nled * NEGP + gnb) * ne + nb];
+                    so = E.sfs[((size_t)slot * NEGP + I) * ne + e];
+                    sn = E.sfs[((size_t)(3 + slot) * NEGP + I) * ne + e];
                 }
                 const double ze_in = role ? zn : zo, qx_in = role ? xn : xo, qy_in = role ? yn : yo;
                 const double ze_ex = role ? zo : zn, qx_ex = role ? xo : xn, qy_ex = role ? yo : yn;
@@ -365,7 +363,7 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
                 const double* pv = tpe[L][I];
                 const double ze_in = trace<DOF>(U[0], pv), qx_in = trace<DOF>(U[1], pv), qy_in = trace<DOF>(U[2], pv);
                 const double hb_in = trace<3>(hb, pv);
-                const double sf = SPH ? E.sfaced[((size_t)L * NEGP + I) * ne + e] : 1.0;
+                const double sf = SPH ? E.sfs[((size_t)slot * NEGP + I) * ne + e] : 1.0;
                 double f0, f1, f2;
                 if (kind == SIDE_LAND) {
                     const double qn_int = qx_in * nx + qy_in * ny, qt_int = qx_in * tx + qy_in * ty;
@@ -443,6 +441,8 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
             drdy = 1.0 / AREA * (X0 - X2);
             dsdy = 1.0 / AREA * (X1 - X0);
         }
+        // wind stress at the element's vertices and the pressure gradient sums (MET)
+        double mw[8];
         if (MET) {
             // nodal WSX2/WSY2/PR2 of this stage (src/met_forcing.F:507-509) and the
             // QP-independent sums PR2(Ni)*DPSIDX(i) of src/rhs_dg_hydro.F:117-122,366-372
@@ -453,16 +453,16 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
             for (int v = 0; v < 3; v++) {
                 const int n = E.nm[(size_t)v * ne + e];
                 const double cd = a.mt.cd[n], wm = a.mt.wm[n];
-                met_lds[v][tid] = ramp * 0.001293 * cd * a.mt.wx[n] * wm;
-                met_lds[3 + v][tid] = ramp * 0.001293 * cd * a.mt.wy[n] * wm;
+                mw[v] = ramp * 0.001293 * cd * a.mt.wx[n] * wm;
+                mw[3 + v] = ramp * 0.001293 * cd * a.mt.wy[n] * wm;
                 const double pr = ramp * a.mt.prn[n];
                 const double dpsidx = drpsi[v] * drdx + dspsi[v] * dsdx;
                 const double dpsidy = drpsi[v] * drdy + dspsi[v] * dsdy;
                 if (v == 0) { pdx = pr * dpsidx; pdy = pr * dpsidy; }
                 else { pdx = pdx + pr * dpsidx; pdy = pdy + pr * dpsidy; }
             }
-            met_lds[6][tid] = pdx;
-            met_lds[7][tid] = pdy;
+            mw[6] = pdx;
+            mw[7] = pdy;
         }
         // hybrid friction factor (1 + (HBREAK/H_c)^FTHETA)^(FGAMMA/FTHETA) from the element-mean
         // depth (src/rhs_dg_hydro.F:306-309): the same at every area point, evaluated once and only
@@ -526,11 +526,11 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
             if (MET) {
                 // wind stress and atmospheric pressure (src/rhs_dg_hydro.F:362-373)
                 const double* ps = T.psi[I];
-                const double wsx = met_lds[0][tid] * ps[0] + met_lds[1][tid] * ps[1] + met_lds[2][tid] * ps[2];
-                const double wsy = met_lds[3][tid] * ps[0] + met_lds[4][tid] * ps[1] + met_lds[5][tid] * ps[2];
+                const double wsx = mw[0] * ps[0] + mw[1] * ps[1] + mw[2] * ps[2];
+                const double wsy = mw[3] * ps[0] + mw[4] * ps[1] + mw[5] * ps[2];
                 const double fw = 1.0 / f1;
-                sx = sx + (NL ? wsx : fw * wsx) - ph.g * sfq * depth * met_lds[6][tid];
-                sy = sy + (NL ? wsy : fw * wsy) - ph.g * depth * met_lds[7][tid];
+                sx = sx + (NL ? wsx : fw * wsx) - ph.g * sfq * depth * mw[6];
+                sy = sy + (NL ? wsy : fw * wsy) - ph.g * depth * mw[7];
             }
             R[1][0] = R[1][0] + T.srfac[I][0] * sx;
             R[2][0] = R[2][0] + T.srfac[I][0] * sy;
