@@ -606,11 +606,24 @@ __global__ void __launch_bounds__(256) node_minmax_kernel(int np, int ne, const 
     const int n = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (n >= np) return;
     double zmin = 99999., zmax = -99999., xmin = 99999., xmax = -99999., ymin = 99999., ymax = -99999.;
-    for (int j = off[n]; j < off[n + 1]; j++) {
-        const int e = n2e[j] >> 2;
-        const double z = u[e], x = u[(size_t)DOF * ne + e], y = u[(size_t)2 * DOF * ne + e];
-        zmin = fmin(zmin, z); xmin = fmin(xmin, x); ymin = fmin(ymin, y);
-        zmax = fmax(zmax, z); xmax = fmax(xmax, x); ymax = fmax(ymax, y);
+    const int j1 = off[n + 1];
+    int j = off[n];
+    // two incidences per iteration, so that their loads are in flight together (min and max are
+    // exact: the order does not matter)
+    for (; j + 1 < j1; j += 2) {
+        const int e0 = n2e[j] >> 2, e1 = n2e[j + 1] >> 2;
+        const double z0 = u[e0], x0 = u[(size_t)DOF * ne + e0], y0 = u[(size_t)2 * DOF * ne + e0];
+        const double z1 = u[e1], x1 = u[(size_t)DOF * ne + e1], y1 = u[(size_t)2 * DOF * ne + e1];
+        zmin = fmin(zmin, z0); xmin = fmin(xmin, x0); ymin = fmin(ymin, y0);
+        zmax = fmax(zmax, z0); xmax = fmax(xmax, x0); ymax = fmax(ymax, y0);
+        zmin = fmin(zmin, z1); xmin = fmin(xmin, x1); ymin = fmin(ymin, y1);
+        zmax = fmax(zmax, z1); xmax = fmax(xmax, x1); ymax = fmax(ymax, y1);
+    }
+    if (j < j1) {
+        const int e0 = n2e[j] >> 2;
+        const double z0 = u[e0], x0 = u[(size_t)DOF * ne + e0], y0 = u[(size_t)2 * DOF * ne + e0];
+        zmin = fmin(zmin, z0); xmin = fmin(xmin, x0); ymin = fmin(ymin, y0);
+        zmax = fmax(zmax, z0); xmax = fmax(xmax, x0); ymax = fmax(ymax, y0);
     }
     mm[n] = zmin; mm[(size_t)np + n] = zmax; mm[2 * (size_t)np + n] = xmin;
     mm[3 * (size_t)np + n] = xmax; mm[4 * (size_t)np + n] = ymin; mm[5 * (size_t)np + n] = ymax;
@@ -634,6 +647,9 @@ __global__ void __launch_bounds__(256) limit_wetdry_kernel(int np, ElemData E, d
     for (int f = 0; f < 3; f++)
 #pragma unroll
         for (int k = 0; k < DOF; k++) U[f][k] = u[(size_t)(f * DOF + k) * ne + e];
+    // the limiter keeps the cell means; wetting and drying changes them only for some elements,
+    // so a cell mean is written back only when its bits changed
+    const double c0in[3] = {U[0][0], U[1][0], U[2][0]};
     if (LIMIT) {
         const int n0 = E.nm[e], n1 = E.nm[(size_t)ne + e], n2 = E.nm[2 * (size_t)ne + e];
 #pragma unroll
@@ -686,9 +702,12 @@ __global__ void __launch_bounds__(256) limit_wetdry_kernel(int np, ElemData E, d
         wd_out[e] = (uint8_t)wetdry_element<P>(T, U, D, hb, dpe_min, wd_in[e], E.scen[sid].h0);
     }
 #pragma unroll
-    for (int f = 0; f < 3; f++)
+    for (int f = 0; f < 3; f++) {
+        if (WETDRY && __double_as_longlong(U[f][0]) != __double_as_longlong(c0in[f]))
+            u[(size_t)(f * DOF) * ne + e] = U[f][0];
 #pragma unroll
-        for (int k = (WETDRY ? 0 : 1); k < DOF; k++) u[(size_t)(f * DOF + k) * ne + e] = U[f][k];
+        for (int k = 1; k < DOF; k++) u[(size_t)(f * DOF + k) * ne + e] = U[f][k];
+    }
 }
 
 // ------------------------------------------------------------------------------------------
