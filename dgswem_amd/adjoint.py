@@ -98,6 +98,38 @@ def twin_problem(case: Case, nsteps: int, ng: int = 50, seed: int = 0, perturb: 
     return p
 
 
+def rest_state(case: Case):
+    """All-wet state at rest (zero elevation and flow)."""
+    z = np.zeros((case.ne, case.dofh))
+    return z, z.copy(), z.copy(), np.ones(case.ne, np.int32)
+
+
+def ensemble_problem(cases, nsteps: int, ng: int = 20, seed: int = 0, perturb: float = 0.2,
+                     state0: str = "rest"):
+    """Batched twin problems: each member gets its own gauges and its own perturbed-Manning twin
+    data (as in twin_problem, seeds seed+i), all with the same loss weight 1/(nsteps*ng).  Returns
+    the ensemble's MisfitProblem (gauges and data concatenated in member order) and the members'
+    own problems; the gradient of the summed loss restricted to a member's nodes is that member's
+    gradient."""
+    w = 1.0 / (nsteps * ng)
+    probs = []
+    for i, c in enumerate(cases):
+        st = rest_state(c) if state0 == "rest" else hump_state(c)
+        probs.append(twin_problem(c, nsteps=nsteps, ng=ng, seed=seed + i, perturb=perturb, weight=w,
+                                  state0=st))
+    ens = gpu.Ensemble([p.solver for p in probs])
+    elems, rs, ss = [], [], []
+    for i, c in enumerate(cases):
+        el, r, s_ = seeded_gauges(c, ng, seed + i)
+        elems.append(el + ens.eoff[i]); rs.append(r); ss.append(s_)
+    ens.set_gauges(np.concatenate(elems).astype(np.int32), np.concatenate(rs), np.concatenate(ss))
+    data = np.concatenate([p.data for p in probs], axis=1)
+    st0 = tuple(np.concatenate([p.state0[j] for p in probs]) for j in range(4))
+    eprob = MisfitProblem(solver=ens, nsteps=nsteps, data=data, weight=w, state0=st0, it0=0)
+    eprob.reset()
+    return eprob, probs
+
+
 def _make_function():
     import torch
 

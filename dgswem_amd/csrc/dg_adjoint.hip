@@ -249,6 +249,8 @@ struct AdjArgs {
     ElemData ed;
     Phys ph;
     Forcing fo;
+    Met mt;                              // NWS=8 node forcing of the step (when met != 0)
+    int met;
     const double* __restrict__ u;        // stage state U(IRK) [3][dof][ne]
     int irk;                             // stage index
     int nsrc;                            // later-stage cotangents entering this stage
@@ -535,6 +537,23 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
         const double cori = E.cori_el ? E.cori_el[e] : sc.cori;
         const double nbar = ph.load_manning ? E.nbar[e] : 0.0;
         double depth_cb = 0.0;
+        // NWS=8: the stage's vertex wind stresses and the pressure-gradient sums, as the forward
+        // forms them; the pressure term -g*SFAC*H*dP/dx depends on the state through H
+        double mw[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, pdx = 0.0, pdy = 0.0;
+        if (a.met) {
+            const double ramp = a.fo.ramp[(size_t)sid * a.fo.rows + row];
+            const double drpsi[3] = {-1.0 / 2.0, 1.0 / 2.0, 0.0}, dspsi[3] = {-1.0 / 2.0, 0.0, 1.0 / 2.0};
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                const int n = E.nm[(size_t)v * ne + e];
+                const double cd = a.mt.cd[n], wm = a.mt.wm[n];
+                mw[v] = ramp * 0.001293 * cd * a.mt.wx[n] * wm;
+                mw[3 + v] = ramp * 0.001293 * cd * a.mt.wy[n] * wm;
+                const double pr = ramp * a.mt.prn[n];
+                pdx += pr * (drpsi[v] * drdx + dspsi[v] * dsdx);
+                pdy += pr * (drpsi[v] * drdy + dspsi[v] * dsdy);
+            }
+        }
 #pragma unroll 1
         for (int I = 0; I < NAGP; I++) {
             const double* pa = T.phi_area[I];
@@ -598,6 +617,18 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
             double zeb = 0.0, qxb = 0.0, qyb = 0.0, hbqb = 0.0, dhbxb = 0.0, dhbyb = 0.0, depthb = 0.0;
             // sx = -(tau*qx) + fg*dhbx*sfq + cori*qy ; sy = -(tau*qy) + fg*dhby - cori*qx
             double taub = -qx * sxb - qy * syb;
+            double f1b_met = 0.0;
+            if (a.met) {
+                // sx += wsx/F1 - g*SFAC*H*pdx ; sy += wsy/F1 - g*H*pdy (F1 = 1 for the nonlinear equations)
+                depthb += -ph.g * sfq * pdx * sxb - ph.g * pdy * syb;
+                if constexpr (!NL) {
+                    const double* ps = T.psi[I];
+                    const double wsx = mw[0] * ps[0] + mw[1] * ps[1] + mw[2] * ps[2];
+                    const double wsy = mw[3] * ps[0] + mw[4] * ps[1] + mw[5] * ps[2];
+                    const double fw = 1.0 / f1;
+                    f1b_met = -fw * fw * (wsx * sxb + wsy * syb);
+                }
+            }
             qxb += -tau * sxb - cori * syb;
             qyb += cori * sxb - tau * syb;
             double fgb = dhbx * sfq * sxb + dhby * syb;
@@ -654,7 +685,7 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
             qyb += ph.nleq * fvb;
             qxb += f1 * sfq * fxb;
             qyb += f1 * fyb;
-            const double f1b = qx * sfq * fxb + qy * fyb;
+            const double f1b = qx * sfq * fxb + qy * fyb + f1b_met;
             hbqb += ph.leq * f1b;
             zeb += depthb;
             hbqb += depthb;

@@ -802,6 +802,11 @@ int launch_stage_v(Ctx& c, const StageArgs& a0, bool sph, bool fuse) {
     StageArgs a = a0;
     a.hoff = it->second.hoff.p; a.hidx = it->second.hidx.p; a.snbl = it->second.snbl.p; a.ns = it->second.ns;
     const size_t lds = it->second.lds;
+    // every pointer the variant dereferences must be set (a missing one is an error, not a launch)
+    if (MET && (!a.mt.row || !a.mt.cd || !a.mt.wm || !a.mt.wx || !a.mt.wy || !a.mt.prn || !a.fo.ramp))
+        return fail("internal: NWS=8 stage launch without its forcing arguments");
+    if (!a.uin || !a.uout || !a.wd_in || !a.hoff || !a.hidx || !a.snbl)
+        return fail("internal: stage launch with a missing buffer");
     if (sph) {
         if (fuse) hipLaunchKernelGGL((stage_kernel<P, NT, true, true, MET, NL>), dim3(grid), dim3(NT), lds, c.stream, a);
         else hipLaunchKernelGGL((stage_kernel<P, NT, true, false, MET, NL>), dim3(grid), dim3(NT), lds, c.stream, a);
@@ -934,9 +939,14 @@ template <int P>
 int launch_step_explicit(Ctx& c, double* const* st) {
     const int nrk = c.rk.nrk;
     const bool sph = c.prm.ics == 2;
+    if (c.met) {
+        hipLaunchKernelGGL(met_kernel, dim3((c.np + 255) / 256), dim3(256), 0, c.stream, met_args(c), nullptr, c.hstep);
+        HIPCHK(hipGetLastError());
+    }
     for (int irk = 0; irk < nrk; irk++) {
         StageArgs a{};
         a.ed = elem_data(c); a.ph = c.phys; a.fo = forcing(c);
+        if (c.met) a.mt = met_args(c);
         a.uin = st[irk];
         a.uout = st[irk + 1];
         a.wd_in = c.wd[c.wdcur].p;
@@ -994,10 +1004,8 @@ int run_chunk(Ctx& c, int n) {
 
 // ---- discrete adjoint driver ------------------------------------------------------------
 int adj_check(Ctx& c) {
-    if (c.S != 1) return fail("the adjoint runs on single runs, not ensembles");
     if (c.prm.slopeflag != 0) return fail("the adjoint differentiates runs without the slope limiter");
     if (c.prm.nolifa >= 2) return fail("the adjoint differentiates runs without wetting and drying");
-    if (c.met) return fail("the adjoint does not differentiate runs with NWS=8 forcing");
     if (c.ng == 0) return fail("no gauges set (dg_adj_set_gauges)");
     return 0;
 }
@@ -1010,6 +1018,9 @@ int launch_adj_nl(Ctx& c, const AdjArgs& a0) {
     AdjArgs a = a0;
     a.hoff = it->second.hoff.p; a.hidx = it->second.hidx.p; a.snbl = it->second.snbl.p; a.ns = it->second.ns;
     const size_t lds = adj_dynamic_lds(c.dof, it->second.ns);
+    if (a.met && (!a.mt.cd || !a.mt.wm || !a.mt.wx || !a.mt.wy || !a.mt.prn || !a.fo.ramp))
+        return fail("internal: NWS=8 adjoint launch without its forcing arguments");
+    if (!a.u || !a.out) return fail("internal: adjoint launch with a missing buffer");
     // NOLIBF=1 (quadratic friction, no hybrid factor) with the nonlinear equations: friction fixed
     // at compile time
     const bool qf = NL && c.phys.iflinbf == 0 && c.phys.ifhybf == 0;
@@ -1110,10 +1121,16 @@ int adj_backward(Ctx& c, int it0, int nsteps, int interval, double weight) {
         }
         for (int k = k1; k > k0; k--) {
             c.hstep = k - k0 - 1;
+            if (c.met) {   // the step's node forcing for the adjoint stages
+                hipLaunchKernelGGL(met_kernel, dim3((c.np + 255) / 256), dim3(256), 0, c.stream, met_args(c), nullptr,
+                                   c.hstep);
+                HIPCHK(hipGetLastError());
+            }
             ub[nrk] = lam;
             for (int i = nrk - 1; i >= 0; i--) {
                 AdjArgs a{};
                 a.ed = elem_data(c); a.ph = c.phys; a.fo = forcing(c);
+                if (c.met) { a.mt = met_args(c); a.met = 1; }
                 a.u = stage(k, i);
                 a.irk = i;
                 a.nsrc = nrk - i;
@@ -1423,16 +1440,21 @@ int dg_gpu_kernel_stats(int h, int kid, double* total_ms, long long* launches, c
 int dg_gpu_set_manning(int h, const double* manning) {
     Ctx* c = get(h);
     if (!c) return 1;
-    if (c->S != 1) return fail("parameters of an ensemble are set on its members");
     if (!c->phys.load_manning) return fail("the run was set up without Manning's n");
-    Member& m = c->mem[0];
-    const int ne = m.ne;
-    for (int e = 0; e < ne; e++) {
-        const int n0 = m.nm[e], n1 = m.nm[(size_t)ne + e], n2 = m.nm[2 * (size_t)ne + e];
-        m.nbar[e] = (manning[n0] + manning[n1] + manning[n2]) / 3.0;
+    // per node of the run (of all members, in member order, for an ensemble)
+    std::vector<double> nb((size_t)c->ne);
+    for (int s = 0; s < c->S; s++) {
+        Member& m = c->mem[s];
+        const int ne = m.ne;
+        const double* mn = manning + c->noff[s];
+        for (int e = 0; e < ne; e++) {
+            const int n0 = m.nm[e], n1 = m.nm[(size_t)ne + e], n2 = m.nm[2 * (size_t)ne + e];
+            m.nbar[e] = (mn[n0] + mn[n1] + mn[n2]) / 3.0;
+            nb[(size_t)c->eoff[s] + e] = m.nbar[e];
+        }
     }
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(c->nbar.upload(m.nbar.data(), ne, c->stream));
+    HIPCHK(c->nbar.upload(nb.data(), nb.size(), c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1440,13 +1462,21 @@ int dg_gpu_set_manning(int h, const double* manning) {
 int dg_gpu_set_depth(int h, const double* dp) {
     Ctx* c = get(h);
     if (!c) return 1;
-    if (c->S != 1) return fail("parameters of an ensemble are set on its members");
-    Member& m = c->mem[0];
-    const int ne = m.ne;
-    for (int v = 0; v < 3; v++)
-        for (int e = 0; e < ne; e++) m.dpv[(size_t)v * ne + e] = dp[m.nm[(size_t)v * ne + e]];
+    // per node of the run (of all members, in member order, for an ensemble)
+    const size_t NE = (size_t)c->ne;
+    std::vector<double> dv(3 * NE);
+    for (int s = 0; s < c->S; s++) {
+        Member& m = c->mem[s];
+        const int ne = m.ne;
+        const double* md = dp + c->noff[s];
+        for (int v = 0; v < 3; v++)
+            for (int e = 0; e < ne; e++) {
+                m.dpv[(size_t)v * ne + e] = md[m.nm[(size_t)v * ne + e]];
+                dv[(size_t)v * NE + c->eoff[s] + e] = m.dpv[(size_t)v * ne + e];
+            }
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(c->dpv.upload(m.dpv.data(), m.dpv.size(), c->stream));
+    HIPCHK(c->dpv.upload(dv.data(), dv.size(), c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }

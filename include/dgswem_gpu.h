@@ -142,13 +142,14 @@ int dg_holland_storm_at(int nrec, const int* rec, const int* ref, int storm_numb
 /* ---- Parameters and the discrete adjoint.  None of these has a counterpart in the reference. */
 
 /* Replace the per-node Manning n (ManningsN; element means rebuilt as at set-up) or the per-node
- * depth DP.  Single runs only. */
+ * depth DP: NP values, for an ensemble the members' nodes in member order (node_offsets). */
 int dg_gpu_set_manning(int handle, const double* manning);
 int dg_gpu_set_depth(int handle, const double* dp);
 
 /* Gauges for the misfit loss: ng points at reference coordinates (r, s) of 1-based elements
  * (the reference triangle is (-1,-1), (1,-1), (-1,1)); the gauge elevation is the element's
- * modal expansion of ZE there. */
+ * modal expansion of ZE there.  For an ensemble the elements are numbered across the members
+ * (elem_offsets), so each member's gauges sit in its own elements. */
 int dg_adj_set_gauges(int handle, int ng, const int* elem, const double* r, const double* s);
 
 /* Advance nsteps steps (IT = it .. it+nsteps-1) and write eta[k][g], the gauge elevations after
@@ -159,8 +160,11 @@ int dg_adj_gauges(int handle, int it, int nsteps, double* eta);
  * state, and its gradient with respect to the per-node Manning n and depth DP, by the discrete
  * adjoint of the stage RHS and the SSP-RK stages.  States are checkpointed every `interval` steps
  * (0: round(sqrt(nsteps))) and each segment is recomputed once on the way back.  Runs without
- * slope limiter, wetting/drying and NWS forcing; single runs.  The handle's state is left as it
- * was.  grad_manning / grad_dp: NP values each (either may be NULL). */
+ * slope limiter and wetting/drying; NWS=8 forcing and ensembles are supported.  For an ensemble
+ * the members are independent, so the gradient of the summed loss restricted to a member's nodes
+ * is that member's own gradient (all members in one batched forward and one batched reverse
+ * sweep).  The handle's state is left as it was.  grad_manning / grad_dp: NP values each (either
+ * may be NULL). */
 int dg_adj_gradient(int handle, int it, int nsteps, const double* data, double weight,
                     int interval, double* loss, double* grad_manning, double* grad_dp);
 

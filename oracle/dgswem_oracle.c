@@ -260,6 +260,12 @@ static void ortho_basis(double X, double Y, int d, double *basis, double *dbx, d
         }
 }
 
+/* The modal basis at reference coordinates (r, s) (gauge points of the adjoint tests). */
+void oracle_basis_at(int p, double r, double s, double *phi) {
+    double dbx[6], dby[6];
+    ortho_basis(r, s, p, phi, dbx, dby);
+}
+
 /* Symmetric triangle rule: multiplicity-3 orbits (src/quadrature_v1.F:245-275,804-862). */
 static void tri_rule(int p, double *px, double *py, double *w) {
     double A[2], W[2]; int L;

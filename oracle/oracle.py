@@ -57,6 +57,7 @@ def lib():
         L.oracle_met_error.restype = C.c_int
         L.oracle_get_met.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.oracle_storm_at.argtypes = [C.c_void_p, C.c_double, _dp]
+        L.oracle_basis_at.argtypes = [C.c_int, C.c_double, C.c_double, _dp]
         L.oracle_storm_at.restype = C.c_int
         _LIB = L
     return _LIB
@@ -207,3 +208,10 @@ class Oracle:
 
     def mass(self) -> float:
         return lib().oracle_mass(self.h)
+
+
+def basis_at(p: int, r: float, s: float) -> np.ndarray:
+    """The modal basis (p=1: 3 modes, p=2: 6) at reference coordinates (r, s)."""
+    out = np.zeros(6)
+    lib().oracle_basis_at(int(p), float(r), float(s), out)
+    return out[:(p + 1) * (p + 2) // 2]
