@@ -132,6 +132,9 @@ struct Ctx {
     DevBuf<double> ramp, mrow, mxc, myc, mcosy, mcor, mwx, mwy, mwm, mcd, mprn;
     DevBuf<uint16_t> nsid;
     int next_it = 1;                             // IT of the next step (for the autotuner's tables)
+    // set while the adjoint's forward sweeps run: WETDRY raises it when it changes an element
+    int* wd_active = nullptr;
+    DevBuf<int> wdact;
     // launch configuration
     int block = 256;
     // adjoint: gauges, checkpoints, segment stage states, cotangents, parameter accumulators
@@ -139,6 +142,10 @@ struct Ctx {
     DevBuf<int> gel, goff, gord;
     DevBuf<double> gphi, geta, gdata, ckpt, seg, lam, gdp, gnb, pout;
     DevBuf<double> ubar[KMAXRK];
+    // frozen-limiter reverse: pre-limiter stage outputs of a segment, vertex-bound cotangents,
+    // node sums and bound-attaining elements
+    DevBuf<double> wseg, lnbc, lgsum;
+    DevBuf<int> larg;
     // graphs of one slot-rotation period, keyed by (ucur, wdcur) at capture
     bool use_graph = false;
     std::map<int, hipGraphExec_t> graphs;
@@ -862,16 +869,43 @@ StageArgs stage_args(Ctx& c, int irk, const int slot[], bool captured = false) {
     a.bcur = c.rk.b[irk][irk];
     a.rout = c.keep_r[irk] ? c.r[irk].p : nullptr;
     a.irk = irk;
+    a.wd_active = c.wd_active;
     return a;
+}
+
+// SLOPELIMITER5 (+ WETDRY) on the stage output u: node min/max, then the element pass.
+template <int P>
+int launch_limiter(Ctx& c, double* u, const ElemData& ed, bool capturing) {
+    const int ne = c.ne, np = c.np, B = 256;
+    const int ge = (ne + B - 1) / B, gn = (np + B - 1) / B;
+    int ev = prof_begin(c, K_MINMAX, capturing);
+    hipLaunchKernelGGL((node_minmax_kernel<P>), dim3(gn), dim3(B), 0, c.stream, np, ne, (const int*)c.n2e_off.p,
+                       (const int*)c.n2e.p, (const double*)u, c.nmm.p);
+    prof_end(c, ev);
+    HIPCHK(hipGetLastError());
+    ev = prof_begin(c, K_LIMIT, capturing);
+    if (c.prm.nolifa >= 2) {
+        hipLaunchKernelGGL((limit_wetdry_kernel<P, true, true>), dim3(ge), dim3(B), 0, c.stream, np, ed, u,
+                           (const double*)c.nmm.p, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p,
+                           c.wd_active);
+        c.wdcur = 1 - c.wdcur;
+    } else {
+        hipLaunchKernelGGL((limit_wetdry_kernel<P, true, false>), dim3(ge), dim3(B), 0, c.stream, np, ed, u,
+                           (const double*)c.nmm.p, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p,
+                           (int*)nullptr);
+    }
+    prof_end(c, ev);
+    HIPCHK(hipGetLastError());
+    return 0;
 }
 
 // One time step: NRK stages (stage kernel, then node min/max + limiter/wet-dry when the slope
 // limiter is on) and the device step-counter increment.
 template <int P>
 int launch_step(Ctx& c, bool capturing) {
-    const int ne = c.ne, np = c.np;
+    const int np = c.np;
     const int B = 256;
-    const int ge = (ne + B - 1) / B, gn = (np + B - 1) / B;
+    const int gn = (np + B - 1) / B;
     const bool wdon = c.prm.nolifa >= 2, lim = c.prm.slopeflag != 0, sph = c.prm.ics == 2;
     const int nrk = c.rk.nrk;
     // stage slots: slot(i) holds U(i+1); slot(0) = current stage-1 state
@@ -895,24 +929,7 @@ int launch_step(Ctx& c, bool capturing) {
         prof_end(c, ev);
         HIPCHK(hipGetLastError());
         if (fuse) c.wdcur = 1 - c.wdcur;
-        if (lim) {
-            ev = prof_begin(c, K_MINMAX, capturing);
-            hipLaunchKernelGGL((node_minmax_kernel<P>), dim3(gn), dim3(B), 0, c.stream, np, ne,
-                               (const int*)c.n2e_off.p, (const int*)c.n2e.p, (const double*)a.uout, c.nmm.p);
-            prof_end(c, ev);
-            HIPCHK(hipGetLastError());
-            ev = prof_begin(c, K_LIMIT, capturing);
-            if (wdon) {
-                hipLaunchKernelGGL((limit_wetdry_kernel<P, true, true>), dim3(ge), dim3(B), 0, c.stream, np, a.ed,
-                                   a.uout, (const double*)c.nmm.p, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p);
-                c.wdcur = 1 - c.wdcur;
-            } else {
-                hipLaunchKernelGGL((limit_wetdry_kernel<P, true, false>), dim3(ge), dim3(B), 0, c.stream, np, a.ed,
-                                   a.uout, (const double*)c.nmm.p, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p);
-            }
-            prof_end(c, ev);
-            HIPCHK(hipGetLastError());
-        }
+        if (lim && launch_limiter<P>(c, a.uout, a.ed, capturing)) return 1;
     }
     if (capturing) {
         hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(1), 0, c.stream, c.step.p, 0, 0);
@@ -934,11 +951,14 @@ int launch_steps(Ctx& c, int n, bool capturing) {
 }
 
 // One step with explicit state pointers: st[0] input, st[1..NRK-1] the intermediate stage states,
-// st[NRK] output (no limiter, no wetting and drying: the adjoint's configurations).
+// st[NRK] output.  With the slope limiter on, each stage's output before limiting is also copied to
+// wsave[irk] (when wsave is given) for the adjoint's frozen-limiter reverse.
 template <int P>
-int launch_step_explicit(Ctx& c, double* const* st) {
+int launch_step_explicit(Ctx& c, double* const* st, double* const* wsave = nullptr) {
     const int nrk = c.rk.nrk;
     const bool sph = c.prm.ics == 2;
+    const bool wdon = c.prm.nolifa >= 2, lim = c.prm.slopeflag != 0, fuse = wdon && !lim;
+    const size_t ssz = (size_t)3 * c.dof * c.ne;
     if (c.met) {
         hipLaunchKernelGGL(met_kernel, dim3((c.np + 255) / 256), dim3(256), 0, c.stream, met_args(c), nullptr, c.hstep);
         HIPCHK(hipGetLastError());
@@ -960,8 +980,16 @@ int launch_step_explicit(Ctx& c, double* const* st) {
         a.bcur = c.rk.b[irk][irk];
         a.rout = c.keep_r[irk] ? c.r[irk].p : nullptr;
         a.irk = irk;
-        if (launch_stage_nt<P>(c, a, sph, false, c.block)) return 1;
+        a.wd_active = c.wd_active;
+        if (launch_stage_nt<P>(c, a, sph, fuse, c.block)) return 1;
         HIPCHK(hipGetLastError());
+        if (fuse) c.wdcur = 1 - c.wdcur;
+        if (lim) {
+            if (wsave)
+                HIPCHK(hipMemcpyAsync(wsave[irk], st[irk + 1], ssz * sizeof(double), hipMemcpyDeviceToDevice,
+                                      c.stream));
+            if (launch_limiter<P>(c, st[irk + 1], a.ed, false)) return 1;
+        }
     }
     c.hstep++;
     return 0;
@@ -1004,8 +1032,6 @@ int run_chunk(Ctx& c, int n) {
 
 // ---- discrete adjoint driver ------------------------------------------------------------
 int adj_check(Ctx& c) {
-    if (c.prm.slopeflag != 0) return fail("the adjoint differentiates runs without the slope limiter");
-    if (c.prm.nolifa >= 2) return fail("the adjoint differentiates runs without wetting and drying");
     if (c.ng == 0) return fail("no gauges set (dg_adj_set_gauges)");
     return 0;
 }
@@ -1095,6 +1121,24 @@ int adj_backward(Ctx& c, int it0, int nsteps, int interval, double weight) {
                            (const double*)(c.gdata.p + (size_t)(k - 1) * c.ng), weight, l);
     };
     seed(nsteps, c.lam.p);
+    // frozen-limiter reverse of one stage: the cotangent `cot` of the limited stage output becomes
+    // the cotangent of the pre-limiter output Wpre (node bounds from Wpre's cell means)
+    const bool lim = c.prm.slopeflag != 0;
+    const int np = c.np, B = 256, ge = (ne + B - 1) / B, gn = (np + B - 1) / B;
+    const ElemData ed = elem_data(c);
+    auto limrev = [&](double* cot, const double* wpre) -> int {
+        hipLaunchKernelGGL((node_minmax_kernel<P>), dim3(gn), dim3(B), 0, c.stream, np, ne, (const int*)c.n2e_off.p,
+                           (const int*)c.n2e.p, wpre, c.nmm.p);
+        hipLaunchKernelGGL((limiter_adj_kernel<P>), dim3(ge), dim3(B), 0, c.stream, np, ed, wpre,
+                           (const double*)c.nmm.p, cot, c.lnbc.p);
+        hipLaunchKernelGGL((limiter_adj_node_kernel<P>), dim3(gn), dim3(B), 0, c.stream, np, ne,
+                           (const int*)c.n2e_off.p, (const int*)c.n2e.p, wpre, (const double*)c.lnbc.p,
+                           c.lgsum.p, c.larg.p);
+        hipLaunchKernelGGL((limiter_adj_gather_kernel<P>), dim3(ge), dim3(B), 0, c.stream, np, ed,
+                           (const double*)c.lgsum.p, (const int*)c.larg.p, cot);
+        HIPCHK(hipGetLastError());
+        return 0;
+    };
     double* lam = c.lam.p;
     double* ub[KMAXRK + 1];
     for (int i = 0; i < nrk; i++) ub[i] = c.ubar[i].p;
@@ -1103,21 +1147,25 @@ int adj_backward(Ctx& c, int it0, int nsteps, int interval, double weight) {
         const int n = k1 - k0;
         // stage states of steps k0+1..k1: seg[(k-k0-1)*nrk + i] holds stage i of step k
         auto stage = [&](int k, int i) { return c.seg.p + ((size_t)(k - k0 - 1) * nrk + i) * ssz; };
+        // pre-limiter outputs: wstage(k, i) is stage i's output of step k before limiting
+        auto wstage = [&](int k, int i) { return c.wseg.p + ((size_t)(k - k0 - 1) * nrk + i) * ssz; };
         if (build_stage_tables(c, it0 + k0, n)) return 1;
         HIPCHK(hipMemcpyAsync(stage(k0 + 1, 0), c.ckpt.p + (size_t)sg * ssz, ssz * sizeof(double),
                               hipMemcpyDeviceToDevice, c.stream));
         c.hstep = 0;
         for (int k = k0 + 1; k < k1; k++) {
             double* st[KMAXRK + 1];
-            for (int i = 0; i < nrk; i++) st[i] = stage(k, i);
+            double* ws[KMAXRK];
+            for (int i = 0; i < nrk; i++) { st[i] = stage(k, i); ws[i] = lim ? wstage(k, i) : nullptr; }
             st[nrk] = stage(k + 1, 0);
-            if (launch_step_explicit<P>(c, st)) return 1;
+            if (launch_step_explicit<P>(c, st, lim ? ws : nullptr)) return 1;
         }
-        if (nrk > 1) {   // the last step's intermediate stages
+        if (nrk > 1 || lim) {   // the last step's intermediate stages (and pre-limiter outputs)
             double* st[KMAXRK + 1];
-            for (int i = 0; i < nrk; i++) st[i] = stage(k1, i);
+            double* ws[KMAXRK];
+            for (int i = 0; i < nrk; i++) { st[i] = stage(k1, i); ws[i] = lim ? wstage(k1, i) : nullptr; }
             st[nrk] = c.pout.p;   // output unused
-            if (launch_step_explicit<P>(c, st)) return 1;
+            if (launch_step_explicit<P>(c, st, lim ? ws : nullptr)) return 1;
         }
         for (int k = k1; k > k0; k--) {
             c.hstep = k - k0 - 1;
@@ -1127,6 +1175,7 @@ int adj_backward(Ctx& c, int it0, int nsteps, int interval, double weight) {
                 HIPCHK(hipGetLastError());
             }
             ub[nrk] = lam;
+            if (lim && limrev(ub[nrk], wstage(k, nrk - 1))) return 1;
             for (int i = nrk - 1; i >= 0; i--) {
                 AdjArgs a{};
                 a.ed = elem_data(c); a.ph = c.phys; a.fo = forcing(c);
@@ -1148,6 +1197,8 @@ int adj_backward(Ctx& c, int it0, int nsteps, int interval, double weight) {
                 else rc = launch_adj<P, 64>(c, a);
                 if (rc) return rc;
                 HIPCHK(hipGetLastError());
+                // ub[i] is now the whole cotangent of U(i), the limited output of stage i-1
+                if (lim && i >= 1 && limrev(ub[i], wstage(k, i - 1))) return 1;
             }
             std::swap(lam, ub[0]);   // cotangent of U^(k-1)
             if (k - 1 >= 1) seed(k - 1, lam);
@@ -1548,12 +1599,33 @@ int dg_adj_gradient(int h, int it, int nsteps, const double* data, double weight
     HIPCHK(c->gdp.alloc((size_t)3 * ne));
     HIPCHK(c->gnb.alloc(ne));
     HIPCHK(c->pout.alloc(std::max(ssz, (size_t)2 * np)));
+    if (c->prm.slopeflag != 0) {
+        HIPCHK(c->wseg.alloc((size_t)interval * nrk * ssz));
+        HIPCHK(c->lnbc.alloc((size_t)18 * ne));
+        HIPCHK(c->lgsum.alloc((size_t)6 * np));
+        HIPCHK(c->larg.alloc((size_t)6 * np));
+    }
+    // WETDRY is differentiated with its active set frozen at the identity: every element wet and
+    // left unchanged.  The forward sweeps flag any element it changes; such runs are refused.
+    HIPCHK(c->wdact.alloc(1));
+    HIPCHK(hipMemsetAsync(c->wdact.p, 0, sizeof(int), c->stream));
+    c->wd_active = c->prm.nolifa >= 2 ? c->wdact.p : nullptr;
     const bool prof = c->prof;
     c->prof = false;
     int rc = c->prm.p == 1 ? adj_forward<1>(*c, it, nsteps, interval, true) : adj_forward<2>(*c, it, nsteps, interval, true);
     if (!rc) rc = c->prm.p == 1 ? adj_backward<1>(*c, it, nsteps, interval, weight) : adj_backward<2>(*c, it, nsteps, interval, weight);
     c->prof = prof;
+    c->wd_active = nullptr;
     if (rc) return rc;
+    int wdflag = 0;
+    HIPCHK(hipMemcpyAsync(&wdflag, c->wdact.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (wdflag) {
+        HIPCHK(hipMemcpyAsync(c->u[c->ucur].p, c->ckpt.p, ssz * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return fail("wetting/drying changed elements during the run: the adjoint differentiates WETDRY "
+                    "only where it leaves every element as it is");
+    }
     // node gradients
     double* gdp_n = c->pout.p;
     double* gn_n = c->pout.p + np;

@@ -54,6 +54,7 @@ struct StageArgs {
     const int* __restrict__ hidx;
     const int* __restrict__ snbl;
     int ns;                              // staged elements per block (LDS row length)
+    int* __restrict__ wd_active;         // set to 1 when WETDRY changes an element, or nullptr
 };
 
 // ------------------------------------------------------------------------------------------
@@ -163,6 +164,22 @@ __device__ __forceinline__ int wetdry_element(const DevTables& T, double (&U)[3]
         return 0;
     }
     return wdflg;
+}
+
+// True when WETDRY leaves the element as it is (every vertex wet and the element already wet: the
+// identity branch of wetdry_element).
+template <int P>
+__device__ __forceinline__ bool wetdry_keeps(const DevTables& T, const double (&U)[3][(P + 1) * (P + 2) / 2],
+                                             const double dpn[3], int wdflg, double H0) {
+    if (wdflg != 1) return false;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        double z = U[0][0];
+#pragma unroll
+        for (int k = 1; k < 3; k++) z = z + T.phi_corner[i][k] * U[0][k];
+        if (dpn[i] + z <= (H0 + 1.0e-6)) return false;
+    }
+    return true;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -584,6 +601,7 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
     if (WDFUSED) {
         const double D[3] = {E.dpv[e], E.dpv[(size_t)ne + e], E.dpv[2 * (size_t)ne + e]};
         const double dpe_min = fmin(fmin(D[0], D[1]), D[2]);
+        if (a.wd_active && !wetdry_keeps<P>(T, W, D, wd, sc.h0)) a.wd_active[0] = 1;
         const int nw = wetdry_element<P>(T, W, D, hb, dpe_min, wd, sc.h0);
         a.wd_out[e] = (uint8_t)nw;
     }
@@ -636,7 +654,8 @@ template <int P, bool LIMIT, bool WETDRY>
 __global__ void __launch_bounds__(256) limit_wetdry_kernel(int np, ElemData E, double* __restrict__ u,
                                                            const double* __restrict__ mm,
                                                            const uint8_t* __restrict__ wd_in,
-                                                           uint8_t* __restrict__ wd_out) {
+                                                           uint8_t* __restrict__ wd_out,
+                                                           int* __restrict__ wd_active) {
     constexpr int DOF = (P + 1) * (P + 2) / 2;
     const DevTables& T = tab<P>();
     const int ne = E.ne;
@@ -699,6 +718,7 @@ __global__ void __launch_bounds__(256) limit_wetdry_kernel(int np, ElemData E, d
         const double hb[3] = {c13 * (D[0] + D[1] + D[2]), -(c16 * (D[0] + D[1])) + c13 * D[2], -(0.5 * D[0]) + 0.5 * D[1]};
         const double dpe_min = fmin(fmin(D[0], D[1]), D[2]);
         const int sid = E.sid ? (int)E.sid[e] : 0;
+        if (wd_active && !wetdry_keeps<P>(T, U, D, wd_in[e], E.scen[sid].h0)) wd_active[0] = 1;
         wd_out[e] = (uint8_t)wetdry_element<P>(T, U, D, hb, dpe_min, wd_in[e], E.scen[sid].h0);
     }
 #pragma unroll

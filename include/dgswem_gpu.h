@@ -159,8 +159,13 @@ int dg_adj_gauges(int handle, int it, int nsteps, double* eta);
 /* Loss J = weight * sum_{k<nsteps} sum_g (eta_g after step it+k - data[k][g])^2 from the current
  * state, and its gradient with respect to the per-node Manning n and depth DP, by the discrete
  * adjoint of the stage RHS and the SSP-RK stages.  States are checkpointed every `interval` steps
- * (0: round(sqrt(nsteps))) and each segment is recomputed once on the way back.  Runs without
- * slope limiter and wetting/drying; NWS=8 forcing and ensembles are supported.  For an ensemble
+ * (0: round(sqrt(nsteps))) and each segment is recomputed once on the way back.  The slope
+ * limiter (SLOPELIMITER5) is differentiated with its active set frozen as the forward run
+ * recomputes it (the selected node bounds, the redistribution counts and signs are those of the
+ * forward run; the reverse of each branch is exact), so the gradient is that of the piecewise
+ * smooth map on the forward run's branch.  Wetting/drying is taken as the identity it is when no
+ * element changes state; if any element wets or dries during the run the call fails (nonzero,
+ * dg_gpu_last_error() says so) and leaves the state as it was.  NWS=8 forcing and ensembles are supported.  For an ensemble
  * the members are independent, so the gradient of the summed loss restricted to a member's nodes
  * is that member's own gradient (all members in one batched forward and one batched reverse
  * sweep).  The handle's state is left as it was.  grad_manning / grad_dp: NP values each (either

@@ -13,6 +13,7 @@ from dgswem_amd import adjoint, gpu, mesh
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-6
+TOL_LIMITER = 1e-5      # slope limiter on: rounding-limited difference steps (see the limiter test)
 
 
 def small_case(p=1, n_elem=2000, seed=31, nsteps=60):
@@ -161,11 +162,56 @@ def test_p2_gradient_matches_central_differences():
         assert abs(fd - ad) <= TOL * abs(ad)
 
 
-def test_adjoint_rejects_unsupported_runs():
+def test_adjoint_rejects_runs_whose_wetting_and_drying_acts():
+    """WETDRY is differentiated with its active set frozen at the identity; a run in which it dries
+    or rewets elements is refused after the forward sweep, with the handle's state restored."""
     case = mesh.delaunay_basin(n_elem=1500, seed=33, lx=6.0e4, ly=3.6e4, depth_range=(2.0, 30.0),
-                               nsteps=10)           # limiter and wetting/drying on
+                               nsteps=10)           # limiter and wetting/drying on, dry land
     s = gpu.GpuSolver(case)
     s.set_gauges(np.array([1, 2], np.int32), np.array([-0.5, -0.5]), np.array([-0.5, -0.5]))
-    with pytest.raises(gpu.GpuError, match="limiter|wetting"):
+    before = s.state()
+    with pytest.raises(gpu.GpuError, match="wetting/drying changed"):
         s.gradient(np.zeros((5, 2)))
+    for a, b in zip(s.state(), before):
+        np.testing.assert_array_equal(a, b)
     s.close()
+
+
+def limiter_case(p=1, wetdry=False, n_elem=2000, seed=34, nsteps=60):
+    """All-wet shallow basin with the slope limiter on (and WETDRY on but inactive when wetdry)."""
+    return mesh.delaunay_basin(n_elem=n_elem, seed=seed, lx=6.0e4, ly=3.6e4, depth_range=(2.0, 10.0),
+                               shelf=False, wetdry=wetdry, slopeflag=5, p=p, nsteps=nsteps, cf=1e-4)
+
+
+@pytest.mark.parametrize("p,wetdry", [(1, False), (1, True), (2, True)])
+def test_frozen_limiter_gradient_matches_central_differences(p, wetdry):
+    """Slope limiter on: the reverse takes the forward's branches (clamps, redistribution tests,
+    min(redfac, redmax)) as fixed; central differences of the true forward agree as long as the
+    perturbation crosses no branch switch.  SLOPELIMITER5 is not even continuous (its
+    redistribution count and sign tests jump), and steps of 1e-7..1e-5 (relative) in bathymetry
+    reach a switch somewhere on these runs (a line scan of J shows the jump), so the step here is
+    1e-8.  At that step FP64 rounding of the loss limits the agreement to about 1e-6, so the
+    tolerance here is TOL_LIMITER; without the limiter the tests above hold 1e-6 at step 1e-4."""
+    nst = 40 if p == 1 else 60
+    case = limiter_case(p=p, wetdry=wetdry, nsteps=nst)
+    if p == 2:
+        case.params.dtdp = case.params.dtdp * 0.5
+    st = adjoint.hump_state(case, amplitude=0.8, width=0.15)
+    # the limiter does act on this run
+    free = mesh.delaunay_basin(n_elem=2000, seed=34, lx=6.0e4, ly=3.6e4, depth_range=(2.0, 10.0),
+                               shelf=False, wetdry=wetdry, slopeflag=0, p=p, nsteps=nst, cf=1e-4)
+    free.params.dtdp = case.params.dtdp
+    a_, b_ = gpu.GpuSolver(case), gpu.GpuSolver(free)
+    for g in (a_, b_):
+        g.set_state(*st)
+        g.step(10)
+    assert not np.array_equal(a_.state()[1], b_.state()[1])
+    a_.close(); b_.close()
+    prob = adjoint.twin_problem(case, nsteps=nst, ng=40, seed=8, state0=st, perturb=0.5)
+    J, gn, gdp = prob.gradient(case.manning, case.dp)
+    assert np.abs(gn).max() > 0.0
+    for vn, vd in directions(case, 3, seed=13):
+        ad = float(gn @ vn + gdp @ vd)
+        fd = central_fd(prob, case, vn, vd, eps=1e-8)
+        print(f"p={p} wetdry={wetdry}: adjoint {ad:.12e} fd {fd:.12e} rel {abs(fd - ad) / abs(ad):.2e}", flush=True)
+        assert abs(fd - ad) <= TOL_LIMITER * abs(ad)

@@ -1,12 +1,14 @@
 """Batched discrete adjoint: an ensemble of p=2 storm basins, each under its own Holland vortex
-(NWS=8) with its own gauges and perturbed-Manning twin data, differentiated in one batched
-forward and one batched reverse sweep (the 32 x 100k p=2 batched-adjoint workload in miniature).
+(NWS=8) with its own gauges and perturbed-Manning twin data, slope limiter and wetting/drying on
+with their active sets frozen from the forward run, differentiated in one batched forward and one
+batched reverse sweep (the 32 x 100k p=2 batched-adjoint workload in miniature).
 
 Checks: each member's slice of the batched gradient equals that member's own single-run gradient
 bit for bit (the members are independent and every element's arithmetic is the same); the batched
 gradient matches fourth-order central differences of the batched GPU forward model (relative
-1e-6); member 0's gradient matches central differences of the CPU oracle in FP64 (the CPU
-reference gradient).
+1e-6 in Manning-only directions, 1e-5 in directions that move the bathymetry, whose difference
+step must stay below the limiter's branch switches); member 0's gradient matches central
+differences of the CPU oracle in FP64 (the CPU reference gradient, relative 1e-6).
 """
 import copy
 
@@ -19,6 +21,7 @@ from oracle.oracle import Oracle, basis_at
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-6
+TOL_LIMITER = 1e-5
 EPS = 1e-4
 NSTEPS = 40
 
@@ -28,7 +31,9 @@ def fd4(loss, eps=EPS):
 
 
 def members():
-    return [mesh.storm_basin(ne, seed=seed, nsteps=NSTEPS, p=2, slopeflag=0, wetdry=False, shelf=False,
+    """p=2 storm basins with the slope limiter and wetting/drying on (all wet: WETDRY's active set
+    is the identity), each under its own Holland vortex."""
+    return [mesh.storm_basin(ne, seed=seed, nsteps=NSTEPS, p=2, slopeflag=5, wetdry=True, shelf=False,
                              cfl=0.15, lx=3.0e5, ly=2.0e5, depth_range=(3.0, 20.0), cf=1e-4)
             for ne, seed in ((1500, 61), (2500, 62), (1200, 63))]
 
@@ -36,7 +41,7 @@ def members():
 @pytest.fixture(scope="module")
 def batch():
     cases = members()
-    eprob, probs = adjoint.ensemble_problem(cases, nsteps=NSTEPS, ng=10, seed=70)
+    eprob, probs = adjoint.ensemble_problem(cases, nsteps=NSTEPS, ng=10, seed=70, state0="hump", perturb=0.5)
     mn = np.concatenate([c.manning for c in cases])
     dp = np.concatenate([c.dp for c in cases])
     J, gn, gdp = eprob.gradient(mn, dp)
@@ -64,9 +69,11 @@ def test_batched_gradient_matches_central_differences(batch):
         vn = rng.standard_normal(mn.size) * mn
         vd = rng.standard_normal(dp.size) * dp if d < 2 else np.zeros(dp.size)
         ad = float(gn @ vn + gdp @ vd)
-        fd = fd4(lambda t: eprob.loss(mn + t * vn, dp + t * vd))
+        # depth steps of 1e-7 and more can reach a limiter branch switch (see test_gpu_adjoint.py);
+        # Manning-only directions move the state little and take the larger, less noisy step
+        fd = fd4(lambda t: eprob.loss(mn + t * vn, dp + t * vd), 1e-8 if d < 2 else EPS)
         print(f"direction {d}: adjoint {ad:.12e} fd {fd:.12e}", flush=True)
-        assert abs(fd - ad) <= TOL * abs(ad)
+        assert abs(fd - ad) <= (TOL_LIMITER if d < 2 else TOL) * abs(ad)
 
 
 def test_member_gradient_matches_cpu_oracle_differences(batch):
