@@ -629,6 +629,10 @@ int init_ctx_common(Ctx& c, const dg_params* prm, bool manning) {
     std::string err;
     if (!build_ref_tables(prm->p, c.T, err)) return fail(err);
     if (!build_rk(prm->rk_stage, prm->rk_order, c.rk, err)) return fail(err);
+    // the kernels start edge traces at mode 1's coefficient, which needs PHI_EDGE(1, .) == 1
+    for (int l = 0; l < 3; l++)
+        for (int q = 0; q < c.T.negp; q++)
+            if (c.T.phi_edge[l][q][0] != 1.0) return fail("basis tables: PHI_EDGE of mode 1 is not 1");
     c.dof = c.T.dof; c.nagp = c.T.nagp; c.negp = c.T.negp;
     c.block = 128;
     {

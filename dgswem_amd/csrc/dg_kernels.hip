@@ -201,6 +201,15 @@ __device__ __forceinline__ double trace(const double (&u)[DOF], const double* ph
     return s;
 }
 
+// The same sum where ph[0] == 1: 0 + u0*1 = u0 for every u0 but -0, so the sum starts at u0.
+template <int DOF>
+__device__ __forceinline__ double trace1(const double (&u)[DOF], const double* ph) {
+    double s = u[0];
+#pragma unroll
+    for (int k = 1; k < DOF; k++) s = s + u[k] * ph[k];
+    return s;
+}
+
 template <int P, int NT, bool SPH, bool WDFUSED, bool MET, bool NL>
 __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
     constexpr int DOF = (P + 1) * (P + 2) / 2;
@@ -231,49 +240,66 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
         tpe[l][g][k] = T.phi_edge[l][g][k];
         teq[l][g][k] = T.edgeq[l][g][k];
     }
+    // per-element loads that do not depend on the staged data, issued before the staging loads so
+    // that the latencies overlap (lanes past the last element read element ne-1 and exit after
+    // the barrier)
+    const int ec = e < ne ? e : ne - 1;
+    const double X0 = E.xv[ec], X1 = E.xv[(size_t)ne + ec], X2 = E.xv[2 * (size_t)ne + ec];
+    const double Y0 = E.yv[ec], Y1 = E.yv[(size_t)ne + ec], Y2 = E.yv[2 * (size_t)ne + ec];
+    const int info = E.sinfo[ec];
+    const int nb0 = E.snbr[ec], nb1 = E.snbr[(size_t)ne + ec], nb2 = E.snbr[2 * (size_t)ne + ec];
+    const int sl = a.snbl[ec];
+    const int sid = E.sid ? (int)E.sid[ec] : 0;
+    const Scen sc = E.scen[sid];
+    const double cori = E.cori_el ? E.cori_el[ec] : sc.cori;
+    const double nbar = ph.load_manning ? E.nbar[ec] : 0.0;
     {
-        // this tile's elements and its halo: all loads issued together, one latency
+        // this tile's elements and its halo, two entries per lane per pass with all their loads
+        // issued before any is used
         const int h0 = a.hoff[tile], nh = a.hoff[tile + 1] - h0;
-        for (int j = tid; j < NT + nh; j += NT) {
-            const int g = j < NT ? tile * NT + j : a.hidx[h0 + j - NT];
-            if (g < ne) {
+        for (int j = tid; j < NT + nh; j += 2 * NT) {
+            const int j2 = j + NT;
+            const bool has2 = j2 < NT + nh;
+            int g = j < NT ? tile * NT + j : a.hidx[h0 + j - NT];
+            int g2 = has2 ? a.hidx[h0 + j2 - NT] : ne;
+            const bool ok = g < ne, ok2 = g2 < ne;
+            g = ok ? g : 0;
+            g2 = ok2 ? g2 : 0;
+            double v[3 * DOF + 3], v2[3 * DOF + 3];
 #pragma unroll
-                for (int f = 0; f < 3 * DOF; f++) stg[f * ns + j] = a.uin[(size_t)f * ne + g];
-                const double d0 = E.dpv[g], d1 = E.dpv[(size_t)ne + g], d2 = E.dpv[2 * (size_t)ne + g];
+            for (int f = 0; f < 3 * DOF; f++) { v[f] = a.uin[(size_t)f * ne + g]; v2[f] = a.uin[(size_t)f * ne + g2]; }
+#pragma unroll
+            for (int m = 0; m < 3; m++) { v[3 * DOF + m] = E.dpv[(size_t)m * ne + g]; v2[3 * DOF + m] = E.dpv[(size_t)m * ne + g2]; }
+            const uint8_t w = a.wd_in[g], w2 = a.wd_in[g2];
+            if (ok) {
+#pragma unroll
+                for (int f = 0; f < 3 * DOF; f++) stg[f * ns + j] = v[f];
+                const double d0 = v[3 * DOF], d1 = v[3 * DOF + 1], d2 = v[3 * DOF + 2];
                 stg[(3 * DOF) * ns + j] = c13 * (d0 + d1 + d2);
                 stg[(3 * DOF + 1) * ns + j] = -(c16 * (d0 + d1)) + c13 * d2;
                 stg[(3 * DOF + 2) * ns + j] = -(0.5 * d0) + 0.5 * d1;
-                stwd[j] = a.wd_in[g];
+                stwd[j] = w;
+            }
+            if (ok2) {
+#pragma unroll
+                for (int f = 0; f < 3 * DOF; f++) stg[f * ns + j2] = v2[f];
+                const double d0 = v2[3 * DOF], d1 = v2[3 * DOF + 1], d2 = v2[3 * DOF + 2];
+                stg[(3 * DOF) * ns + j2] = c13 * (d0 + d1 + d2);
+                stg[(3 * DOF + 1) * ns + j2] = -(c16 * (d0 + d1)) + c13 * d2;
+                stg[(3 * DOF + 2) * ns + j2] = -(0.5 * d0) + 0.5 * d1;
+                stwd[j2] = w2;
             }
         }
     }
     __syncthreads();
     if (e >= ne) return;
-    const int sid = E.sid ? (int)E.sid[e] : 0;
-    const Scen sc = E.scen[sid];
     const int row = table_step(a.fo) * a.fo.nrk + a.irk;
 
-    double U[3][DOF];
-#pragma unroll
-    for (int f = 0; f < 3; f++)
-#pragma unroll
-        for (int k = 0; k < DOF; k++) U[f][k] = stg[(f * DOF + k) * ns + tid];
     const int wd = stwd[tid];
-    double AREAS, hb[3];
-    {
-        const double X0 = E.xv[e], X1 = E.xv[(size_t)ne + e], X2 = E.xv[2 * (size_t)ne + e];
-        const double Y0 = E.yv[e], Y1 = E.yv[(size_t)ne + e], Y2 = E.yv[2 * (size_t)ne + e];
-        AREAS = (X0 - X2) * (Y1 - Y2) + (X2 - X1) * (Y0 - Y2);
-#pragma unroll
-        for (int m = 0; m < 3; m++) hb[m] = stg[(3 * DOF + m) * ns + tid];
-    }
+    const double AREAS = (X0 - X2) * (Y1 - Y2) + (X2 - X1) * (Y0 - Y2);
 #pragma unroll
     for (int j = 0; j < 3 * DOF; j++) { acc_lds[0][j][tid] = 0.0; acc_lds[1][j][tid] = 0.0; }
 
-    // the three sides' neighbours (global indices, and staged indices of internal neighbours)
-    const int info = E.sinfo[e];
-    const int nb0 = E.snbr[e], nb1 = E.snbr[(size_t)ne + e], nb2 = E.snbr[2 * (size_t)ne + e];
-    const int sl = a.snbl[e];
 #pragma unroll 1
     for (int slot = 0; slot < 3; slot++) {
         const int bits = side_bits(info, slot);
@@ -282,8 +308,10 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
         // edge geometry in the reference's node order (the counter-clockwise order of NEDEL(1))
         int va = (L + 1) % 3, vb = (L + 2) % 3;
         if (role) { const int t = va; va = vb; vb = t; }
-        const double dy = E.yv[(size_t)vb * ne + e] - E.yv[(size_t)va * ne + e];
-        const double dx = E.xv[(size_t)vb * ne + e] - E.xv[(size_t)va * ne + e];
+        const double xa = va == 0 ? X0 : (va == 1 ? X1 : X2), ya = va == 0 ? Y0 : (va == 1 ? Y1 : Y2);
+        const double xb = vb == 0 ? X0 : (vb == 1 ? X1 : X2), yb = vb == 0 ? Y0 : (vb == 1 ? Y1 : Y2);
+        const double dy = yb - ya;
+        const double dx = xb - xa;
         const double xlen = sqrt(dy * dy + dx * dx);
         const double nx = dy / xlen, ny = -dx / xlen;
         const double tx = -ny, ty = nx;
@@ -291,14 +319,23 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
             const int ln = (sl >> (10 * slot)) & 1023;
             const int wdn = stwd[ln];
             if (wd == 0 && wdn == 0) continue;
-            double V[3][DOF];
+            // the edge's "in" (NEDEL(1)) and "ex" (NEDEL(2)) elements as staged rows, their local
+            // edge numbers and wet/dry flags: the traces are formed in in/ex order directly
+            const int si = role ? ln : tid, sx = role ? tid : ln;
+            const int Li = role ? nled : L, Lx = role ? L : nled;
+            double Vi[3][DOF], Vx[3][DOF], hbi[3], hbx[3];
 #pragma unroll
             for (int f = 0; f < 3; f++)
 #pragma unroll
-                for (int k = 0; k < DOF; k++) V[f][k] = stg[(f * DOF + k) * ns + ln];
-            double hbn[3];
+                for (int k = 0; k < DOF; k++) {
+                    Vi[f][k] = stg[(f * DOF + k) * ns + si];
+                    Vx[f][k] = stg[(f * DOF + k) * ns + sx];
+                }
 #pragma unroll
-            for (int m = 0; m < 3; m++) hbn[m] = stg[(3 * DOF + m) * ns + ln];
+            for (int m = 0; m < 3; m++) {
+                hbi[m] = stg[(3 * DOF + m) * ns + si];
+                hbx[m] = stg[(3 * DOF + m) * ns + sx];
+            }
             const int wd_in_el = role ? wdn : wd, wd_ex_el = role ? wd : wdn;
             const double edfac = xlen / AREAS;
             double acc[3][DOF];
@@ -309,24 +346,22 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
 #pragma unroll
             for (int I = 0; I < NEGP; I++) {
                 const int gi = I, gx = NEGP - 1 - I;
-                const int gown = role ? gx : gi, gnb = role ? gi : gx;
-                const double* po = tpe[L][gown];
-                const double* pn = tpe[nled][gnb];
-                const double zo = trace<DOF>(U[0], po), xo = trace<DOF>(U[1], po), yo = trace<DOF>(U[2], po);
-                const double zn = trace<DOF>(V[0], pn), xn = trace<DOF>(V[1], pn), yn = trace<DOF>(V[2], pn);
-                const double ho = trace<3>(hb, po), hn = trace<3>(hbn, pn);
-                double so = 1.0, sn = 1.0;
+                const int gown = role ? gx : gi;
+                const double* pi_ = tpe[Li][gi];
+                const double* px_ = tpe[Lx][gx];
+                // PHI_EDGE mode 1 is 1 (checked at set-up), so the sum from zero starts at mode 1's
+                // coefficient with the same value
+                const double ze_in = trace1<DOF>(Vi[0], pi_), qx_in = trace1<DOF>(Vi[1], pi_), qy_in = trace1<DOF>(Vi[2], pi_);
+                const double ze_ex = trace1<DOF>(Vx[0], px_), qx_ex = trace1<DOF>(Vx[1], px_), qy_ex = trace1<DOF>(Vx[2], px_);
+                const double hb_in = trace1<3>(hbi, pi_), hb_ex = trace1<3>(hbx, px_);
+                double sf_in = 1.0, sf_ex = 1.0;
                 if (SPH) {
-                    so = E.sfs[((size_t)slot * NEGP + I) * ne + e];
-                    sn = E.sfs[((size_t)(3 + slot) * NEGP + I) * ne + e];
+                    sf_in = E.sfs[((size_t)((role ? 3 : 0) + slot) * NEGP + I) * ne + e];
+                    sf_ex = E.sfs[((size_t)((role ? 0 : 3) + slot) * NEGP + I) * ne + e];
                 }
-                const double ze_in = role ? zn : zo, qx_in = role ? xn : xo, qy_in = role ? yn : yo;
-                const double ze_ex = role ? zo : zn, qx_ex = role ? xo : xn, qy_ex = role ? yo : yn;
-                const double hb_in = role ? hn : ho, hb_ex = role ? ho : hn;
-                const double sf_in = role ? sn : so, sf_ex = role ? so : sn;
                 double f0, f1, f2;
                 llf<NL>(ph, ze_in, ze_ex, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_ex, sf_in, sf_ex, nx, ny, ph.g, f0, f1, f2);
-                double fo = f0, gm = role ? f1 : f1, hm = f2;
+                double fo = f0, gm = f1, hm = f2;
                 // dry-side treatment (src/internal_edge_hydro.F:212-302): 1 reflect into a dry
                 // "in" element, 2 zero-gravity flux for a dry "in", 3 reflect into a dry "ex",
                 // 4 zero-gravity flux for a dry "ex"; only what this element's side receives
@@ -375,6 +410,13 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
                 for (int k = 0; k < DOF; k++) acc_lds[1][f * DOF + k][tid] = acc_lds[1][f * DOF + k][tid] + acc[f][k];
         } else {
             if (wd == 0) continue;
+            double U[3][DOF], hb[3];
+#pragma unroll
+            for (int f = 0; f < 3; f++)
+#pragma unroll
+                for (int k = 0; k < DOF; k++) U[f][k] = stg[(f * DOF + k) * ns + tid];
+#pragma unroll
+            for (int m = 0; m < 3; m++) hb[m] = stg[(3 * DOF + m) * ns + tid];
 #pragma unroll 1
             for (int I = 0; I < NEGP; I++) {
                 const double* pv = tpe[L][I];
@@ -432,12 +474,32 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
             }
         }
     }
+    // the element's own state and bathymetry modes, staged
+    double U[3][DOF], hb[3];
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+#pragma unroll
+        for (int k = 0; k < DOF; k++) U[f][k] = stg[(f * DOF + k) * ns + tid];
+#pragma unroll
+    for (int m = 0; m < 3; m++) hb[m] = stg[(3 * DOF + m) * ns + tid];
     // add_internal_flux: RHS = RHS(boundary edges) + slot sum
     double R[3][DOF];
 #pragma unroll
     for (int f = 0; f < 3; f++)
 #pragma unroll
         for (int k = 0; k < DOF; k++) R[f][k] = acc_lds[0][f * DOF + k][tid] + acc_lds[1][f * DOF + k][tid];
+
+    // the first RK base state, loaded before the volume quadrature so that its latency overlaps
+    // the quadrature's arithmetic
+    const bool pre0 = a.nprev > 0 && a.aprev[0] != 0.0;
+    double B0[3][DOF];
+    if (pre0) {
+        const double* up = a.uprev[0];
+#pragma unroll
+        for (int f = 0; f < 3; f++)
+#pragma unroll
+            for (int k = 0; k < DOF; k++) B0[f][k] = up[(size_t)(f * DOF + k) * ne + e];
+    }
 
     // ---- volume quadrature (RHS_DG_HYDRO)
     if (sc.rain > 0.0) {
@@ -449,8 +511,6 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
     if (wd != 0) {
         double drdx, dsdx, drdy, dsdy;
         {
-            const double X0 = E.xv[e], X1 = E.xv[(size_t)ne + e], X2 = E.xv[2 * (size_t)ne + e];
-            const double Y0 = E.yv[e], Y1 = E.yv[(size_t)ne + e], Y2 = E.yv[2 * (size_t)ne + e];
             double AREA = (X0 - X2) * (Y1 - Y2) + (X2 - X1) * (Y0 - Y2);
             AREA = AREA * 0.5;
             drdx = 1.0 / AREA * (Y2 - Y0);
@@ -490,8 +550,6 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
             const double fh_nl_c = NL ? 1.0 / depth_c : 1.0 / (ph.nleq * depth_c + ph.leq);
             hybf = pow(1.0 + pow(ph.hbreak * fh_nl_c, ph.ftheta), ph.fgamma / ph.ftheta);
         }
-        const double cori = E.cori_el ? E.cori_el[e] : sc.cori;
-        const double nbar = ph.load_manning ? E.nbar[e] : 0.0;
 #pragma unroll 1
         for (int I = 0; I < NAGP; I++) {
             const double* pa = T.phi_area[I];
@@ -578,7 +636,12 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
 #pragma unroll 1
     for (int i = 0; i < a.nprev; i++) {
         const double ai = a.aprev[i], bi = a.bprev[i] * sc.dtdp;
-        if (ai != 0.0) {
+        if (i == 0 && pre0) {
+#pragma unroll
+            for (int f = 0; f < 3; f++)
+#pragma unroll
+                for (int k = 0; k < DOF; k++) W[f][k] = W[f][k] + ai * B0[f][k];
+        } else if (ai != 0.0) {
             const double* up = a.uprev[i];
 #pragma unroll
             for (int f = 0; f < 3; f++)
