@@ -211,7 +211,7 @@ __device__ __forceinline__ double trace1(const double (&u)[DOF], const double* p
 }
 
 template <int P, int NT, bool SPH, bool WDFUSED, bool MET, bool NL>
-__global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 ? 2 : 1))) stage_kernel(StageArgs a) {
     constexpr int DOF = (P + 1) * (P + 2) / 2;
     constexpr int NAGP = (P == 1) ? 3 : 6;
     constexpr int NEGP = P + 2;
@@ -337,12 +337,22 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
                 hbx[m] = stg[(3 * DOF + m) * ns + sx];
             }
             const int wd_in_el = role ? wdn : wd, wd_ex_el = role ? wd : wdn;
+            // spherical factors of both sides at every point, loaded together
+            double sfi[NEGP], sfx[NEGP];
+#pragma unroll
+            for (int I = 0; I < NEGP; I++) {
+                sfi[I] = 1.0;
+                sfx[I] = 1.0;
+                if (SPH) {
+                    sfi[I] = E.sfs[((size_t)((role ? 3 : 0) + slot) * NEGP + I) * ne + e];
+                    sfx[I] = E.sfs[((size_t)((role ? 0 : 3) + slot) * NEGP + I) * ne + e];
+                }
+            }
+            // the "in" side subtracts w*F and the "ex" side adds it; -(a*b) == (-a)*b exactly, so
+            // the sign goes on the edge factor once and one update serves both roles
             const double edfac = xlen / AREAS;
+            const double edfs = role ? edfac : -edfac;
             double acc[3][DOF];
-#pragma unroll
-            for (int f = 0; f < 3; f++)
-#pragma unroll
-                for (int k = 0; k < DOF; k++) acc[f][k] = 0.0;
 #pragma unroll
             for (int I = 0; I < NEGP; I++) {
                 const int gi = I, gx = NEGP - 1 - I;
@@ -354,11 +364,7 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
                 const double ze_in = trace1<DOF>(Vi[0], pi_), qx_in = trace1<DOF>(Vi[1], pi_), qy_in = trace1<DOF>(Vi[2], pi_);
                 const double ze_ex = trace1<DOF>(Vx[0], px_), qx_ex = trace1<DOF>(Vx[1], px_), qy_ex = trace1<DOF>(Vx[2], px_);
                 const double hb_in = trace1<3>(hbi, pi_), hb_ex = trace1<3>(hbx, px_);
-                double sf_in = 1.0, sf_ex = 1.0;
-                if (SPH) {
-                    sf_in = E.sfs[((size_t)((role ? 3 : 0) + slot) * NEGP + I) * ne + e];
-                    sf_ex = E.sfs[((size_t)((role ? 0 : 3) + slot) * NEGP + I) * ne + e];
-                }
+                const double sf_in = sfi[I], sf_ex = sfx[I];
                 double f0, f1, f2;
                 llf<NL>(ph, ze_in, ze_ex, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_ex, sf_in, sf_ex, nx, ny, ph.g, f0, f1, f2);
                 double fo = f0, gm = f1, hm = f2;
@@ -394,14 +400,17 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
                     else if (mode == 3) { fo = r0; gm = role ? 0.0 : r1; hm = role ? 0.0 : r2; }
                     else { gm = r1; hm = r2; }
                 }
-                // the "in" side subtracts w*F and the "ex" side adds it: a - b = a + (-b) exactly,
-                // so one branch-free update serves both roles
+                // the slot sums start from zero: 0 + x == x for every x but -0, so the first
+                // point's terms start them
                 const double* eq = teq[L][gown];
 #pragma unroll
                 for (int k = 0; k < DOF; k++) {
-                    const double w = edfac * eq[k];
-                    const double ws = role ? w : -w;
-                    acc[0][k] = acc[0][k] + ws * fo; acc[1][k] = acc[1][k] + ws * gm; acc[2][k] = acc[2][k] + ws * hm;
+                    const double ws = edfs * eq[k];
+                    if (I == 0) {
+                        acc[0][k] = ws * fo; acc[1][k] = ws * gm; acc[2][k] = ws * hm;
+                    } else {
+                        acc[0][k] = acc[0][k] + ws * fo; acc[1][k] = acc[1][k] + ws * gm; acc[2][k] = acc[2][k] + ws * hm;
+                    }
                 }
             }
 #pragma unroll
@@ -489,8 +498,20 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
 #pragma unroll
         for (int k = 0; k < DOF; k++) R[f][k] = acc_lds[0][f * DOF + k][tid] + acc_lds[1][f * DOF + k][tid];
 
-    // the first RK base state, loaded before the volume quadrature so that its latency overlaps
-    // the quadrature's arithmetic
+    // loads of the volume quadrature and the RK update, issued together before the quadrature's
+    // arithmetic: the nodal wind and pressure (MET), the area-point spherical factors (SPH) and
+    // the first RK base state
+    double mcd[3], mwm[3], mwx[3], mwy[3], mpr[3];
+    if (MET) {
+#pragma unroll
+        for (int v = 0; v < 3; v++) {
+            const int n = E.nm[(size_t)v * ne + e];
+            mcd[v] = a.mt.cd[n]; mwm[v] = a.mt.wm[n]; mwx[v] = a.mt.wx[n]; mwy[v] = a.mt.wy[n]; mpr[v] = a.mt.prn[n];
+        }
+    }
+    double sfqv[NAGP];
+#pragma unroll
+    for (int I = 0; I < NAGP; I++) sfqv[I] = SPH ? E.sfacq[(size_t)I * ne + e] : 1.0;
     const bool pre0 = a.nprev > 0 && a.aprev[0] != 0.0;
     double B0[3][DOF];
     if (pre0) {
@@ -528,11 +549,10 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
             double pdx = 0.0, pdy = 0.0;
 #pragma unroll
             for (int v = 0; v < 3; v++) {
-                const int n = E.nm[(size_t)v * ne + e];
-                const double cd = a.mt.cd[n], wm = a.mt.wm[n];
-                mw[v] = ramp * 0.001293 * cd * a.mt.wx[n] * wm;
-                mw[3 + v] = ramp * 0.001293 * cd * a.mt.wy[n] * wm;
-                const double pr = ramp * a.mt.prn[n];
+                const double cd = mcd[v], wm = mwm[v];
+                mw[v] = ramp * 0.001293 * cd * mwx[v] * wm;
+                mw[3 + v] = ramp * 0.001293 * cd * mwy[v] * wm;
+                const double pr = ramp * mpr[v];
                 const double dpsidx = drpsi[v] * drdx + dspsi[v] * dsdx;
                 const double dpsidy = drpsi[v] * drdy + dspsi[v] * dsdy;
                 if (v == 0) { pdx = pr * dpsidx; pdy = pr * dpsidy; }
@@ -569,7 +589,7 @@ __global__ void __launch_bounds__(NT) stage_kernel(StageArgs a) {
                 dhbx = dhbx + hb[k] * dpxk;
                 dhby = dhby + hb[k] * dpyk;
             }
-            const double sfq = SPH ? E.sfacq[(size_t)I * ne + e] : 1.0;
+            const double sfq = sfqv[I];
             const double depth = ze + hbq;
             // with NL (NLEQ = 1, LEQ = 0) the constant factors fold exactly
             const double f1 = NL ? 1.0 : ph.nleq + ph.leq * hbq;
@@ -688,23 +708,26 @@ __global__ void __launch_bounds__(256) node_minmax_kernel(int np, int ne, const 
     if (n >= np) return;
     double zmin = 99999., zmax = -99999., xmin = 99999., xmax = -99999., ymin = 99999., ymax = -99999.;
     const int j1 = off[n + 1];
-    int j = off[n];
-    // two incidences per iteration, so that their loads are in flight together (min and max are
-    // exact: the order does not matter)
-    for (; j + 1 < j1; j += 2) {
-        const int e0 = n2e[j] >> 2, e1 = n2e[j + 1] >> 2;
-        const double z0 = u[e0], x0 = u[(size_t)DOF * ne + e0], y0 = u[(size_t)2 * DOF * ne + e0];
-        const double z1 = u[e1], x1 = u[(size_t)DOF * ne + e1], y1 = u[(size_t)2 * DOF * ne + e1];
-        zmin = fmin(zmin, z0); xmin = fmin(xmin, x0); ymin = fmin(ymin, y0);
-        zmax = fmax(zmax, z0); xmax = fmax(xmax, x0); ymax = fmax(ymax, y0);
-        zmin = fmin(zmin, z1); xmin = fmin(xmin, x1); ymin = fmin(ymin, y1);
-        zmax = fmax(zmax, z1); xmax = fmax(xmax, x1); ymax = fmax(ymax, y1);
-    }
-    if (j < j1) {
-        const int e0 = n2e[j] >> 2;
-        const double z0 = u[e0], x0 = u[(size_t)DOF * ne + e0], y0 = u[(size_t)2 * DOF * ne + e0];
-        zmin = fmin(zmin, z0); xmin = fmin(xmin, x0); ymin = fmin(ymin, y0);
-        zmax = fmax(zmax, z0); xmax = fmax(xmax, x0); ymax = fmax(ymax, y0);
+    // up to eight incidences per pass: all their indices, then all their cell means, are loaded
+    // before any is used; past the node's last incidence the pass repeats that incidence (min and
+    // max are exact and idempotent, so neither the order nor a repeat changes them)
+    constexpr int B = 8;
+    for (int j = off[n]; j < j1; j += B) {
+        int ei[B];
+#pragma unroll
+        for (int q = 0; q < B; q++) ei[q] = n2e[min(j + q, j1 - 1)] >> 2;
+        double z[B], x[B], y[B];
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            z[q] = u[ei[q]];
+            x[q] = u[(size_t)DOF * ne + ei[q]];
+            y[q] = u[(size_t)2 * DOF * ne + ei[q]];
+        }
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            zmin = fmin(zmin, z[q]); xmin = fmin(xmin, x[q]); ymin = fmin(ymin, y[q]);
+            zmax = fmax(zmax, z[q]); xmax = fmax(xmax, x[q]); ymax = fmax(ymax, y[q]);
+        }
     }
     mm[n] = zmin; mm[(size_t)np + n] = zmax; mm[2 * (size_t)np + n] = xmin;
     mm[3 * (size_t)np + n] = xmax; mm[4 * (size_t)np + n] = ymin; mm[5 * (size_t)np + n] = ymax;
@@ -732,13 +755,33 @@ __global__ void __launch_bounds__(256) limit_wetdry_kernel(int np, ElemData E, d
     // the limiter keeps the cell means; wetting and drying changes them only for some elements,
     // so a cell mean is written back only when its bits changed
     const double c0in[3] = {U[0][0], U[1][0], U[2][0]};
+    // every load the element needs is issued here, before any arithmetic, so that the latencies
+    // overlap: the node bounds of all three fields and the wet/dry inputs
+    double bmin[3][3], bmax[3][3];
     if (LIMIT) {
         const int n0 = E.nm[e], n1 = E.nm[(size_t)ne + e], n2 = E.nm[2 * (size_t)ne + e];
 #pragma unroll
         for (int iv = 0; iv < 3; iv++) {
             const double* mn = mm + (size_t)(2 * iv) * np;
             const double* mx = mm + (size_t)(2 * iv + 1) * np;
-            const double vmin[3] = {mn[n0], mn[n1], mn[n2]}, vmax[3] = {mx[n0], mx[n1], mx[n2]};
+            bmin[iv][0] = mn[n0]; bmin[iv][1] = mn[n1]; bmin[iv][2] = mn[n2];
+            bmax[iv][0] = mx[n0]; bmax[iv][1] = mx[n1]; bmax[iv][2] = mx[n2];
+        }
+    }
+    double D[3] = {0.0, 0.0, 0.0}, h0 = 0.0;
+    int wdi = 0;
+    if (WETDRY) {
+#pragma unroll
+        for (int v = 0; v < 3; v++) D[v] = E.dpv[(size_t)v * ne + e];
+        const int sid = E.sid ? (int)E.sid[e] : 0;
+        h0 = E.scen[sid].h0;
+        wdi = wd_in[e];
+    }
+    if (LIMIT) {
+#pragma unroll
+        for (int iv = 0; iv < 3; iv++) {
+            const double* vmin = bmin[iv];
+            const double* vmax = bmax[iv];
             const double c0 = U[iv][0];
             double v[3];
 #pragma unroll
@@ -774,15 +817,11 @@ __global__ void __launch_bounds__(256) limit_wetdry_kernel(int np, ElemData E, d
         }
     }
     if (WETDRY) {
-        double D[3];
-#pragma unroll
-        for (int v = 0; v < 3; v++) D[v] = E.dpv[(size_t)v * ne + e];
         const double c13 = 1.0 / 3.0, c16 = 1.0 / 6.0;
         const double hb[3] = {c13 * (D[0] + D[1] + D[2]), -(c16 * (D[0] + D[1])) + c13 * D[2], -(0.5 * D[0]) + 0.5 * D[1]};
         const double dpe_min = fmin(fmin(D[0], D[1]), D[2]);
-        const int sid = E.sid ? (int)E.sid[e] : 0;
-        if (wd_active && !wetdry_keeps<P>(T, U, D, wd_in[e], E.scen[sid].h0)) wd_active[0] = 1;
-        wd_out[e] = (uint8_t)wetdry_element<P>(T, U, D, hb, dpe_min, wd_in[e], E.scen[sid].h0);
+        if (wd_active && !wetdry_keeps<P>(T, U, D, wdi, h0)) wd_active[0] = 1;
+        wd_out[e] = (uint8_t)wetdry_element<P>(T, U, D, hb, dpe_min, wdi, h0);
     }
 #pragma unroll
     for (int f = 0; f < 3; f++) {
