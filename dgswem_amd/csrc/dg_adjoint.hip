@@ -467,13 +467,11 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
                 } else if (kind == SIDE_OCEAN) {
                     double ze_ex = 0.0;
                     const double xe = T.xegp[I];
+                    // the constituents' endpoint terms EMO*RFF*COS(ARGJ-EFA) of this stage (tide_kernel)
                     for (int jj = 0; jj < a.fo.nbfr; jj++) {
-                        const size_t t = ((size_t)sid * a.fo.rows + row) * a.fo.nbfr + jj;
-                        const double argj = a.fo.argj[t];
-                        const double rff = a.fo.rff[t];
                         const size_t o = ((size_t)nb * a.fo.nbfr + jj) * 2;
-                        const double z1 = a.fo.emo[o] * rff * cos(argj - a.fo.efa[o]);
-                        const double z2 = a.fo.emo[o + 1] * rff * cos(argj - a.fo.efa[o + 1]);
+                        const double z1 = a.fo.tz[(size_t)a.irk * a.fo.tzn + o];
+                        const double z2 = a.fo.tz[(size_t)a.irk * a.fo.tzn + o + 1];
                         ze_ex = ze_ex + (0.5 * (z1 + z2) + 0.5 * (z2 - z1) * xe);
                     }
                     const bool eirik = (ze_ex * ph.ifnlfa + hb_in) <= 0;
@@ -584,7 +582,7 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
             bool fric_active = false;
             double pw = 0.0;
             if (ph.load_manning) {
-                pw = cbrt(depth);
+                pw = crm::cr_pow_third(depth);
                 fric = ph.g * (nbar * nbar) / pw;
                 if (fric < sc.cf) fric = sc.cf;
                 else fric_active = true;

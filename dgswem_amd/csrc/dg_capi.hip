@@ -125,6 +125,9 @@ struct Ctx {
     // forcing
     DevBuf<double> emo, efa, qnam, qnph, argj, rff, fargj, frff;
     DevBuf<int> step;                            // device step counter within the current tables
+    DevBuf<double> tz;                           // tidal terms of the current step [stage][needs][nbfr][2]
+    DevBuf<uint16_t> osid;                       // scenario of each ocean edge (ensembles)
+    int tzn = 0;                                 // tidal terms per stage
     int hstep = 0;                               // the same index for eager (uncaptured) launches
     int rows = 0;                                // table rows per scenario
     // NWS=8 meteorological forcing
@@ -252,6 +255,7 @@ Forcing forcing(Ctx& c, bool captured = false) {
     F.emo = c.emo.p; F.efa = c.efa.p; F.qnam = c.qnam.p; F.qnph = c.qnph.p;
     F.argj = c.argj.p; F.rff = c.rff.p; F.fargj = c.fargj.p; F.frff = c.frff.p;
     F.ramp = c.ramp.p;
+    F.tz = c.tz.p; F.tzn = c.tzn;
     return F;
 }
 
@@ -464,6 +468,14 @@ int upload_forcing(Ctx& c) {
     HIPCHK(c.efa.upload(ef.data(), ef.size(), c.stream));
     HIPCHK(c.qnam.upload(qa.data(), qa.size(), c.stream));
     HIPCHK(c.qnph.upload(qp.data(), qp.size(), c.stream));
+    c.tzn = tot_o * c.nbfr * 2;
+    if (c.tzn > 0) {
+        std::vector<uint16_t> os(tot_o);
+        for (int s = 0; s < c.S; s++)
+            for (int L = c.ooff[s]; L < c.ooff[s + 1]; L++) os[L] = (uint16_t)s;
+        HIPCHK(c.osid.upload(os.data(), os.size(), c.stream));
+        HIPCHK(c.tz.alloc((size_t)c.rk.nrk * c.tzn));
+    }
     const size_t rows = (size_t)kTableChunk * c.rk.nrk;
     HIPCHK(c.argj.alloc((size_t)c.S * rows * std::max(c.nbfr, 1)));
     HIPCHK(c.rff.alloc((size_t)c.S * rows * std::max(c.nbfr, 1)));
@@ -828,6 +840,17 @@ int launch_stage_v(Ctx& c, const StageArgs& a0, bool sph, bool fuse) {
     return 0;
 }
 
+// The step's tidal terms for all stages (before the step's stage launches; the forcing rows of the
+// step must be in the tables).
+int launch_tide(Ctx& c, bool capturing) {
+    if (c.tzn == 0) return 0;
+    const int n = c.rk.nrk * c.tzn;
+    hipLaunchKernelGGL(tide_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, forcing(c, capturing),
+                       (const uint16_t*)(c.S > 1 ? c.osid.p : nullptr), c.tz.p);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 // Nonlinear runs (NOLICAT != 0) use the kernels with NLEQ/LEQ folded at compile time; linearised
 // runs use the general kernels with 64-thread blocks.
 template <int P, int NT>
@@ -907,6 +930,7 @@ int launch_limiter(Ctx& c, double* u, const ElemData& ed, bool capturing) {
 // limiter is on) and the device step-counter increment.
 template <int P>
 int launch_step(Ctx& c, bool capturing) {
+    if (launch_tide(c, capturing)) return 1;
     const int np = c.np;
     const int B = 256;
     const int gn = (np + B - 1) / B;
@@ -963,6 +987,7 @@ int launch_step_explicit(Ctx& c, double* const* st, double* const* wsave = nullp
     const bool sph = c.prm.ics == 2;
     const bool wdon = c.prm.nolifa >= 2, lim = c.prm.slopeflag != 0, fuse = wdon && !lim;
     const size_t ssz = (size_t)3 * c.dof * c.ne;
+    if (launch_tide(c, false)) return 1;
     if (c.met) {
         hipLaunchKernelGGL(met_kernel, dim3((c.np + 255) / 256), dim3(256), 0, c.stream, met_args(c), nullptr, c.hstep);
         HIPCHK(hipGetLastError());
@@ -1173,6 +1198,7 @@ int adj_backward(Ctx& c, int it0, int nsteps, int interval, double weight) {
         }
         for (int k = k1; k > k0; k--) {
             c.hstep = k - k0 - 1;
+            if (launch_tide(c, false)) return 1;   // the step's tidal terms for the adjoint stages
             if (c.met) {   // the step's node forcing for the adjoint stages
                 hipLaunchKernelGGL(met_kernel, dim3((c.np + 255) / 256), dim3(256), 0, c.stream, met_args(c), nullptr,
                                    c.hstep);
@@ -1736,6 +1762,7 @@ int dg_gpu_autotune(int h, int* block) {
     const bool sph = c->prm.ics == 2, fuse = c->prm.nolifa >= 2 && c->prm.slopeflag == 0;
     const std::vector<int> cand = c->prm.p == 1 ? std::vector<int>{64, 128, 256} : std::vector<int>{64, 128};
     if (build_stage_tables(*c, c->next_it, 1)) return 1;
+    if (launch_tide(*c, false)) return 1;
     if (c->met) {
         hipLaunchKernelGGL(met_kernel, dim3((c->np + 255) / 256), dim3(256), 0, c->stream, met_args(*c),
                            nullptr, 0);

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "dg_crmath.hpp"
+
 namespace dg {
 
 constexpr int KMAXDOF = 6;
@@ -94,6 +96,10 @@ struct Forcing {
     const double* __restrict__ fargj;   // [nstage][nffr]
     const double* __restrict__ frff;    // [nstage][nffr]
     const double* __restrict__ ramp;    // [scenario][row] RAMPDG of each stage (meteorological forcing)
+    // tidal elevation terms EMO*RFF*COS(ARGJ-EFA) of the current step, [stage][needs][nbfr][2]
+    // (tzn = needs*nbfr*2 per stage), written by tide_kernel before the step's stages
+    const double* __restrict__ tz;
+    int tzn;
 };
 
 // Step index within the current forcing tables.

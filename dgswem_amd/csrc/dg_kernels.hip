@@ -443,13 +443,11 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
                 } else if (kind == SIDE_OCEAN) {
                     double ze_ex = 0.0;
                     const double xe = T.xegp[I];
+                    // the constituents' endpoint terms EMO*RFF*COS(ARGJ-EFA) of this stage (tide_kernel)
                     for (int jj = 0; jj < a.fo.nbfr; jj++) {
-                        const size_t t = ((size_t)sid * a.fo.rows + row) * a.fo.nbfr + jj;
-                        const double argj = a.fo.argj[t];
-                        const double rff = a.fo.rff[t];
                         const size_t o = ((size_t)nb * a.fo.nbfr + jj) * 2;
-                        const double z1 = a.fo.emo[o] * rff * cos(argj - a.fo.efa[o]);
-                        const double z2 = a.fo.emo[o + 1] * rff * cos(argj - a.fo.efa[o + 1]);
+                        const double z1 = a.fo.tz[(size_t)a.irk * a.fo.tzn + o];
+                        const double z2 = a.fo.tz[(size_t)a.irk * a.fo.tzn + o + 1];
                         ze_ex = ze_ex + (0.5 * (z1 + z2) + 0.5 * (z2 - z1) * xe);
                     }
                     if ((ze_ex * ph.ifnlfa + hb_in) <= 0) ze_ex = fabs(hb_in) + sc.h0;
@@ -604,8 +602,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
             const double hx = huv * sfq, hy = hvv + gh2;
             double fric = sc.fric;
             if (ph.load_manning) {
-                // DEPTH**(1/3) (src/rhs_dg_hydro.F:302); cbrt is within an ulp of pow(x, 1/3)
-                fric = ph.g * (nbar * nbar) / cbrt(depth);
+                // DEPTH**(1.D0/3.D0) (src/rhs_dg_hydro.F:302), correctly rounded
+                fric = ph.g * (nbar * nbar) / crm::cr_pow_third(depth);
                 if (fric < sc.cf) fric = sc.cf;
             }
             double tau = fric;
@@ -692,6 +690,24 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
     for (int f = 0; f < 3; f++)
 #pragma unroll
         for (int k = 0; k < DOF; k++) a.uout[(size_t)(f * DOF + k) * ne + e] = W[f][k];
+}
+
+// ------------------------------------------------------------------------------------------
+// Tidal forcing of one step at the ocean boundary (src/ocean_edge_hydro.F:459-488): for every
+// stage, ocean edge, constituent and end node the term EMO*RFF*COS(ARGJ - EFA) that the edge's
+// elevation sums, with the cosine correctly rounded.  One thread per term; the stage kernels
+// read the stage's terms.
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) tide_kernel(Forcing fo, const uint16_t* __restrict__ osid,
+                                                   double* __restrict__ tz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fo.nrk * fo.tzn) return;
+    const int irk = i / fo.tzn, o = i - irk * fo.tzn;       // o = (edge*nbfr + jj)*2 + end
+    const int jj = (o >> 1) % fo.nbfr, edge = (o >> 1) / fo.nbfr;
+    const int sid = osid ? (int)osid[edge] : 0;
+    const int row = table_step(fo) * fo.nrk + irk;
+    const size_t t = ((size_t)sid * fo.rows + row) * fo.nbfr + jj;
+    tz[i] = fo.emo[o] * fo.rff[t] * crm::cr_cos(fo.argj[t] - fo.efa[o]);
 }
 
 // ------------------------------------------------------------------------------------------
