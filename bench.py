@@ -148,6 +148,44 @@ def adjoint_leg(gpu, mesh, n_elem: int, steps: int, seed: int):
                         "per-node Manning n and depth"}
 
 
+def batched_adjoint_leg(mesh, n_members: int, n_elem: int, steps: int, seed: int):
+    """Per-member gauge-misfit gradients (Manning n and depth) of a p=2 storm ensemble: every member
+    a seeded spherical basin under its own Holland vortex with its own gauges and perturbed-Manning
+    twin data, slope limiter and wetting/drying on with their active sets frozen from the forward
+    run, all members in one batched forward and one batched checkpointed reverse sweep:
+    wall(forward + adjoint) / wall(forward)."""
+    import numpy as np
+    from dgswem_amd import adjoint
+    t0 = time.perf_counter()
+    cases = [mesh.storm_basin(n_elem, seed=seed + 300 + i, nsteps=steps, p=2, slopeflag=5, wetdry=True,
+                              shelf=False, cfl=0.15, depth_range=(10.0, 300.0), cf=1e-4)
+             for i in range(n_members)]
+    eprob, probs = adjoint.ensemble_problem(cases, nsteps=steps, ng=20, seed=seed + 400)
+    for p in probs:
+        p.solver.close()
+    mn = np.concatenate([c.manning for c in cases])
+    dp = np.concatenate([c.dp for c in cases])
+    log(f"batched adjoint: {n_members} members, {sum(c.ne for c in cases)} elements, p=2 "
+        f"({time.perf_counter() - t0:.1f} s to set up)")
+    eprob.solver.autotune()
+    eprob.loss(mn, dp)                              # warm-up
+    t0 = time.perf_counter()
+    eprob.loss(mn, dp)
+    fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J, gn, gdp = eprob.gradient(mn, dp)
+    grad = time.perf_counter() - t0
+    finite = bool(np.isfinite(gn).all() and np.isfinite(gdp).all())
+    eprob.solver.close()
+    log(f"batched adjoint: forward {fwd:.3f} s, forward+adjoint {grad:.3f} s, ratio {grad / fwd:.2f}")
+    return {"cost_ratio": grad / fwd, "forward_s": fwd, "gradient_s": grad, "members": n_members,
+            "elements": int(sum(c.ne for c in cases)), "p": 2, "steps": steps,
+            "checkpoint_interval": max(1, int(round(steps ** 0.5))), "loss": J, "gradient_finite": finite,
+            "workload": f"{n_members} spherical storm basins of {n_elem} elements, p=2, limiter 5 and "
+                        "wetting/drying on (active sets frozen), own Holland vortex, 20 gauges and "
+                        "perturbed-Manning twin each; per-member gradients w.r.t. Manning n and depth"}
+
+
 def copy_calibration() -> int:
     """2 GiB (larger than the 256 MiB last-level cache) copied device to device three times."""
     import torch
@@ -180,6 +218,10 @@ def main() -> int:
     ap.add_argument("--adj-steps", type=int, default=2000)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--badj-members", type=int, default=32)
+    ap.add_argument("--badj-elements", type=int, default=100_000)
+    ap.add_argument("--badj-steps", type=int, default=1000)
+    ap.add_argument("--no-batched-adjoint", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 2 GiB device buffer three times (a plain copy kernel of known "
                          "size, to scale the FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
@@ -250,6 +292,8 @@ def main() -> int:
                                                      10, args.seed)
     adj = None if args.no_adjoint else adjoint_leg(gpu, mesh, args.adj_elements, args.adj_steps,
                                                    args.seed)
+    badj = None if args.no_batched_adjoint else batched_adjoint_leg(mesh, args.badj_members, args.badj_elements,
+                                                                    args.badj_steps, args.seed)
 
     out = {
         "metric": "DOF-stage updates/s FP64 p=1 1M elems; ensemble speedup; adjoint/forward cost",
@@ -278,6 +322,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "ensemble": ens,
         "adjoint": adj,
+        "batched_adjoint": badj,
     }
     if cpu:
         out["speedup_vs_cpu_baseline"] = value / cpu["value"]
