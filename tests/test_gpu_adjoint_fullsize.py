@@ -3,8 +3,7 @@ basin (GRF depth 5-300 m, CF 1e-4, no limiter), 2000 steps with sqrt(N) checkpoi
 misfit against a perturbed-Manning twin: the benchmark's adjoint workload itself.
 
 The gradient with respect to per-node Manning n and depth must match fourth-order central
-differences of the same GPU forward model in four seeded directions to 1e-6 relative, the
-north-star bar.
+differences of the same GPU forward model in four seeded directions to 1e-6 relative.
 """
 import numpy as np
 import pytest

@@ -3,7 +3,7 @@
 Batching, graph replay and the block size change only how launches are grouped, never the
 arithmetic of an element, so every comparison with the single-run eager path is bit for bit.
 The first member is the reference fixture `delaunay4k`, checked against dgswem_serial's FP64 state
-to the north-star tolerance (relative L2 <= 1e-10, identical wet/dry mask).
+to the parity tolerance (relative L2 <= 1e-10, identical wet/dry mask).
 """
 import numpy as np
 import pytest

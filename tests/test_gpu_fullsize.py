@@ -4,7 +4,7 @@ GPU, checked through properties that do not depend on size.
 * Against the oracle for a short run: p=1 with limiter 5, wetting/drying and five tidal
   constituents for 100 steps, and p=2 for 20 steps (the oracle's OpenMP build finishes each in
   seconds): relative L2 <= 1e-10 in elevation and velocity and identical wet/dry masks, the
-  north-star bar at the size the benchmark measures.
+  parity bar, at the size the benchmark measures.
 * Mass: with every boundary made land, no tide and all elements wet, the total volume is conserved
   to round-off over 200 steps with the limiter on (the limiter keeps cell means; the edge fluxes of
   the two sides of an edge agree bit for bit).

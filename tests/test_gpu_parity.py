@@ -1,6 +1,6 @@
 """GPU step (through the C ABI) against the reference's FP64 states and the oracle.
 
-Tolerances: north-star parity is a relative L2 error <= 1e-10 in elevation and velocity with
+Tolerances: parity is a relative L2 error <= 1e-10 in elevation and velocity with
 identical wet/dry masks.  The GPU uses the device's cos and evaluates Manning's DEPTH**(1/3) as a
 cube root; both can differ from glibc's pow/cos in the last bit, so states that involve tides or
 Manning friction are not always bit-identical; all other arithmetic is rounded operation by
