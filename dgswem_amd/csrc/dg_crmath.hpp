@@ -141,11 +141,11 @@ DG_CRM_FN double cr_cos(double x) {
 // refined to double-double by one Newton step from the library's cbrt, and the second factor is
 // 1 - d*ln(x) to 2^-110.  The correction terms need only modest relative accuracy (the Newton
 // quotient 2^-45, ln(x) 2^-20: their errors enter at 2^-97 and 2^-75 of the result), so the device
-// forms them in single precision.  Outside that range: 0 for 0, NaN for negative x (as pow), the
+// forms them in single precision.  Outside that range as pow: +0 for +-0, NaN for negative x; the
 // library's cube root otherwise.
 DG_CRM_FN double cr_pow_third(double x) {
     const double c0 = std::cbrt(x);
-    if (!(x >= 0x1p-1000 && x <= 0x1p+1000)) return x < 0.0 ? (x - x) / (x - x) : c0;
+    if (!(x >= 0x1p-1000 && x <= 0x1p+1000)) return x < 0.0 ? (x - x) / (x - x) : (x == 0.0 ? 0.0 : c0);
     const dd s = two_prod(c0, c0);
     const dd t = two_prod(c0, s.hi);
     const double e = (t.hi - x) + (t.lo + c0 * s.lo);     // c0^3 - x (t.hi - x is exact)

@@ -105,7 +105,7 @@ def test_pow_third_special_arguments(lib):
     for c in (0.5, 1.0, 2.0, 3.0, 10.0, 1234.0):
         assert lib.t_pow3(c ** 3) == exact_pow3(c ** 3)
     assert lib.t_pow3(8.0) == 2.0 and lib.t_pow3(1000.0) == 9.999999999999998
-    assert lib.t_pow3(0.0) == 0.0
+    assert lib.t_pow3(0.0) == 0.0 and math.copysign(1.0, lib.t_pow3(-0.0)) == 1.0
     assert math.isnan(lib.t_pow3(-8.0))
     assert lib.t_pow3(1e-310) == pytest.approx(1e-310 ** (1.0 / 3.0), rel=1e-15)
 
