@@ -645,6 +645,14 @@ int init_ctx_common(Ctx& c, const dg_params* prm, bool manning) {
     for (int l = 0; l < 3; l++)
         for (int q = 0; q < c.T.negp; q++)
             if (c.T.phi_edge[l][q][0] != 1.0) return fail("basis tables: PHI_EDGE of mode 1 is not 1");
+    // the stage kernel forms the P1 bathymetry gradient once per element: the linear modes'
+    // derivative tables must be the same at every area point
+    for (int q = 0; q < c.T.nagp; q++)
+        if (c.T.phi_area[q][0] != 1.0) return fail("basis tables: PHI_AREA of mode 1 is not 1");
+    for (int q = 1; q < c.T.nagp; q++)
+        for (int k = 0; k < 3; k++)
+            if (c.T.drphi[q][k] != c.T.drphi[0][k] || c.T.dsphi[q][k] != c.T.dsphi[0][k])
+                return fail("basis tables: derivatives of the linear modes vary between area points");
     c.dof = c.T.dof; c.nagp = c.T.nagp; c.negp = c.T.negp;
     c.block = 128;
     {

@@ -568,6 +568,16 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
             const double fh_nl_c = NL ? 1.0 / depth_c : 1.0 / (ph.nleq * depth_c + ph.leq);
             hybf = pow(1.0 + pow(ph.hbreak * fh_nl_c, ph.ftheta), ph.fgamma / ph.ftheta);
         }
+        // the P1 bathymetry's gradient: the linear modes' derivative tables are the same at every
+        // area point (checked at set-up), so the sums are those of every point
+        double dhbx = 0.0, dhby = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const double dpxk = T.drphi[0][k] * drdx + T.dsphi[0][k] * dsdx;
+            const double dpyk = T.drphi[0][k] * drdy + T.dsphi[0][k] * dsdy;
+            dhbx = dhbx + hb[k] * dpxk;
+            dhby = dhby + hb[k] * dpyk;
+        }
 #pragma unroll 1
         for (int I = 0; I < NAGP; I++) {
             const double* pa = T.phi_area[I];
@@ -578,15 +588,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
                 qx = qx + U[1][k] * pa[k];
                 qy = qy + U[2][k] * pa[k];
             }
-            double hbq = 0.0, dhbx = 0.0, dhby = 0.0;
+            // PHI_AREA of mode 1 is 1 (checked at set-up): the sum from zero starts at hb[0]
+            double hbq = hb[0];
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const double dpxk = T.drphi[I][k] * drdx + T.dsphi[I][k] * dsdx;
-                const double dpyk = T.drphi[I][k] * drdy + T.dsphi[I][k] * dsdy;
-                hbq = hbq + hb[k] * pa[k];
-                dhbx = dhbx + hb[k] * dpxk;
-                dhby = dhby + hb[k] * dpyk;
-            }
+            for (int k = 1; k < 3; k++) hbq = hbq + hb[k] * pa[k];
             const double sfq = sfqv[I];
             const double depth = ze + hbq;
             // with NL (NLEQ = 1, LEQ = 0) the constant factors fold exactly
