@@ -900,46 +900,22 @@ __global__ void __launch_bounds__(256) limiter_adj_node_kernel(int np, int ne, c
     constexpr int DOF = (P + 1) * (P + 2) / 2;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= np) return;
-    double lo[3], hi[3], slo[3], shi[3];
-    int alo[3], ahi[3];
-#pragma unroll
-    for (int f = 0; f < 3; f++) { lo[f] = 99999.; hi[f] = -99999.; slo[f] = 0.0; shi[f] = 0.0; alo[f] = -1; ahi[f] = -1; }
-    const int j1 = off[n + 1];
-    // up to eight incidences per pass, all their loads in flight together; past the node's last
-    // incidence a pass repeats that incidence, which the strict comparisons ignore and the sums
-    // skip, so the results are those of the plain loop in list order
-    constexpr int B = 8;
-    for (int j0 = off[n]; j0 < j1; j0 += B) {
-        int ev[B], vv[B];
-#pragma unroll
-        for (int q = 0; q < B; q++) {
-            const int x = n2e[min(j0 + q, j1 - 1)];
-            ev[q] = x >> 2;
-            vv[q] = x & 3;
-        }
-#pragma unroll
-        for (int f = 0; f < 3; f++) {
-            double c[B], bl[B], bh[B];
-#pragma unroll
-            for (int q = 0; q < B; q++) {
-                c[q] = W[(size_t)(f * DOF) * ne + ev[q]];
-                bl[q] = nbc[((size_t)(f * 2) * 3 + vv[q]) * ne + ev[q]];
-                bh[q] = nbc[((size_t)(f * 2 + 1) * 3 + vv[q]) * ne + ev[q]];
-            }
-#pragma unroll
-            for (int q = 0; q < B; q++) {
-                if (c[q] < lo[f]) { lo[f] = c[q]; alo[f] = ev[q]; }
-                if (c[q] > hi[f]) { hi[f] = c[q]; ahi[f] = ev[q]; }
-                if (j0 + q < j1) { slo[f] += bl[q]; shi[f] += bh[q]; }
-            }
-        }
-    }
 #pragma unroll
     for (int f = 0; f < 3; f++) {
-        gsum[(size_t)(f * 2) * np + n] = slo[f];
-        gsum[(size_t)(f * 2 + 1) * np + n] = shi[f];
-        arg[(size_t)(f * 2) * np + n] = alo[f];
-        arg[(size_t)(f * 2 + 1) * np + n] = ahi[f];
+        double lo = 99999., hi = -99999., slo = 0.0, shi = 0.0;
+        int alo = -1, ahi = -1;
+        for (int j = off[n]; j < off[n + 1]; j++) {
+            const int e = n2e[j] >> 2, v = n2e[j] & 3;
+            const double c = W[(size_t)(f * DOF) * ne + e];
+            if (c < lo) { lo = c; alo = e; }
+            if (c > hi) { hi = c; ahi = e; }
+            slo += nbc[((size_t)(f * 2) * 3 + v) * ne + e];
+            shi += nbc[((size_t)(f * 2 + 1) * 3 + v) * ne + e];
+        }
+        gsum[(size_t)(f * 2) * np + n] = slo;
+        gsum[(size_t)(f * 2 + 1) * np + n] = shi;
+        arg[(size_t)(f * 2) * np + n] = alo;
+        arg[(size_t)(f * 2 + 1) * np + n] = ahi;
     }
 }
 
