@@ -8,10 +8,13 @@
 //                    order, so both see bit-identical values and no scatter or atomics are needed.
 //   node_minmax      one thread per node: min/max of the updated cell means over incident elements
 //   limit_wetdry     one thread per element: SLOPELIMITER5 on modes 2-3, then WETDRY
-// plus modal2nodal (one thread per node) on output steps.
+// preceded once per step by tide_kernel (the tidal boundary terms of every stage) and, with NWS=8,
+// met_kernel (the Holland profile at the nodes); plus modal2nodal (one thread per node) on output
+// steps.
 //
 // Arithmetic follows the reference statement by statement (listed per kernel) and is compiled with
-// -ffp-contract=off, so each product and sum is rounded as in dgswem_serial.
+// -ffp-contract=off, so each product and sum is rounded as in dgswem_serial; the cosine of the
+// tidal terms and Manning's DEPTH**(1/3) are correctly rounded (dg_crmath.hpp).
 #include "dg_device.hpp"
 #include "dg_setup.hpp"
 
