@@ -779,10 +779,12 @@ __global__ void __launch_bounds__(64) gauge_seed_kernel(int nu, int ne, const in
 // branch every clamp, every redistribution test and every min(redfac, redmax) took; given those,
 // the limiter is linear in (the element's linear modes, the node bounds) and its transpose maps
 // the cotangent of the limited modes (in ub, in place) to the cotangent of W's modes, plus the
-// cotangents of the vertex bounds (nbc [field][min|max][vertex][ne]).  Modes >= 3 pass through.
+// cotangents of the vertex bounds (nbc: a record of six doubles [field][min|max] per element and
+// vertex, record 3e+v, so that a node reads each incident element's six values from one place).
+// Modes >= 3 pass through.
 // ------------------------------------------------------------------------------------------
-template <int P>
-__global__ void __launch_bounds__(256) limiter_adj_kernel(int np, ElemData E, const double* __restrict__ W,
+template <int P, int NB = 256>
+__global__ void __launch_bounds__(NB) limiter_adj_kernel(int np, ElemData E, const double* __restrict__ W,
                                                           const double* __restrict__ mm, double* __restrict__ ub,
                                                           double* __restrict__ nbc) {
     // no contraction: the recorded branches must be those of the forward limiter, operation by
@@ -790,132 +792,159 @@ __global__ void __launch_bounds__(256) limiter_adj_kernel(int np, ElemData E, co
     constexpr int DOF = (P + 1) * (P + 2) / 2;
     const DevTables& T = tab<P>();
     const int ne = E.ne;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ne) return;
-    const int nn[3] = {E.nm[e], E.nm[(size_t)ne + e], E.nm[2 * (size_t)ne + e]};
+    const int tid = threadIdx.x, e0 = blockIdx.x * NB;
+    const int e = e0 + tid;
+    // the block's bound-cotangent records, written out together after the element loop: the
+    // block's elements hold one contiguous range of records
+    __shared__ double rec[NB * 18];
+    if (e < ne) {
+        const int nn[3] = {E.nm[e], E.nm[(size_t)ne + e], E.nm[2 * (size_t)ne + e]};
 #pragma unroll 1
-    for (int iv = 0; iv < 3; iv++) {
-        const double* mn = mm + (size_t)(2 * iv) * np;
-        const double* mx = mm + (size_t)(2 * iv + 1) * np;
-        const double vmin[3] = {mn[nn[0]], mn[nn[1]], mn[nn[2]]}, vmax[3] = {mx[nn[0]], mx[nn[1]], mx[nn[2]]};
-        const double c0 = W[(size_t)(iv * DOF) * ne + e];
-        const double w1 = W[(size_t)(iv * DOF + 1) * ne + e], w2 = W[(size_t)(iv * DOF + 2) * ne + e];
-        // forward, recording the branches
-        int sel[3];
-        double v[3];
+        for (int iv = 0; iv < 3; iv++) {
+            const double* mn = mm + (size_t)(2 * iv) * np;
+            const double* mx = mm + (size_t)(2 * iv + 1) * np;
+            const double vmin[3] = {mn[nn[0]], mn[nn[1]], mn[nn[2]]}, vmax[3] = {mx[nn[0]], mx[nn[1]], mx[nn[2]]};
+            const double c0 = W[(size_t)(iv * DOF) * ne + e];
+            const double w1 = W[(size_t)(iv * DOF + 1) * ne + e], w2 = W[(size_t)(iv * DOF + 2) * ne + e];
+            // forward, recording the branches
+            int sel[3];
+            double v[3];
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            double t = c0;
-            t = t + T.phi_corner[c][1] * w1;
-            t = t + T.phi_corner[c][2] * w2;
-            double x = t;
-            sel[c] = 0;
-            if (!(t <= vmax[c])) { x = vmax[c]; sel[c] = 1; }
-            if (x < vmin[c]) { x = vmin[c]; sel[c] = 2; }
-            v[c] = x;
-        }
-        double sgs[3], divs[3][3];
-        int pos = 0, usemax = 0;
-#pragma unroll
-        for (int ll = 0; ll < 3; ll++) {
-            const double sumloc = (v[0] + v[1] + v[2]) / 3.0;
-            double sumdif = (sumloc - c0) * 3.0;
-            const double sg = copysign(1.0, sumdif);
-            sgs[ll] = sg;
-            double dif[3];
-#pragma unroll
-            for (int c = 0; c < 3; c++) dif[c] = (v[c] - c0) * sg;
-            int kdp = (dif[0] > 0) + (dif[1] > 0) + (dif[2] > 0);
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const double div = fmax(1.0, (double)kdp);
-                divs[ll][k] = div;
-                double redfac = 0.0;
-                if (dif[k] > 0) { redfac = sumdif * sg / div; kdp = kdp - 1; pos |= 1 << (3 * ll + k); }
-                const double redmax = (sg > 0) ? v[k] - vmin[k] : vmax[k] - v[k];
-                if (!(redfac <= redmax)) { redfac = redmax; usemax |= 1 << (3 * ll + k); }
-                sumdif = sumdif - redfac * sg;
-                v[k] = v[k] - redfac * sg;
+            for (int c = 0; c < 3; c++) {
+                double t = c0;
+                t = t + T.phi_corner[c][1] * w1;
+                t = t + T.phi_corner[c][2] * w2;
+                double x = t;
+                sel[c] = 0;
+                if (!(t <= vmax[c])) { x = vmax[c]; sel[c] = 1; }
+                if (x < vmin[c]) { x = vmin[c]; sel[c] = 2; }
+                v[c] = x;
             }
-        }
-        // reverse: U1 = -(v0+v1)/6 + v2/3, U2 = (v1-v0)/2, U0 = c0
-        const double u1b = ub[(size_t)(iv * DOF + 1) * ne + e], u2b = ub[(size_t)(iv * DOF + 2) * ne + e];
-        double vb[3] = {-(1.0 / 6.0) * u1b - 0.5 * u2b, -(1.0 / 6.0) * u1b + 0.5 * u2b, (1.0 / 3.0) * u1b};
-        double c0b = ub[(size_t)(iv * DOF) * ne + e];
-        double vminb[3] = {0.0, 0.0, 0.0}, vmaxb[3] = {0.0, 0.0, 0.0};
+            double sgs[3], divs[3][3];
+            int pos = 0, usemax = 0;
 #pragma unroll
-        for (int ll = 2; ll >= 0; ll--) {
-            const double sg = sgs[ll];
-            double sb = 0.0;
+            for (int ll = 0; ll < 3; ll++) {
+                const double sumloc = (v[0] + v[1] + v[2]) / 3.0;
+                double sumdif = (sumloc - c0) * 3.0;
+                const double sg = copysign(1.0, sumdif);
+                sgs[ll] = sg;
+                double dif[3];
 #pragma unroll
-            for (int k = 2; k >= 0; k--) {
-                // v_k -= redfac*sg; sumdif -= redfac*sg
-                const double rb = -sg * vb[k] - sg * sb;
-                if (usemax & (1 << (3 * ll + k))) {
-                    // redfac = redmax = sg > 0 ? v_k - vmin_k : vmax_k - v_k
-                    if (sg > 0) { vb[k] += rb; vminb[k] -= rb; }
-                    else { vmaxb[k] += rb; vb[k] -= rb; }
-                } else if (pos & (1 << (3 * ll + k))) {
-                    sb += rb * sg / divs[ll][k];      // redfac = sumdif*sg/div
+                for (int c = 0; c < 3; c++) dif[c] = (v[c] - c0) * sg;
+                int kdp = (dif[0] > 0) + (dif[1] > 0) + (dif[2] > 0);
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const double div = fmax(1.0, (double)kdp);
+                    divs[ll][k] = div;
+                    double redfac = 0.0;
+                    if (dif[k] > 0) { redfac = sumdif * sg / div; kdp = kdp - 1; pos |= 1 << (3 * ll + k); }
+                    const double redmax = (sg > 0) ? v[k] - vmin[k] : vmax[k] - v[k];
+                    if (!(redfac <= redmax)) { redfac = redmax; usemax |= 1 << (3 * ll + k); }
+                    sumdif = sumdif - redfac * sg;
+                    v[k] = v[k] - redfac * sg;
                 }
             }
-            // sumdif = ((v0+v1+v2)/3 - c0)*3
-            vb[0] += sb; vb[1] += sb; vb[2] += sb;
-            c0b += -3.0 * sb;
-        }
-        double w1b = 0.0, w2b = 0.0;
+            // reverse: U1 = -(v0+v1)/6 + v2/3, U2 = (v1-v0)/2, U0 = c0
+            const double u1b = ub[(size_t)(iv * DOF + 1) * ne + e], u2b = ub[(size_t)(iv * DOF + 2) * ne + e];
+            double vb[3] = {-(1.0 / 6.0) * u1b - 0.5 * u2b, -(1.0 / 6.0) * u1b + 0.5 * u2b, (1.0 / 3.0) * u1b};
+            double c0b = ub[(size_t)(iv * DOF) * ne + e];
+            double vminb[3] = {0.0, 0.0, 0.0}, vmaxb[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            if (sel[c] == 0) {
-                c0b += vb[c];
-                w1b += T.phi_corner[c][1] * vb[c];
-                w2b += T.phi_corner[c][2] * vb[c];
-            } else if (sel[c] == 1) {
-                vmaxb[c] += vb[c];
-            } else {
-                vminb[c] += vb[c];
+            for (int ll = 2; ll >= 0; ll--) {
+                const double sg = sgs[ll];
+                double sb = 0.0;
+#pragma unroll
+                for (int k = 2; k >= 0; k--) {
+                    // v_k -= redfac*sg; sumdif -= redfac*sg
+                    const double rb = -sg * vb[k] - sg * sb;
+                    if (usemax & (1 << (3 * ll + k))) {
+                        // redfac = redmax = sg > 0 ? v_k - vmin_k : vmax_k - v_k
+                        if (sg > 0) { vb[k] += rb; vminb[k] -= rb; }
+                        else { vmaxb[k] += rb; vb[k] -= rb; }
+                    } else if (pos & (1 << (3 * ll + k))) {
+                        sb += rb * sg / divs[ll][k];      // redfac = sumdif*sg/div
+                    }
+                }
+                // sumdif = ((v0+v1+v2)/3 - c0)*3
+                vb[0] += sb; vb[1] += sb; vb[2] += sb;
+                c0b += -3.0 * sb;
+            }
+            double w1b = 0.0, w2b = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                if (sel[c] == 0) {
+                    c0b += vb[c];
+                    w1b += T.phi_corner[c][1] * vb[c];
+                    w2b += T.phi_corner[c][2] * vb[c];
+                } else if (sel[c] == 1) {
+                    vmaxb[c] += vb[c];
+                } else {
+                    vminb[c] += vb[c];
+                }
+            }
+            ub[(size_t)(iv * DOF) * ne + e] = c0b;
+            ub[(size_t)(iv * DOF + 1) * ne + e] = w1b;
+            ub[(size_t)(iv * DOF + 2) * ne + e] = w2b;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                rec[tid * 18 + c * 6 + iv * 2] = vminb[c];
+                rec[tid * 18 + c * 6 + iv * 2 + 1] = vmaxb[c];
             }
         }
-        ub[(size_t)(iv * DOF) * ne + e] = c0b;
-        ub[(size_t)(iv * DOF + 1) * ne + e] = w1b;
-        ub[(size_t)(iv * DOF + 2) * ne + e] = w2b;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            nbc[((size_t)(iv * 2) * 3 + c) * ne + e] = vminb[c];
-            nbc[((size_t)(iv * 2 + 1) * 3 + c) * ne + e] = vmaxb[c];
-        }
     }
+    __syncthreads();
+    const int nvalid = min(NB, ne - e0) * 18;
+    double* dst = nbc + (size_t)e0 * 18;
+    for (int k = tid; k < nvalid; k += NB) dst[k] = rec[k];
 }
 
 // Per node: the sums of the incident elements' vertex-bound cotangents, and the element whose
 // cell mean attains each bound (the first in the node's element list on ties; -1 when the
-// +-99999 starting bound stands).  gsum, arg: [field][min|max][np].
+// +-99999 starting bound stands).  gsum, arg: a record [field][min|max] of six per node.  The three
+// fields go through the node's list together: every incident element's cell means and its bound
+// cotangent record (nbc) are loaded in one pass, and each field's sums still run in list order.
 template <int P>
 __global__ void __launch_bounds__(256) limiter_adj_node_kernel(int np, int ne, const int* __restrict__ off,
                                                                const int* __restrict__ n2e,
                                                                const double* __restrict__ W,
                                                                const double* __restrict__ nbc,
                                                                double* __restrict__ gsum, int* __restrict__ arg) {
-#pragma clang fp contract(fast)
     constexpr int DOF = (P + 1) * (P + 2) / 2;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= np) return;
+    double lo[3], hi[3], slo[3], shi[3];
+    int alo[3], ahi[3];
 #pragma unroll
     for (int f = 0; f < 3; f++) {
-        double lo = 99999., hi = -99999., slo = 0.0, shi = 0.0;
-        int alo = -1, ahi = -1;
-        for (int j = off[n]; j < off[n + 1]; j++) {
-            const int e = n2e[j] >> 2, v = n2e[j] & 3;
-            const double c = W[(size_t)(f * DOF) * ne + e];
-            if (c < lo) { lo = c; alo = e; }
-            if (c > hi) { hi = c; ahi = e; }
-            slo += nbc[((size_t)(f * 2) * 3 + v) * ne + e];
-            shi += nbc[((size_t)(f * 2 + 1) * 3 + v) * ne + e];
+        lo[f] = 99999.; hi[f] = -99999.; slo[f] = 0.0; shi[f] = 0.0;
+        alo[f] = -1; ahi[f] = -1;
+    }
+    const int j1 = off[n + 1];
+    for (int j = off[n]; j < j1; j++) {
+        const int ent = n2e[j];
+        const int e = ent >> 2, v = ent & 3;
+        const double2* rec = reinterpret_cast<const double2*>(nbc + ((size_t)e * 3 + v) * 6);
+        double c[3];
+        double2 b[3];
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            c[f] = W[(size_t)(f * DOF) * ne + e];
+            b[f] = rec[f];
         }
-        gsum[(size_t)(f * 2) * np + n] = slo;
-        gsum[(size_t)(f * 2 + 1) * np + n] = shi;
-        arg[(size_t)(f * 2) * np + n] = alo;
-        arg[(size_t)(f * 2 + 1) * np + n] = ahi;
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            if (c[f] < lo[f]) { lo[f] = c[f]; alo[f] = e; }
+            if (c[f] > hi[f]) { hi[f] = c[f]; ahi[f] = e; }
+            slo[f] += b[f].x;
+            shi[f] += b[f].y;
+        }
+    }
+    double2* g = reinterpret_cast<double2*>(gsum + (size_t)n * 6);
+    int2* a = reinterpret_cast<int2*>(arg + (size_t)n * 6);
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        g[f] = make_double2(slo[f], shi[f]);
+        a[f] = make_int2(alo[f], ahi[f]);
     }
 }
 
@@ -929,19 +958,24 @@ __global__ void __launch_bounds__(256) limiter_adj_gather_kernel(int np, ElemDat
     if (e >= ne) return;
     // the node indices, then every node's bound owners and cotangent sums together (a sum that is
     // not this element's adds zero, which leaves the running sum as it is)
-    int nv[3];
+    int2 a[3][3];
+    double2 g[3][3];
 #pragma unroll
-    for (int v = 0; v < 3; v++) nv[v] = E.nm[(size_t)v * ne + e];
+    for (int v = 0; v < 3; v++) {
+        const int n = E.nm[(size_t)v * ne + e];
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            a[v][f] = reinterpret_cast<const int2*>(arg + (size_t)n * 6)[f];
+            g[v][f] = reinterpret_cast<const double2*>(gsum + (size_t)n * 6)[f];
+        }
+    }
 #pragma unroll
     for (int f = 0; f < 3; f++) {
         double add = 0.0;
 #pragma unroll
         for (int v = 0; v < 3; v++) {
-            const int n = nv[v];
-            const int alo = arg[(size_t)(f * 2) * np + n], ahi = arg[(size_t)(f * 2 + 1) * np + n];
-            const double glo = gsum[(size_t)(f * 2) * np + n], ghi = gsum[(size_t)(f * 2 + 1) * np + n];
-            add += alo == e ? glo : 0.0;
-            add += ahi == e ? ghi : 0.0;
+            add += a[v][f].x == e ? g[v][f].x : 0.0;
+            add += a[v][f].y == e ? g[v][f].y : 0.0;
         }
         ub[(size_t)(f * DOF) * ne + e] += add;
     }
