@@ -799,13 +799,25 @@ __global__ void __launch_bounds__(NB) limiter_adj_kernel(int np, ElemData E, con
     __shared__ double rec[NB * 18];
     if (e < ne) {
         const int nn[3] = {E.nm[e], E.nm[(size_t)ne + e], E.nm[2 * (size_t)ne + e]};
-#pragma unroll 1
+        // every load of the three fields first (node bounds, linear modes, their cotangents), so
+        // that their latencies overlap
+        double bmn[3][3], bmx[3][3], wm[3][3], wb[3][3];
+#pragma unroll
         for (int iv = 0; iv < 3; iv++) {
             const double* mn = mm + (size_t)(2 * iv) * np;
             const double* mx = mm + (size_t)(2 * iv + 1) * np;
-            const double vmin[3] = {mn[nn[0]], mn[nn[1]], mn[nn[2]]}, vmax[3] = {mx[nn[0]], mx[nn[1]], mx[nn[2]]};
-            const double c0 = W[(size_t)(iv * DOF) * ne + e];
-            const double w1 = W[(size_t)(iv * DOF + 1) * ne + e], w2 = W[(size_t)(iv * DOF + 2) * ne + e];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                bmn[iv][c] = mn[nn[c]];
+                bmx[iv][c] = mx[nn[c]];
+                wm[iv][c] = W[(size_t)(iv * DOF + c) * ne + e];
+                wb[iv][c] = ub[(size_t)(iv * DOF + c) * ne + e];
+            }
+        }
+#pragma unroll
+        for (int iv = 0; iv < 3; iv++) {
+            const double vmin[3] = {bmn[iv][0], bmn[iv][1], bmn[iv][2]}, vmax[3] = {bmx[iv][0], bmx[iv][1], bmx[iv][2]};
+            const double c0 = wm[iv][0], w1 = wm[iv][1], w2 = wm[iv][2];
             // forward, recording the branches
             int sel[3];
             double v[3];
@@ -845,9 +857,9 @@ __global__ void __launch_bounds__(NB) limiter_adj_kernel(int np, ElemData E, con
                 }
             }
             // reverse: U1 = -(v0+v1)/6 + v2/3, U2 = (v1-v0)/2, U0 = c0
-            const double u1b = ub[(size_t)(iv * DOF + 1) * ne + e], u2b = ub[(size_t)(iv * DOF + 2) * ne + e];
+            const double u1b = wb[iv][1], u2b = wb[iv][2];
             double vb[3] = {-(1.0 / 6.0) * u1b - 0.5 * u2b, -(1.0 / 6.0) * u1b + 0.5 * u2b, (1.0 / 3.0) * u1b};
-            double c0b = ub[(size_t)(iv * DOF) * ne + e];
+            double c0b = wb[iv][0];
             double vminb[3] = {0.0, 0.0, 0.0}, vmaxb[3] = {0.0, 0.0, 0.0};
 #pragma unroll
             for (int ll = 2; ll >= 0; ll--) {
