@@ -286,6 +286,21 @@ __device__ __forceinline__ void load_lambda(const AdjArgs& a, int ne, int el, do
     }
 }
 
+// Element g's P1 bathymetry modes and AREAS (twice the area), from its vertex depths and
+// coordinates, with the products and sums of the reference's set-up (src/read_input.F:1408,
+// src/prep_DG.F:959-1163) and of the forward stage kernel.
+__device__ __forceinline__ void elem_hb_areas(const ElemData& E, int ne, int g, double hb[3], double& areas) {
+#pragma clang fp contract(fast)
+    const double c13 = 1.0 / 3.0, c16 = 1.0 / 6.0;
+    const double d0 = E.dpv[g], d1 = E.dpv[(size_t)ne + g], d2 = E.dpv[2 * (size_t)ne + g];
+    hb[0] = c13 * (d0 + d1 + d2);
+    hb[1] = -(c16 * (d0 + d1)) + c13 * d2;
+    hb[2] = -(0.5 * d0) + 0.5 * d1;
+    const double X0 = E.xv[g], X1 = E.xv[(size_t)ne + g], X2 = E.xv[2 * (size_t)ne + g];
+    const double Y0 = E.yv[g], Y1 = E.yv[(size_t)ne + g], Y2 = E.yv[2 * (size_t)ne + g];
+    areas = (X0 - X2) * (Y1 - Y2) + (X2 - X1) * (Y0 - Y2);
+}
+
 // QF: quadratic bottom friction without the hybrid factor (NOLIBF=1), fixed at compile time.
 template <int P, int NT, bool SPH, bool NL, bool QF>
 __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a) {
@@ -295,9 +310,11 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
     constexpr int NEGP = P + 2;
     __shared__ double ub_lds[3 * DOF][NT];     // cotangent of this element's modes
     __shared__ double tpe[3][NEGP][DOF], teq[3][NEGP][DOF];   // edge tables (lane-divergent index)
-    // staged rows (ns entries each): state [3*DOF], RHS cotangent [3*DOF], bathymetry modes [3], AREAS
+    // staged rows (ns entries each): state [3*DOF], RHS cotangent [3*DOF].  The bathymetry modes
+    // and AREAS are formed from the vertex depths and coordinates where they are needed: staging
+    // them too would take the p=2 block past a quarter of the CU's LDS.
     extern __shared__ double stg[];
-    constexpr int RL = 3 * DOF, RH = 6 * DOF, RA = 6 * DOF + 3;
+    constexpr int RL = 3 * DOF;
     const DevTables& T = tab<P>();
     const ElemData& E = a.ed;
     const Phys& ph = a.ph;
@@ -327,13 +344,6 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
                 for (int f = 0; f < 3; f++)
 #pragma unroll
                     for (int k = 0; k < DOF; k++) stg[(RL + f * DOF + k) * ns + j] = L0[f][k];
-                const double d0 = E.dpv[g], d1 = E.dpv[(size_t)ne + g], d2 = E.dpv[2 * (size_t)ne + g];
-                stg[RH * ns + j] = c13 * (d0 + d1 + d2);
-                stg[(RH + 1) * ns + j] = -(c16 * (d0 + d1)) + c13 * d2;
-                stg[(RH + 2) * ns + j] = -(0.5 * d0) + 0.5 * d1;
-                const double X0 = E.xv[g], X1 = E.xv[(size_t)ne + g], X2 = E.xv[2 * (size_t)ne + g];
-                const double Y0 = E.yv[g], Y1 = E.yv[(size_t)ne + g], Y2 = E.yv[2 * (size_t)ne + g];
-                stg[RA * ns + j] = (X0 - X2) * (Y1 - Y2) + (X2 - X1) * (Y0 - Y2);
             }
         }
     }
@@ -355,9 +365,7 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
     for (int j = 0; j < 3 * DOF; j++) ub_lds[j][tid] = 0.0;
     double hbb[3] = {0.0, 0.0, 0.0};
     double AREAS, hb[3];
-    AREAS = stg[RA * ns + tid];
-#pragma unroll
-    for (int m = 0; m < 3; m++) hb[m] = stg[(RH + m) * ns + tid];
+    elem_hb_areas(E, ne, e, hb, AREAS);
     const int info = E.sinfo[e];
     const int nb0 = E.snbr[e], nb1 = E.snbr[(size_t)ne + e], nb2 = E.snbr[2 * (size_t)ne + e];
     const int sl = a.snbl[e];
@@ -383,10 +391,8 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
                     V[f][k] = stg[(f * DOF + k) * ns + ln];
                     Ln[f][k] = stg[(RL + f * DOF + k) * ns + ln];
                 }
-            double hbn[3];
-#pragma unroll
-            for (int m = 0; m < 3; m++) hbn[m] = stg[(RH + m) * ns + ln];
-            const double areas_n = stg[RA * ns + ln];
+            double hbn[3], areas_n;
+            elem_hb_areas(E, ne, nb, hbn, areas_n);
             const double edfac = xlen / AREAS, edfac_n = xlen / areas_n;
 #pragma unroll 1
             for (int I = 0; I < NEGP; I++) {

@@ -767,8 +767,8 @@ size_t stage_static_lds(int dof, int negp, int nt, bool met) {
     return (size_t)8 * (2 * 3 * dof * nt + 2 * 3 * negp * dof);
 }
 
-// Dynamic LDS bytes of adj_stage_kernel: 6*dof + 4 staged rows of ns doubles.
-size_t adj_dynamic_lds(int dof, int ns) { return (size_t)8 * (6 * dof + 4) * ns; }
+// Dynamic LDS bytes of adj_stage_kernel: 6*dof staged rows of ns doubles.
+size_t adj_dynamic_lds(int dof, int ns) { return (size_t)8 * (6 * dof) * ns; }
 
 // Block size the stage kernel runs with: the chosen one for nonlinear runs, 64 for linearised runs.
 int stage_nt(const Ctx& c, int nt) { return c.prm.nolicat != 0 ? nt : 64; }
@@ -1107,7 +1107,7 @@ int launch_adj(Ctx& c, const AdjArgs& a) {
 int adj_nt(int p) { return p == 1 ? 128 : 64; }
 
 // The adjoint kernel's block-local staging: the forward's halo lists for its block size, with its
-// own row count (state, RHS cotangent, bathymetry modes, AREAS per staged element).
+// own row count (state and RHS cotangent per staged element).
 int ensure_adj_staging(Ctx& c) {
     const int nt = adj_nt(c.prm.p);
     if (ensure_staging(c, nt)) return 1;
