@@ -7,8 +7,10 @@
 // src/DG_hydro_timestep.F:238-290).  Non-smooth operations take the branch the forward values
 // select: the LLF wave-speed bound differentiates through the largest of its six candidates, the
 // Manning friction floor CF and the 0.9/DT friction cap pass no derivative when active, and
-// Eirik's dry-ocean-edge fix differentiates |h|.  The configurations differentiated are those
-// without the slope limiter and without wetting and drying (every element wet).
+// Eirik's dry-ocean-edge fix differentiates |h|.  SLOPELIMITER5 is differentiated with its active
+// set frozen (limiter_adj_kernel and the node passes after it); WETDRY only where it is the
+// identity (every element wet and staying wet): the forward sweeps flag any element WETDRY changes,
+// and the gradient call then fails.
 //
 // Kernel layout follows the forward stage kernel: one thread per element, gathers only.  The
 // cotangent of an internal edge flux is the sum of what both neighbours' RHS cotangents ask of it;
