@@ -826,10 +826,16 @@ __global__ void __launch_bounds__(256) limit_wetdry_kernel(int np, ElemData E, d
                 int kdp = (dif[0] > 0) + (dif[1] > 0) + (dif[2] > 0);
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
-                    const double div = fmax(1.0, (double)kdp);
                     double redfac;
-                    if (dif[k] > 0) { redfac = sumdif * signdif / div; kdp = kdp - 1; }
-                    else redfac = 0;
+                    if (dif[k] > 0) {
+                        // (sumdif*signdif)/DIV with DIV = kdp, which is 1, 2 or 3 here: x/1 and
+                        // x/2 are x and x*0.5 exactly, so only DIV = 3 needs a division
+                        const double a = sumdif * signdif;
+                        redfac = kdp == 1 ? a : (kdp == 2 ? a * 0.5 : a / 3.0);
+                        kdp = kdp - 1;
+                    } else {
+                        redfac = 0;
+                    }
                     const double redmax = (signdif > 0) ? v[k] - vmin[k] : vmax[k] - v[k];
                     redfac = fmin(redfac, redmax);
                     sumdif = sumdif - redfac * signdif;
