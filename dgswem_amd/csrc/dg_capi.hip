@@ -122,6 +122,12 @@ struct Ctx {
     int max_lds = 65536;                         // LDS bytes per block the device allows
     DevBuf<double> nmm, outbuf, partial;
     DevBuf<int> nodecode;
+    // modal2nodal: node rows in the host's ELETAB order (default: the node->element CSR) and the
+    // running maximum ETAMAX, when the host has handed it over
+    DevBuf<int> m2n_off, m2n;
+    bool m2n_set = false;
+    DevBuf<double> etamax;
+    bool has_etamax = false;
     // forcing
     DevBuf<double> emo, efa, qnam, qnph, argj, rff, fargj, frff;
     DevBuf<int> step;                            // device step counter within the current tables
@@ -138,8 +144,10 @@ struct Ctx {
     // set while the adjoint's forward sweeps run: WETDRY raises it when it changes an element
     int* wd_active = nullptr;
     DevBuf<int> wdact;
+    DevBuf<uint8_t> wdsave;                      // wet/dry flags at the start of a gradient call
     // launch configuration
     int block = 256;
+    int dev = 0;                                 // device current at dg_gpu_init
     // adjoint: gauges, checkpoints, segment stage states, cotangents, parameter accumulators
     int ng = 0, ngu = 0;                         // gauges, distinct gauge elements
     DevBuf<int> gel, goff, gord;
@@ -660,6 +668,7 @@ int init_ctx_common(Ctx& c, const dg_params* prm, bool manning) {
         HIPCHK(hipGetDevice(&dev));
         HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
         c.max_lds = lds;
+        c.dev = dev;
     }
     HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     {
@@ -1258,6 +1267,17 @@ int validate(const dg_params* p) {
     if (p->nramp < 0 || p->nramp > 1) return fail("only NRAMP 0 or 1 is supported");
     if (p->rainfall < 0 || p->rainfall > 1) return fail("only rainfall 0 or 1 is supported");
     if (p->nolibf < 0 || p->nolibf > 2) return fail("nolibf must be 0, 1 or 2");
+    // inputs whose terms the step does not have: refused, never run without them
+    if (p->nws != 0 && p->nws != 8)
+        return fail("NWS=" + std::to_string(p->nws) + ": only NWS=0 and NWS=8 (Holland vortex) are supported");
+    if (p->ntip != 0) return fail("NTIP=" + std::to_string(p->ntip) + ": tidal potential forcing is not supported");
+    if (p->artdif != 0) return fail("artdif=" + std::to_string(p->artdif) + ": LDG artificial diffusion is not supported");
+    if (p->padapt != 0) return fail("padapt=" + std::to_string(p->padapt) + ": p-adaptivity is not supported");
+    if (p->pl != p->p)
+        return fail("fort.dg pl=" + std::to_string(p->pl) + " differs from ph=" + std::to_string(p->p) +
+                    ": mixed orders are not supported");
+    if (p->load_quadratic_fric != 0)
+        return fail("a per-node quadratic friction attribute (FRIC) is not supported: CF must be uniform");
     return 0;
 }
 
@@ -1437,6 +1457,7 @@ int dg_gpu_step(int h, int it, int nsteps) {
     for (auto& m : c->mem) {
         if (m.needs > 0 && m.nbfr == 0) return fail("tidal forcing not set (dg_gpu_set_tide)");
         if (m.nfeds > 0 && m.nffr == 0) return fail("flux forcing not set (dg_gpu_set_flow)");
+        if (m.prm.nws == 8 && !m.met) return fail("NWS=8 forcing not set (dg_gpu_set_holland)");
     }
     if (ensure_staging(*c, stage_nt(*c, c->block))) return 1;
     for (int s0 = 0; s0 < nsteps; s0 += kTableChunk) {
@@ -1456,31 +1477,91 @@ int dg_gpu_synchronize(int h) {
     return 0;
 }
 
-int dg_gpu_modal2nodal(int h, double* eta2, double* uu2, double* vv2, int* nodecode) {
+int dg_gpu_nodal_update(int h) {
     Ctx* c = get(h);
     if (!c) return 1;
     const int np = c->np;
     ElemData E = elem_data(*c);
     const int B = 256, gn = (np + B - 1) / B;
+    const int* off = c->m2n_set ? c->m2n_off.p : c->n2e_off.p;
+    const int* row = c->m2n_set ? c->m2n.p : c->n2e.p;
+    double* em = c->has_etamax ? c->etamax.p : nullptr;
     int ev = prof_begin(*c, K_M2N, false);
     if (c->prm.p == 1)
-        hipLaunchKernelGGL((modal2nodal_kernel<1>), dim3(gn), dim3(B), 0, c->stream, np, E, (const int*)c->n2e_off.p,
-                           (const int*)c->n2e.p, (const double*)c->u[c->ucur].p, (const uint8_t*)c->wd[c->wdcur].p,
-                           c->phys.nleq, c->phys.leq, c->outbuf.p, c->nodecode.p);
+        hipLaunchKernelGGL((modal2nodal_kernel<1>), dim3(gn), dim3(B), 0, c->stream, np, E, off, row,
+                           (const double*)c->u[c->ucur].p, (const uint8_t*)c->wd[c->wdcur].p,
+                           c->phys.nleq, c->phys.leq, c->outbuf.p, c->nodecode.p, em);
     else
-        hipLaunchKernelGGL((modal2nodal_kernel<2>), dim3(gn), dim3(B), 0, c->stream, np, E, (const int*)c->n2e_off.p,
-                           (const int*)c->n2e.p, (const double*)c->u[c->ucur].p, (const uint8_t*)c->wd[c->wdcur].p,
-                           c->phys.nleq, c->phys.leq, c->outbuf.p, c->nodecode.p);
+        hipLaunchKernelGGL((modal2nodal_kernel<2>), dim3(gn), dim3(B), 0, c->stream, np, E, off, row,
+                           (const double*)c->u[c->ucur].p, (const uint8_t*)c->wd[c->wdcur].p,
+                           c->phys.nleq, c->phys.leq, c->outbuf.p, c->nodecode.p, em);
     prof_end(*c, ev);
     HIPCHK(hipGetLastError());
-    std::vector<double> out((size_t)3 * np);
-    HIPCHK(hipMemcpyAsync(out.data(), c->outbuf.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    if (nodecode) HIPCHK(hipMemcpyAsync(nodecode, c->nodecode.p, (size_t)np * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    return 0;
+}
+
+int dg_gpu_get_nodal(int h, double* eta2, double* uu2, double* vv2, int* nodecode, double* etamax) {
+    Ctx* c = get(h);
+    if (!c) return 1;
+    const size_t np = (size_t)c->np;
+    if (etamax && !c->has_etamax) return fail("ETAMAX was not set (dg_gpu_set_etamax)");
+    double* dst[3] = {eta2, uu2, vv2};
+    for (int k = 0; k < 3; k++)
+        if (dst[k]) HIPCHK(hipMemcpyAsync(dst[k], c->outbuf.p + k * np, np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (nodecode) HIPCHK(hipMemcpyAsync(nodecode, c->nodecode.p, np * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (etamax) HIPCHK(hipMemcpyAsync(etamax, c->etamax.p, np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (c->prof && prof_flush(*c)) return 1;
-    if (eta2) std::memcpy(eta2, out.data(), (size_t)np * sizeof(double));
-    if (uu2) std::memcpy(uu2, out.data() + np, (size_t)np * sizeof(double));
-    if (vv2) std::memcpy(vv2, out.data() + 2 * (size_t)np, (size_t)np * sizeof(double));
+    return 0;
+}
+
+int dg_gpu_modal2nodal(int h, double* eta2, double* uu2, double* vv2, int* nodecode) {
+    if (dg_gpu_nodal_update(h)) return 1;
+    return dg_gpu_get_nodal(h, eta2, uu2, vv2, nodecode, nullptr);
+}
+
+int dg_gpu_set_etamax(int h, const double* etamax) {
+    Ctx* c = get(h);
+    if (!c) return 1;
+    if (!etamax) return fail("null ETAMAX");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(c->etamax.upload(etamax, (size_t)c->np, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->has_etamax = true;
+    return 0;
+}
+
+int dg_gpu_set_eletab(int h, int mnp, const int* el_count, const int* eletab) {
+    Ctx* c = get(h);
+    if (!c) return 1;
+    if (c->S != 1) return fail("dg_gpu_set_eletab applies to single runs");
+    const Member& m = c->mem[0];
+    const int np = m.np, ne = m.ne;
+    if (mnp < np) return fail("ELETAB has fewer rows than nodes");
+    std::vector<int> off(np + 1, 0), row;
+    row.reserve(m.n2e.size());
+    for (int i = 0; i < np; i++) {
+        const int cnt = el_count[i];
+        if (cnt != m.n2e_off[i + 1] - m.n2e_off[i])
+            return fail("ELETAB: node " + std::to_string(i + 1) + " lists " + std::to_string(cnt) +
+                        " elements, the mesh has " + std::to_string(m.n2e_off[i + 1] - m.n2e_off[i]));
+        for (int j = 0; j < cnt; j++) {
+            const int e = eletab[(size_t)(1 + j) * mnp + i] - 1;   // ELETAB(I, 1+J)
+            if (e < 0 || e >= ne) return fail("ELETAB: element index out of range");
+            int k = 0;
+            while (k < 3 && m.nm[(size_t)k * ne + e] != i) k++;
+            if (k == 3) return fail("ELETAB: node " + std::to_string(i + 1) + " is not a corner of element " + std::to_string(e + 1));
+            for (int q = off[i]; q < (int)row.size(); q++)
+                if ((row[q] >> 2) == e) return fail("ELETAB: element listed twice for node " + std::to_string(i + 1));
+            row.push_back(e << 2 | k);
+        }
+        off[i + 1] = (int)row.size();
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(c->m2n_off.upload(off.data(), off.size(), c->stream));
+    HIPCHK(c->m2n.upload(row.data(), row.size(), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->m2n_set = true;
     return 0;
 }
 
@@ -1643,6 +1724,17 @@ int dg_adj_gradient(int h, int it, int nsteps, const double* data, double weight
         HIPCHK(c->lgsum.alloc((size_t)6 * np));
         HIPCHK(c->larg.alloc((size_t)6 * np));
     }
+    // the state and wet/dry flags at entry: every exit path puts them back (the sweeps rotate the
+    // stage slots and flip the flag buffers)
+    HIPCHK(c->wdsave.alloc(ne));
+    HIPCHK(hipMemcpyAsync(c->ckpt.p, c->u[c->ucur].p, ssz * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->wdsave.p, c->wd[c->wdcur].p, ne, hipMemcpyDeviceToDevice, c->stream));
+    auto restore = [&]() -> int {
+        HIPCHK(hipMemcpyAsync(c->u[c->ucur].p, c->ckpt.p, ssz * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->wd[c->wdcur].p, c->wdsave.p, ne, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return 0;
+    };
     // WETDRY is differentiated with its active set frozen at the identity: every element wet and
     // left unchanged.  The forward sweeps flag any element it changes; such runs are refused.
     HIPCHK(c->wdact.alloc(1));
@@ -1654,13 +1746,16 @@ int dg_adj_gradient(int h, int it, int nsteps, const double* data, double weight
     if (!rc) rc = c->prm.p == 1 ? adj_backward<1>(*c, it, nsteps, interval, weight) : adj_backward<2>(*c, it, nsteps, interval, weight);
     c->prof = prof;
     c->wd_active = nullptr;
-    if (rc) return rc;
+    if (rc) {
+        const std::string msg = g_err;
+        (void)restore();
+        return fail(msg);
+    }
     int wdflag = 0;
     HIPCHK(hipMemcpyAsync(&wdflag, c->wdact.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (wdflag) {
-        HIPCHK(hipMemcpyAsync(c->u[c->ucur].p, c->ckpt.p, ssz * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
+        if (restore()) return 1;
         return fail("wetting/drying changed elements during the run: the adjoint differentiates WETDRY "
                     "only where it leaves every element as it is");
     }
@@ -1677,8 +1772,7 @@ int dg_adj_gradient(int h, int it, int nsteps, const double* data, double weight
     if (grad_dp) HIPCHK(hipMemcpyAsync(grad_dp, gdp_n, (size_t)np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     if (grad_manning) HIPCHK(hipMemcpyAsync(grad_manning, gn_n, (size_t)np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     // the handle's state returns to where the call started
-    HIPCHK(hipMemcpyAsync(c->u[c->ucur].p, c->ckpt.p, ssz * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    if (restore()) return 1;
     if (loss) {
         double J = 0.0;
         for (size_t k = 0; k < eta.size(); k++) {

@@ -863,7 +863,10 @@ __global__ void __launch_bounds__(256) limit_wetdry_kernel(int np, ElemData E, d
 }
 
 // ------------------------------------------------------------------------------------------
-// modal2nodal (src/write_results.F:14-174) + NODECODE from the wet flags.
+// modal2nodal (src/write_results.F:14-174) + NODECODE from the wet flags.  Each node's incident
+// elements are summed in the order of its CSR row (the host's ELETAB order when the host gave it).
+// With etamax set, ETAMAX is raised as the reference raises it: after every wet element's
+// contribution to ETA2, so it also sees the partial sums (src/write_results.F:164).
 // ------------------------------------------------------------------------------------------
 template <int P>
 __global__ void __launch_bounds__(256) modal2nodal_kernel(int np, ElemData E, const int* __restrict__ off,
@@ -871,7 +874,8 @@ __global__ void __launch_bounds__(256) modal2nodal_kernel(int np, ElemData E, co
                                                           const double* __restrict__ u,
                                                           const uint8_t* __restrict__ wd, double nleq,
                                                           double leq, double* __restrict__ out,
-                                                          int* __restrict__ nodecode) {
+                                                          int* __restrict__ nodecode,
+                                                          double* __restrict__ etamax) {
     constexpr int DOF = (P + 1) * (P + 2) / 2;
     const DevTables& T = tab<P>();
     const int ne = E.ne;
@@ -889,6 +893,7 @@ __global__ void __launch_bounds__(256) modal2nodal_kernel(int np, ElemData E, co
         asum = asum + 0.5 * areas;
     }
     double e2 = 0.0, u2 = 0.0, v2 = 0.0;
+    double emax = etamax ? etamax[n] : 0.0;
     for (int j = off[n]; j < off[n + 1]; j++) {
         const int e = n2e[j] >> 2, c = n2e[j] & 3;
         if (wd[e] == 0) continue;
@@ -910,11 +915,13 @@ __global__ void __launch_bounds__(256) modal2nodal_kernel(int np, ElemData E, co
         const double depth = zd + hd;
         const double fh = 1.0 / (nleq * depth + leq);
         e2 = e2 + area * zd;
+        if (emax < e2) emax = e2;
         u2 = u2 + area * xd * fh;
         v2 = v2 + area * yd * fh;
     }
     out[n] = e2; out[(size_t)np + n] = u2; out[2 * (size_t)np + n] = v2;
     nodecode[n] = nc;
+    if (etamax) etamax[n] = emax;
 }
 
 // ------------------------------------------------------------------------------------------

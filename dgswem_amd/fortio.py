@@ -289,6 +289,16 @@ def read_fort13(path: str, names: List[str], n_nodes: int) -> dict:
 def read_case(dirname: str) -> Case:
     """Read fort.14/fort.15/fort.dg (+fort.13 when NWP>0) from a run directory."""
     dg = read_fort_dg(os.path.join(dirname, "fort.dg"))
+    # with padapt=0 the reference runs every element at order pl (src/prep_DG.F:90,104,124); the
+    # step here runs one order, ph, so the two must agree
+    if dg.padapt != 0:
+        raise NotImplementedError(f"fort.dg padapt={dg.padapt}: p-adaptivity is not supported")
+    if dg.pl != dg.ph:
+        raise NotImplementedError(f"fort.dg pl={dg.pl} differs from ph={dg.ph}: mixed orders are "
+                                  "not supported")
+    if dg.artdif != 0:
+        raise NotImplementedError(f"fort.dg artdif={dg.artdif}: LDG artificial diffusion is not "
+                                  "supported")
     f15 = _Records(os.path.join(dirname, "fort.15"))
     f14 = _Records(os.path.join(dirname, "fort.14"))
     p = Params()
@@ -382,6 +392,9 @@ def read_case(dirname: str) -> Case:
     p.eslm = f15.reals(2)[0] if p.im == 10 else f15.reals(1)
     manning = None
     if p.nwp > 0:
+        if "quadratic_friction_coefficient_at_sea_floor" in attr_names:
+            raise NotImplementedError("a per-node quadratic friction attribute (FRIC): the step "
+                                      "takes CF uniform")
         attrs = read_fort13(os.path.join(dirname, "fort.13"), attr_names, n_nodes)
         manning = attrs.get("mannings_n_at_sea_floor")
     p.cori = f15.reals(1)

@@ -2,26 +2,42 @@
 !
 ! Drop-in for the reference's SUBROUTINE DG_TIMESTEP (src/DG_timestep.F:11-50): linked in place of
 ! DG_timestep.o, it replaces the call DG_HYDRO_TIMESTEP(IT) with dg_gpu_step and the host
-! modal2nodal with dg_gpu_modal2nodal, then calls the reference's own WRITE_RESULTS, so the run
-! still reads fort.14/fort.15/fort.dg(/fort.13) and writes fort.6x and the DG hot start.
-! The first call hands the module arrays built by READ_INPUT and PREP_DG to the library.
+! modal2nodal with dg_gpu_nodal_update, then calls the reference's own WRITE_RESULTS, so the run
+! still reads fort.14/fort.15/fort.dg(/fort.13/fort.22) and writes fort.6x, maxele.63 and the DG
+! hot start.  The first call hands the module arrays built by READ_INPUT and PREP_DG to the
+! library; inputs whose terms the library does not have (NWS other than 0 and 8, NTIP, artdif,
+! p-adaptivity, a per-node quadratic friction attribute) make dg_gpu_init fail and the run STOP.
+!
+! The nodal values (ETA2, UU2, VV2, NODECODE), the running maximum ETAMAX and the modal state stay
+! on the GPU; they are copied to the host only on the steps at which WRITE_RESULTS reads them: an
+! output whose spool counter fires (src/write_results.F:205-208,285-288,343-346,402-403,458-461,
+! 660-663,716-717,790-793), harmonic analysis, a hot start, the DG hot start, and the last step.
 subroutine DG_TIMESTEP(IT)
-  use sizes, only : sz
+  use sizes, only : sz, MNP
   use global, only : NP, NE, X, Y, DP, SFAC, CORIF, NM, G, DTDP, STATIM, DRAMP, H0, SFEA0, ICS, &
        NOLIFA, NOLICA, NOLICAT, NRAMP, NT, NBFR, AMIG, FF, FACE, PER, ETA2, UU2, VV2, NODECODE, &
-       ETAMAX
-  use dg, only : PH, RK_STAGE, RK_ORDER, SLOPEFLAG, RAINFALL, DGHOT, DGHOTSPOOL, NEDGES, NEDNO, &
-       NEDEL, NEDSD, NIEDS, NIEDN, NLEDS, NLEDN, NEEDS, NEEDN, NFEDS, NFEDN, EMO_DG, EFA_DG, &
-       QNAM_DG, QNPH_DG, ZE, QX, QY, WDFLG
+       ETAMAX, NWS, NTIP, IREFYR, IREFMO, IREFDAY, IREFHR, SLAM, SFEA, EL_COUNT, ELETAB
+  use global, only : NOUTE, NTCYSE, NTCYFE, NSCOUE, NSPOOLE, NOUTV, NTCYSV, NTCYFV, NSCOUV, &
+       NSPOOLV, NOUTC, NTCYSC, NTCYFC, NSCOUC, NSPOOLC, NOUTM, NTCYSM, NTCYFM, NSCOUM, NSPOOLM, &
+       NOUTGE, NTCYSGE, NTCYFGE, NSCOUGE, NSPOOLGE, NOUTGV, NTCYSGV, NTCYFGV, NSCOUGV, NSPOOLGV, &
+       NOUTGC, NTCYSGC, NTCYFGC, NSCOUGC, NSPOOLGC, NOUTGW, NTCYSGW, NTCYFGW, NSCOUGW, NSPOOLGW, &
+       IHARIND, ITHAS, NHSTAR, NHSINC
+  use dg, only : PH, PL, PADAPT, ARTDIF, RK_STAGE, RK_ORDER, SLOPEFLAG, RAINFALL, DGHOT, &
+       DGHOTSPOOL, NEDGES, NEDNO, NEDEL, NEDSD, NIEDS, NIEDN, NLEDS, NLEDN, NEEDS, NEEDN, NFEDS, &
+       NFEDN, EMO_DG, EFA_DG, QNAM_DG, QNPH_DG, ZE, QX, QY, WDFLG
   use global, only : NFFR, FAMIG, FFF, FFACE, FPER
-  use NodalAttributes, only : NoLiBF, CF, HBreak, FTheta, FGamma, LoadManningsN, ManningsN
+  use NodalAttributes, only : NoLiBF, CF, HBreak, FTheta, FGamma, LoadManningsN, ManningsN, &
+       LoadQuadraticFric
+  use wind, only : StormNumber, BLAdj
   use dg_gpu_interface
   implicit none
   integer, intent(in) :: IT
   integer(c_int), save :: handle = -1
   type(dg_params) :: prm
   real(c_double) :: nodummy(1)
-  integer :: i
+  integer(c_int), allocatable :: eletab_i(:, :)
+  integer :: ncol, i, j
+  logical :: host
 
   if (handle < 0) then
      prm%p = PH
@@ -36,6 +52,12 @@ subroutine DG_TIMESTEP(IT)
      prm%slopeflag = SLOPEFLAG
      prm%rainfall = RAINFALL
      prm%load_manning = merge(1, 0, LoadManningsN)
+     prm%nws = NWS
+     prm%ntip = NTIP
+     prm%artdif = ARTDIF
+     prm%pl = PL
+     prm%padapt = PADAPT
+     prm%load_quadratic_fric = merge(1, 0, LoadQuadraticFric)
      prm%g = G
      prm%dtdp = DTDP
      prm%statim = STATIM
@@ -64,23 +86,99 @@ subroutine DG_TIMESTEP(IT)
         call dg_gpu_check(dg_gpu_set_flow(handle, NFFR, FAMIG, FFF, FFACE, FPER, &
              QNAM_DG(1:NFFR, 1:NFEDS, 1:2), QNPH_DG(1:NFFR, 1:NFEDS, 1:2)), 'dg_gpu_set_flow')
      end if
+     if (NWS == 8) call set_holland_track()
+     ! modal2nodal visits each node's elements in PREP_DG's order (ELETAB holds them as reals)
+     ncol = maxval(EL_COUNT(1:NP)) + 1
+     allocate(eletab_i(MNP, ncol))
+     eletab_i = 0
+     do i = 1, NP
+        do j = 1, EL_COUNT(i) + 1
+           eletab_i(i, j) = nint(ELETAB(i, j))
+        end do
+     end do
+     call dg_gpu_check(dg_gpu_set_eletab(handle, MNP, EL_COUNT, eletab_i), 'dg_gpu_set_eletab')
+     deallocate(eletab_i)
+     call dg_gpu_check(dg_gpu_set_etamax(handle, ETAMAX), 'dg_gpu_set_etamax')
      ! the state PREP_DG prepared (cold or DG hot start)
      call dg_gpu_check(dg_gpu_set_state(handle, ZE(:, 1:NE, 1), QX(:, 1:NE, 1), QY(:, 1:NE, 1), &
           WDFLG), 'dg_gpu_set_state')
   end if
 
-  ! DG_HYDRO_TIMESTEP(IT) on the GPU, then modal2nodal (both every step, as the reference does)
+  ! DG_HYDRO_TIMESTEP(IT) and modal2nodal on the GPU (both every step, as the reference does)
   call dg_gpu_check(dg_gpu_step(handle, IT, 1), 'dg_gpu_step')
-  call dg_gpu_check(dg_gpu_modal2nodal(handle, ETA2, UU2, VV2, NODECODE), 'dg_gpu_modal2nodal')
-  do i = 1, NP
-     if (ETAMAX(i) < ETA2(i)) ETAMAX(i) = ETA2(i)
-  end do
-  ! the modal state is only read on the host when the DG hot start is written
-  if (DGHOT == 1) then
-     if ((mod(IT, DGHOTSPOOL) == 0) .or. (IT == NT)) then
-        call dg_gpu_check(dg_gpu_get_state(handle, ZE(:, 1:NE, 1), QX(:, 1:NE, 1), QY(:, 1:NE, 1), &
-             WDFLG), 'dg_gpu_get_state')
-     end if
+  call dg_gpu_check(dg_gpu_nodal_update(handle), 'dg_gpu_nodal_update')
+
+  ! copy back only where WRITE_RESULTS(IT) reads the nodal values or the modal state
+  host = (IT == NT) &
+       .or. fires(NOUTE, NTCYSE, NTCYFE, NSCOUE, NSPOOLE) &
+       .or. fires(NOUTV, NTCYSV, NTCYFV, NSCOUV, NSPOOLV) &
+       .or. fires(NOUTC, NTCYSC, NTCYFC, NSCOUC, NSPOOLC) &
+       .or. fires(NOUTM, NTCYSM, NTCYFM, NSCOUM, NSPOOLM) &
+       .or. fires(NOUTGE, NTCYSGE, NTCYFGE, NSCOUGE, NSPOOLGE) &
+       .or. fires(NOUTGV, NTCYSGV, NTCYFGV, NSCOUGV, NSPOOLGV) &
+       .or. fires(NOUTGC, NTCYSGC, NTCYFGC, NSCOUGC, NSPOOLGC) &
+       .or. fires(NOUTGW, NTCYSGW, NTCYFGW, NSCOUGW, NSPOOLGW) &
+       .or. ((IHARIND == 1) .and. (IT > ITHAS))
+  if (NHSTAR == 1) host = host .or. (mod(IT, NHSINC) == 0)
+  if (DGHOT == 1) host = host .or. (mod(IT, DGHOTSPOOL) == 0)
+  if (host) then
+     call dg_gpu_check(dg_gpu_get_nodal(handle, ETA2, UU2, VV2, NODECODE, ETAMAX), 'dg_gpu_get_nodal')
+     call dg_gpu_check(dg_gpu_get_state(handle, ZE(:, 1:NE, 1), QX(:, 1:NE, 1), QY(:, 1:NE, 1), &
+          WDFLG), 'dg_gpu_get_state')
   end if
   call WRITE_RESULTS(IT, .FALSE.)
+
+contains
+
+  ! WRITE_RESULTS' test for one output: inside its window, and the spool counter it is about to
+  ! increment reaches the spool interval
+  logical function fires(nout, ntcys, ntcyf, nscou, nspool)
+    integer, intent(in) :: nout, ntcys, ntcyf, nscou, nspool
+    fires = (nout /= 0) .and. (IT > ntcys) .and. (IT <= ntcyf) .and. (nscou + 1 == nspool)
+  end function fires
+
+  ! fort.22 (ATCF track) for NWS=8, read with the fixed columns GetHollandStormData reads
+  ! (src/wind.F:2224-2227, FORMAT 14 at src/wind.F:2440): 12 integers per record for the library
+  subroutine set_holland_track()
+    integer :: u, ios, n, k
+    integer(c_int), allocatable :: rec(:, :)
+    integer(c_int) :: ref(4)
+    integer :: iyr, imo, idy, ihr, ifc, ila, ilo, isp, icp, irr, irm
+    character(len=4) :: cast
+    open(newunit=u, file='fort.22', status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+       print *, 'DG GPU ERROR: NWS=8 needs fort.22'
+       stop
+    end if
+    n = 0
+    do
+       read(u, '(A)', iostat=ios)
+       if (ios /= 0) exit
+       n = n + 1
+    end do
+    rewind(u)
+    allocate(rec(12, n))
+    do k = 1, n
+       read(u, 14, iostat=ios) iyr, imo, idy, ihr, cast, ifc, ila, ilo, isp, icp, irr, irm
+       if (ios /= 0) then
+          print *, 'DG GPU ERROR: cannot read fort.22 line ', k
+          stop
+       end if
+       rec(:, k) = [iyr, imo, idy, ihr, 2, ifc, ila, ilo, isp, icp, irr, irm]
+       if (cast == 'BEST') rec(5, k) = 0
+       if (cast == 'OFCL') rec(5, k) = 1
+       if (rec(5, k) == 2) then
+          print *, 'DG GPU ERROR: fort.22 line ', k, ' has cast type ', cast, &
+               ' (BEST or OFCL expected)'
+          stop
+       end if
+    end do
+    close(u)
+    ref = [IREFYR, IREFMO, IREFDAY, IREFHR]
+    call dg_gpu_check(dg_gpu_set_holland(handle, n, rec, ref, StormNumber, BLAdj, SLAM, SFEA), &
+         'dg_gpu_set_holland')
+    deallocate(rec)
+14  format(8x, i4, i2, i2, i2, 6x, a4, 2x, i3, 1x, i4, 3x, i4, 3x, i3, 2x, i4, 47x, i3, 2x, i3)
+  end subroutine set_holland_track
+
 end subroutine DG_TIMESTEP

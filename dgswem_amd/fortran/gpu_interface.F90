@@ -10,6 +10,7 @@ module dg_gpu_interface
   type, bind(C) :: dg_params
      integer(c_int) :: p, rk_stage, rk_order, ics, nolibf, nolifa, nolica, nolicat, nramp
      integer(c_int) :: slopeflag, rainfall, load_manning
+     integer(c_int) :: nws, ntip, artdif, pl, padapt, load_quadratic_fric
      real(c_double) :: g, dtdp, statim, dramp, h0, cf, hbreak, ftheta, fgamma, sfea0
   end type dg_params
 
@@ -72,6 +73,41 @@ module dg_gpu_interface
        integer(c_int), intent(out) :: nodecode(*)
      end function dg_gpu_modal2nodal
 
+     integer(c_int) function dg_gpu_nodal_update(handle) bind(C, name='dg_gpu_nodal_update')
+       import :: c_int
+       integer(c_int), value :: handle
+     end function dg_gpu_nodal_update
+
+     integer(c_int) function dg_gpu_get_nodal(handle, eta2, uu2, vv2, nodecode, etamax) &
+          bind(C, name='dg_gpu_get_nodal')
+       import :: c_int, c_double
+       integer(c_int), value :: handle
+       real(c_double), intent(out) :: eta2(*), uu2(*), vv2(*), etamax(*)
+       integer(c_int), intent(out) :: nodecode(*)
+     end function dg_gpu_get_nodal
+
+     integer(c_int) function dg_gpu_set_etamax(handle, etamax) bind(C, name='dg_gpu_set_etamax')
+       import :: c_int, c_double
+       integer(c_int), value :: handle
+       real(c_double), intent(in) :: etamax(*)
+     end function dg_gpu_set_etamax
+
+     integer(c_int) function dg_gpu_set_eletab(handle, mnp, el_count, eletab) &
+          bind(C, name='dg_gpu_set_eletab')
+       import :: c_int
+       integer(c_int), value :: handle, mnp
+       integer(c_int), intent(in) :: el_count(*), eletab(*)
+     end function dg_gpu_set_eletab
+
+     integer(c_int) function dg_gpu_set_holland(handle, nrec, rec, ref, storm_number, bladj, &
+          slam, sfea) bind(C, name='dg_gpu_set_holland')
+       import :: c_int, c_double
+       integer(c_int), value :: handle, nrec, storm_number
+       integer(c_int), intent(in) :: rec(*), ref(*)
+       real(c_double), value :: bladj
+       real(c_double), intent(in) :: slam(*), sfea(*)
+     end function dg_gpu_set_holland
+
      type(c_ptr) function dg_gpu_last_error() bind(C, name='dg_gpu_last_error')
        import :: c_ptr
      end function dg_gpu_last_error
@@ -105,7 +141,7 @@ contains
     do i = 1, min(n, 512)
        text(i:i) = msg(i)
     end do
-    print *, 'DG GPU ERROR in ', where, ': ', trim(text)
+    write(*, '(4A)') 'DG GPU ERROR in ', where, ': ', trim(text)
     stop
   end subroutine dg_gpu_check
 

@@ -27,7 +27,8 @@ K_STAGE, K_MINMAX, K_LIMIT, K_M2N, K_MET = 0, 1, 2, 3, 4
 # every symbol include/dgswem_gpu.h declares
 EXPORTED = ["dg_create_edge_data", "dg_gpu_init", "dg_gpu_set_tide", "dg_gpu_set_flow",
             "dg_gpu_set_state", "dg_gpu_get_state", "dg_gpu_step", "dg_gpu_synchronize",
-            "dg_gpu_modal2nodal", "dg_gpu_mass", "dg_gpu_profile", "dg_gpu_kernel_stats",
+            "dg_gpu_modal2nodal", "dg_gpu_nodal_update", "dg_gpu_get_nodal",
+            "dg_gpu_set_etamax", "dg_gpu_set_eletab", "dg_gpu_mass", "dg_gpu_profile", "dg_gpu_kernel_stats",
             "dg_gpu_set_holland", "dg_holland_storm_at", "dg_gpu_set_manning", "dg_gpu_set_depth",
             "dg_adj_set_gauges", "dg_adj_gauges", "dg_adj_gradient", "dg_ens_create", "dg_ens_layout", "dg_gpu_use_graph", "dg_gpu_autotune",
             "dg_gpu_set_block", "dg_gpu_last_error", "dg_gpu_finalize"]
@@ -37,7 +38,9 @@ class DgParams(C.Structure):
     _fields_ = [("p", C.c_int), ("rk_stage", C.c_int), ("rk_order", C.c_int), ("ics", C.c_int),
                 ("nolibf", C.c_int), ("nolifa", C.c_int), ("nolica", C.c_int),
                 ("nolicat", C.c_int), ("nramp", C.c_int), ("slopeflag", C.c_int),
-                ("rainfall", C.c_int), ("load_manning", C.c_int), ("g", C.c_double),
+                ("rainfall", C.c_int), ("load_manning", C.c_int), ("nws", C.c_int),
+                ("ntip", C.c_int), ("artdif", C.c_int), ("pl", C.c_int), ("padapt", C.c_int),
+                ("load_quadratic_fric", C.c_int), ("g", C.c_double),
                 ("dtdp", C.c_double), ("statim", C.c_double), ("dramp", C.c_double),
                 ("h0", C.c_double), ("cf", C.c_double), ("hbreak", C.c_double),
                 ("ftheta", C.c_double), ("fgamma", C.c_double), ("sfea0", C.c_double)]
@@ -68,6 +71,11 @@ def load_library(path: str = LIB_PATH):
     L.dg_gpu_step.argtypes = [C.c_int, C.c_int, C.c_int]
     L.dg_gpu_synchronize.argtypes = [C.c_int]
     L.dg_gpu_modal2nodal.argtypes = [C.c_int, _dp, _dp, _dp, _ip]
+    L.dg_gpu_nodal_update.argtypes = [C.c_int]
+    L.dg_gpu_get_nodal.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p]
+    L.dg_gpu_set_etamax.argtypes = [C.c_int, _dp]
+    L.dg_gpu_set_eletab.argtypes = [C.c_int, C.c_int, _ip, _ip]
     L.dg_gpu_mass.argtypes = [C.c_int, C.POINTER(C.c_double)]
     L.dg_gpu_profile.argtypes = [C.c_int, C.c_int]
     L.dg_gpu_kernel_stats.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double),
@@ -112,7 +120,9 @@ def params_of(case: Case) -> DgParams:
     return DgParams(p=case.dg.ph, rk_stage=case.dg.rk_stage, rk_order=case.dg.rk_order, ics=p.ics,
                     nolibf=p.nolibf, nolifa=p.nolifa, nolica=p.nolica, nolicat=p.nolicat,
                     nramp=p.nramp, slopeflag=case.dg.slopeflag, rainfall=case.dg.rainfall,
-                    load_manning=1 if case.manning is not None else 0, g=p.g, dtdp=p.dtdp,
+                    load_manning=1 if case.manning is not None else 0, nws=p.nws, ntip=p.ntip,
+                    artdif=case.dg.artdif, pl=case.dg.pl, padapt=case.dg.padapt,
+                    load_quadratic_fric=0, g=p.g, dtdp=p.dtdp,
                     statim=p.statim, dramp=p.dramp, h0=p.h0, cf=p.cf, hbreak=p.hbreak,
                     ftheta=p.ftheta, fgamma=p.fgamma, sfea0=p.sfea0)
 
@@ -237,6 +247,23 @@ class _Handle:
         nc = np.empty(self.np_, np.int32)
         _check(load_library().dg_gpu_modal2nodal(self.h, e, u, v, nc))
         return e, u, v, nc
+
+    def nodal_update(self) -> None:
+        """modal2nodal on the device only (raises ETAMAX there once set); nothing is copied."""
+        _check(load_library().dg_gpu_nodal_update(self.h))
+
+    def get_nodal(self):
+        """ETA2, UU2, VV2, NODECODE and ETAMAX of the latest nodal_update (ETAMAX None if unset)."""
+        e, u, v = np.empty(self.np_), np.empty(self.np_), np.empty(self.np_)
+        nc = np.empty(self.np_, np.int32)
+        em = np.empty(self.np_) if getattr(self, "_etamax_set", False) else None
+        ptr = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+        _check(load_library().dg_gpu_get_nodal(self.h, ptr(e), ptr(u), ptr(v), ptr(nc), ptr(em)))
+        return e, u, v, nc, em
+
+    def set_etamax(self, etamax) -> None:
+        _check(load_library().dg_gpu_set_etamax(self.h, _f(etamax)))
+        self._etamax_set = True
 
     def mass(self) -> float:
         m = C.c_double()

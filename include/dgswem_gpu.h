@@ -32,6 +32,13 @@ typedef struct dg_params {
     int slopeflag;    /* 0 off, otherwise SLOPELIMITER5                                           */
     int rainfall;     /* 0 off, 1 constant rain rate                                              */
     int load_manning; /* LoadManningsN: per-node Manning's n is given                            */
+    /* Inputs the step does not implement, passed so that the library can refuse them instead of
+     * running without their terms (each must be as stated, else dg_gpu_init fails):             */
+    int nws;          /* NWS: 0, or 8 (then dg_gpu_set_holland before the first step)             */
+    int ntip;         /* NTIP: 0 (tidal potential forcing is not implemented)                      */
+    int artdif;       /* fort.dg artdif: 0 (no LDG artificial diffusion)                           */
+    int pl, padapt;   /* fort.dg pl and padapt: pl == p and padapt == 0 (no p-adaptivity)          */
+    int load_quadratic_fric; /* LoadQuadraticFric: 0 (FRIC must be the uniform CF)                */
     double g;         /* G                                                                       */
     double dtdp;      /* DTDP, the time step (s)                                                  */
     double statim;    /* STATIM (days)                                                            */
@@ -102,8 +109,31 @@ int dg_gpu_step(int handle, int it, int nsteps);
 int dg_gpu_synchronize(int handle);
 
 /* modal2nodal (src/write_results.F:14-174): ETA2, UU2, VV2 at the nodes and NODECODE (1 where an
- * incident element is wet; WETDRY's rule, src/wetdry.F:142-169,525-548). Synchronous. */
+ * incident element is wet; WETDRY's rule, src/wetdry.F:142-169,525-548). Synchronous.  The same
+ * as dg_gpu_nodal_update followed by dg_gpu_get_nodal without ETAMAX. */
 int dg_gpu_modal2nodal(int handle, double* eta2, double* uu2, double* vv2, int* nodecode);
+
+/* Device-resident modal2nodal, for hosts that read the nodal values only on output steps.
+ * dg_gpu_nodal_update replaces `call modal2nodal()` at src/DG_timestep.F:31: it computes ETA2,
+ * UU2, VV2, NODECODE into device arrays (and raises ETAMAX there, when set) and copies nothing;
+ * asynchronous on the handle's stream.  dg_gpu_get_nodal copies the latest values to the host
+ * (any pointer may be NULL; etamax only after dg_gpu_set_etamax).  Synchronous. */
+int dg_gpu_nodal_update(int handle);
+int dg_gpu_get_nodal(int handle, double* eta2, double* uu2, double* vv2, int* nodecode,
+                     double* etamax);
+
+/* ETAMAX(NP), the running maximum of ETA2 that modal2nodal keeps for maxele.63
+ * (src/write_results.F:164, set to -99999 by src/prep_DG.F:295): upload the host's values.  From
+ * then on every modal2nodal on the device raises it as the reference does, after each wet
+ * element's contribution is added to ETA2 (so partial sums count too). */
+int dg_gpu_set_etamax(int handle, const double* etamax);
+
+/* The order in which modal2nodal visits each node's elements: EL_COUNT(MNP) and ELETAB(MNP,*)
+ * as PREP_DG builds them (src/prep_DG.F:292-330; column-major, ELETAB(I,1+J) is the J-th element
+ * of node I).  The order fixes the rounding of ETA2 and which partial sums ETAMAX sees; without
+ * this call the elements are visited in ascending element number.  Every node must list exactly
+ * its incident elements.  Single runs only. */
+int dg_gpu_set_eletab(int handle, int mnp, const int* el_count, const int* eletab);
 
 /* Total water volume sum_e A_e*(ze_1 + hb_1) of the stage-1 state (mode 1 is the cell mean). */
 int dg_gpu_mass(int handle, double* mass);

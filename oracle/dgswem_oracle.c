@@ -97,6 +97,7 @@ typedef struct oracle {
     double *dp_node;                  /* [ne][3] */
     /* node -> element table */
     int *el_count, *eletab_off, *eletab;
+    double *etamax;          /* ETAMAX, the running maximum modal2nodal keeps (write_results.F:164) */
     /* state */
     double *ze, *qx, *qy;             /* [(nrk+1)][ne][dofh] */
     double *rhs_ze, *rhs_qx, *rhs_qy; /* [nrk][ne][dofh] */
@@ -556,6 +557,8 @@ int oracle_prep(oracle_t *o) {
     }
 
     /* node -> element table (ELETAB; order only affects modal2nodal round-off) */
+    o->etamax = (double *)xcalloc(o->np, sizeof(double));
+    for (int i = 0; i < o->np; i++) o->etamax[i] = -99999.0;      /* src/prep_DG.F:295 */
     o->el_count = (int *)xcalloc(o->np, sizeof(int));
     o->eletab_off = (int *)xcalloc(o->np + 1, sizeof(int));
     for (int e = 0; e < ne; e++) for (int k = 0; k < 3; k++) o->eletab_off[o->nm[3 * e + k] + 1]++;
@@ -1537,6 +1540,9 @@ void oracle_modal2nodal(oracle_t *o, double *eta2, double *uu2, double *vv2) {
             double depth = zd + hd;
             double fh = 1.0 / (o->nleq * depth + o->leq);
             e2 = e2 + area * zd;
+            /* the reference compares after every wet element's contribution (write_results.F:164),
+               so ETAMAX also sees the partial sums */
+            if (o->etamax[i] < e2) o->etamax[i] = e2;
             u2 = u2 + area * xd * fh;
             v2 = v2 + area * yd * fh;
         }
@@ -1573,6 +1579,8 @@ void oracle_clear_rhs(oracle_t *o) {
     memset(o->rhs_qy, 0, n * sizeof(double));
 }
 void oracle_get_nodecode(const oracle_t *o, int *nc) { memcpy(nc, o->nodecode, o->np * sizeof(int)); }
+void oracle_get_etamax(const oracle_t *o, double *etamax) { memcpy(etamax, o->etamax, o->np * sizeof(double)); }
+void oracle_set_etamax(oracle_t *o, const double *etamax) { memcpy(o->etamax, etamax, o->np * sizeof(double)); }
 int oracle_num_edges(const oracle_t *o, int *counts) {
     counts[0] = o->nedges; counts[1] = o->nieds; counts[2] = o->nleds; counts[3] = o->needs; counts[4] = o->nfeds;
     return o->nedges;
@@ -1638,7 +1646,7 @@ void oracle_free(oracle_t *o) {
                     o->dbathdy, o->sfac_elem, o->xfac, o->yfac, o->srfac, o->bathed, o->sfaced, o->edgeq,
                     o->dp_node, o->el_count, o->eletab_off, o->eletab, o->ze, o->qx, o->qy, o->rhs_ze,
                     o->rhs_qx, o->rhs_qy, o->ze_edge, o->qx_edge, o->qy_edge, o->wdflg, o->nodecode,
-                    o->zemin, o->zemax, o->qxmin, o->qxmax, o->qymin, o->qymax};
+                    o->zemin, o->zemax, o->qxmin, o->qxmax, o->qymin, o->qymax, o->etamax};
     for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; k++) free(ptrs[k]);
     free(o);
 }

@@ -44,6 +44,8 @@ def lib():
         L.oracle_get_rhs.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
         L.oracle_clear_rhs.argtypes = [C.c_void_p]
         L.oracle_get_nodecode.argtypes = [C.c_void_p, _ip]
+        L.oracle_get_etamax.argtypes = [C.c_void_p, _dp]
+        L.oracle_set_etamax.argtypes = [C.c_void_p, _dp]
         L.oracle_num_edges.argtypes = [C.c_void_p, _ip]
         L.oracle_num_edges.restype = C.c_int
         L.oracle_get_edges.argtypes = [C.c_void_p, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _dp]
@@ -175,6 +177,15 @@ class Oracle:
         nc = np.empty(self.np_, dtype=np.int32)
         lib().oracle_get_nodecode(self.h, nc)
         return e, u, v, nc
+
+    def etamax(self):
+        """ETAMAX as modal2nodal has kept it so far (-99999 before the first call)."""
+        out = np.empty(self.np_)
+        lib().oracle_get_etamax(self.h, out)
+        return out
+
+    def set_etamax(self, etamax):
+        lib().oracle_set_etamax(self.h, np.ascontiguousarray(etamax, dtype=np.float64))
 
     def edges(self):
         L = lib()

@@ -34,7 +34,7 @@ def load_case(name):
 
 
 # (fixture case, steps) pairs holding dgswem_serial FP64 states
-GOLDEN_STATES = [("c1", 1), ("c1", 10), ("c1", 1000), ("c1var", 200), ("delaunay4k", 100),
+GOLDEN_STATES = [("c1", 1), ("c1", 10), ("c1", 1000), ("c1", 8640), ("c1var", 200), ("delaunay4k", 100),
                  ("delaunay4k", 1000), ("quarter_annular", 100), ("quarter_annular", 2000),
                  ("wetdry", 50), ("wetdry", 500), ("mass_conservation", 1000), ("storm3k", 100),
                  ("storm3k", 500), ("delaunay4k_nolim", 300)]

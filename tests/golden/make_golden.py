@@ -7,7 +7,8 @@ Run in the build container (needs oracle/_ref/dgswem_serial, built by oracle/bui
 Cases taken from the reference's own tests also keep their input files and the last snapshot of
 the reference's fort.63.true (<case>_fort63_last.npz).
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # every fixture
+    python tests/golden/make_golden.py c1_8640    # only the named groups
 """
 from __future__ import annotations
 
@@ -122,19 +123,44 @@ def storm_case():
     return case
 
 
-def main() -> None:
+def maxele_case(name: str, case, nsteps: int) -> None:
+    """maxele.63 (ETAMAX, src/write_results.F:618-624) and the last fort.63 snapshot of a run whose
+    global elevation output fires at its last step."""
+    case.params.nt = nsteps
+    with tempfile.TemporaryDirectory() as d:
+        mesh.write_run_dir(d, case, dghot=0, dghotspool=nsteps, nspoolge=nsteps)
+        run_ref(d)
+        vals = np.loadtxt(os.path.join(d, "maxele.63"), skiprows=1)
+        t, it, eta = fortio.read_fort63(os.path.join(d, "fort.63"))
+        assert it[-1] == nsteps, it
+        np.savez_compressed(os.path.join(HERE, f"{name}_maxele_step{nsteps}.npz"), etamax=vals,
+                            eta=eta[-1], it=np.int64(nsteps))
+        print(f"  {name}: maxele.63 after step {nsteps} written")
+
+
+def delaunay4k():
+    return mesh.delaunay_basin(n_elem=4000, seed=1, lx=6.0e4, ly=3.6e4, depth_range=(2.0, 30.0),
+                               cfl=0.3)
+
+
+def main(which=None) -> None:
     if not os.path.exists(BIN):
         raise SystemExit(f"{BIN} missing: run oracle/build_ref.sh first")
-    generated_case("c1", mesh.structured_basin(), [1, 10, 1000])
+    if which:   # a subset: python tests/golden/make_golden.py c1_8640 maxele
+        if "c1_8640" in which:
+            generated_case("c1", mesh.structured_basin(), [8640])
+        if "maxele" in which:
+            maxele_case("delaunay4k", delaunay4k(), 100)
+        return
+    generated_case("c1", mesh.structured_basin(), [1, 10, 1000, 8640])
+    maxele_case("delaunay4k", delaunay4k(), 100)
     # linear friction (NOLIBF=0 needs a fort.15 TAU, so no Manning attribute), Coriolis, limiter and
     # a flux-specified side
     generated_case("c1var", mesh.structured_basin(nx=24, ny=12, slopeflag=5, cori=1.0e-4, nolibf=0,
                                                   manning=None, flow_side=True, flow_amp=2.0,
                                                   depth=8.0),
                    [200])
-    generated_case("delaunay4k", mesh.delaunay_basin(n_elem=4000, seed=1, lx=6.0e4, ly=3.6e4,
-                                                     depth_range=(2.0, 30.0), cfl=0.3),
-                   [100, 1000])
+    generated_case("delaunay4k", delaunay4k(), [100, 1000])
     # wetting and drying without the slope limiter (WETDRY fused into the stage)
     generated_case("delaunay4k_nolim", mesh.delaunay_basin(n_elem=4000, seed=2, lx=6.0e4, ly=3.6e4,
                                                            depth_range=(2.0, 30.0), cfl=0.3,
@@ -147,4 +173,4 @@ def main() -> None:
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

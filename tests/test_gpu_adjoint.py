@@ -177,6 +177,33 @@ def test_adjoint_rejects_runs_whose_wetting_and_drying_acts():
     s.close()
 
 
+def test_adjoint_refusal_restores_the_wet_dry_flags():
+    """A closed basin whose shoreline moves: WETDRY changes one element's flag at step 52 of the
+    60-step window (the oracle run below shows it).  The refused gradient call must leave both the
+    modal state and the wet/dry flags as they were, so the next step continues from there."""
+    from oracle.oracle import Oracle
+    from test_gpu_edge_cases import closed, wetdry_state
+    case = closed(12, 6, 1, nolifa=2, slopeflag=5)
+    case.dp = 2.0 - 0.0004 * case.x
+    st = wetdry_state(case, 2.0)
+    o = Oracle(case, threads=4)
+    o.set_state(*st)
+    o.step(60)
+    assert (o.state()[3] != st[3]).any()       # the window does change a flag
+    s = gpu.GpuSolver(case)
+    s.set_state(*st)
+    s.set_gauges(np.array([1, 40], np.int32), np.array([-0.5, -0.5]), np.array([-0.5, -0.5]))
+    with pytest.raises(gpu.GpuError, match="wetting/drying changed"):
+        s.gradient(np.zeros((60, 2)))
+    for a, b in zip(s.state(), st):
+        np.testing.assert_array_equal(a, b)
+    # and stepping on from the restored state equals the oracle from the same state
+    s.step(60)
+    for a, b in zip(s.state(), o.state()):
+        np.testing.assert_array_equal(a, b)
+    s.close()
+
+
 def limiter_case(p=1, wetdry=False, n_elem=2000, seed=34, nsteps=60):
     """All-wet shallow basin with the slope limiter on (and WETDRY on but inactive when wetdry)."""
     return mesh.delaunay_basin(n_elem=n_elem, seed=seed, lx=6.0e4, ly=3.6e4, depth_range=(2.0, 10.0),

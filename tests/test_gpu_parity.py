@@ -1,10 +1,11 @@
 """GPU step (through the C ABI) against the reference's FP64 states and the oracle.
 
 Tolerances: parity is a relative L2 error <= 1e-10 in elevation and velocity with
-identical wet/dry masks.  The GPU uses the device's cos and evaluates Manning's DEPTH**(1/3) as a
-cube root; both can differ from glibc's pow/cos in the last bit, so states that involve tides or
-Manning friction are not always bit-identical; all other arithmetic is rounded operation by
-operation like the reference (-ffp-contract=off).
+identical wet/dry masks.  Every operation is rounded like the reference's (-ffp-contract=off, its
+operation order); the step's transcendental terms (the tidal cosine, Manning's DEPTH**(1/3)) are
+correctly rounded on the GPU, while glibc's cos and pow round a small share of arguments (about
+0.1 %) the other way, so states that involve tides or Manning friction can differ from the
+reference's in the last bits; states without them agree bit for bit (tested below).
 """
 import numpy as np
 import pytest
@@ -80,6 +81,34 @@ def test_modal2nodal_matches_oracle(name):
         np.testing.assert_array_equal(gnc, onc)
     for a, b in ((ge, oe), (gu, ou), (gv, ov)):
         np.testing.assert_allclose(a, b, rtol=1e-11, atol=1e-13 * max(np.abs(b).max(), 1.0))
+    g.close()
+
+
+@pytest.mark.parametrize("name", ["wetdry", "delaunay4k"])
+def test_device_nodal_values_and_etamax_match_oracle(name):
+    """modal2nodal kept on the device every step (dg_gpu_nodal_update) with ETAMAX raised there,
+    copied once at the end (dg_gpu_get_nodal), against the oracle's modal2nodal run every step.
+    Both visit each node's elements in ascending element order and raise ETAMAX after every wet
+    contribution (so partial sums count, as in src/write_results.F:164)."""
+    case = load_case(name)
+    steps = 60
+    g = gpu.GpuSolver(case)
+    g.set_etamax(np.full(case.np_, -99999.0))
+    o = Oracle(case, threads=4)
+    for _ in range(steps):
+        g.step(1)
+        g.nodal_update()
+        o.step(1)
+        o.nodal()
+    ge, gu, gv, gnc, gem = g.get_nodal()
+    oe, ou, ov, onc = o.nodal()      # one more call: the values are those of the last step
+    oem = o.etamax()
+    np.testing.assert_array_equal(gnc, onc)
+    for a, b in ((ge, oe), (gu, ou), (gv, ov)):
+        np.testing.assert_allclose(a, b, rtol=1e-11, atol=1e-13 * max(np.abs(b).max(), 1.0))
+    # the oracle's extra call repeats the last step's partial sums, which cannot raise ETAMAX again
+    np.testing.assert_allclose(gem, oem, rtol=1e-11, atol=1e-13)
+    assert (gem > -99999.0).any()
     g.close()
 
 
