@@ -14,9 +14,12 @@ Prints one JSON line.  `value` = NE * 3 * Np * NRK * steps / wall time of the ti
 state resident in HBM.  `roofline` is for the stage kernel: algorithmic bytes per launch
 (300 B per element-stage for p=1, 480 B for p=2) over its mean launch time from HIP events
 recorded on the library's stream during the timed region (the launches of every 10th step are
-bracketed, so the events barely perturb the wall time that `value` comes from).  `cpu_baseline` times the CPU oracle
-(a bit-exact restatement of dgswem_serial's step) with OpenMP on this host's cores on a short
-sample of the same mesh.
+bracketed -- of every step on short runs, at least 20 launches -- so the events barely perturb the
+wall time that `value` comes from); `roofline.step_level` is the same byte count over the whole
+timed loop.  `cpu_baseline` times dgswem_serial, built from the reference's sources, on one core of
+this host (two runs differenced to remove set-up), with the CPU model and a full-host estimate
+(rate x cores); `cpu_port` times the CPU oracle (a bit-exact restatement of dgswem_serial's step)
+with OpenMP on this host's cores on a short sample of the same mesh.
 
 The same line carries the other two parts of the metric:
 * `ensemble` -- 64 uneven spherical meshes (NE log-uniform in 20k-300k) under their own Holland
@@ -67,6 +70,48 @@ def cpu_baseline(case, sample_steps: int, threads: int):
     dt = time.perf_counter() - t0
     dof = case.ne * 3 * case.dofh * case.dg.rk_stage * sample_steps
     return dof / dt, dt
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def reference_serial_rate(case, n1: int, n2: int):
+    """dgswem_serial (built from the reference's sources into oracle/_ref; a serial program) on one
+    core: two runs of the same mesh with n1 and n2 steps; the difference of their wall times
+    removes the set-up (reading the mesh, PREP_DG).  Returns (DOF-stage/s, seconds per step, total
+    seconds), or None when the binary is not there.  Called before this process touches the GPU
+    (the runs are child processes)."""
+    import subprocess
+    import tempfile
+    from dgswem_amd import mesh as _mesh
+    exe = os.path.join(ROOT, "oracle", "_ref", "dgswem_serial")
+    if not os.path.exists(exe):
+        return None
+    walls = []
+    t_all = time.perf_counter()
+    with tempfile.TemporaryDirectory() as d:
+        for n in (n1, n2):
+            case.params.nt = n
+            rd = os.path.join(d, f"run{n}")
+            _mesh.write_run_dir(rd, case, nspoolge=n)
+            env = dict(os.environ, OMP_NUM_THREADS="1")
+            t0 = time.perf_counter()
+            r = subprocess.run([exe], cwd=rd, capture_output=True, text=True, env=env)
+            walls.append(time.perf_counter() - t0)
+            if r.returncode != 0 or "CPU_TIME" not in r.stdout:
+                log(f"dgswem_serial failed: {r.stdout[-300:]} {r.stderr[-300:]}")
+                return None
+    per_step = (walls[1] - walls[0]) / (n2 - n1)
+    dof = case.ne * 3 * case.dofh * case.dg.rk_stage
+    return dof / per_step, per_step, time.perf_counter() - t_all
 
 
 def ensemble_leg(gpu, mesh, n_members: int, steps: int, warmup: int, seed: int):
@@ -125,7 +170,7 @@ def adjoint_leg(gpu, mesh, n_elem: int, steps: int, seed: int):
     basin without limiter: wall(forward + checkpointed adjoint) / wall(forward)."""
     import numpy as np
     from dgswem_amd import adjoint
-    case = mesh.delaunay_basin(n_elem=n_elem, seed=seed, depth_range=(5.0, 300.0), shelf=False,
+    case = mesh.delaunay_basin(n_elem=n_elem, seed=seed, depth_range=(10.0, 3000.0), shelf=False,
                                wetdry=False, slopeflag=0, nsteps=steps, cf=1e-4)
     prob = adjoint.twin_problem(case, nsteps=steps, ng=50, seed=seed)
     s = prob.solver
@@ -143,7 +188,7 @@ def adjoint_leg(gpu, mesh, n_elem: int, steps: int, seed: int):
     return {"cost_ratio": grad / fwd, "forward_s": fwd, "gradient_s": grad, "elements": case.ne,
             "nodes": case.np_, "steps": steps, "checkpoint_interval": interval, "loss": J2,
             "params": 2 * case.np_,
-            "workload": "all-wet seeded Delaunay basin (GRF depth 5-300 m, CF 1e-4), p=1, no limiter, "
+            "workload": "all-wet seeded Delaunay basin (GRF depth 10-3000 m, CF 1e-4), p=1, no limiter, "
                         "50 gauges, misfit against a perturbed-Manning twin; gradient w.r.t. "
                         "per-node Manning n and depth"}
 
@@ -241,6 +286,16 @@ def main() -> int:
                                cfl=0.3 if args.p == 1 else 0.15, nsteps=args.steps + args.warmup)
     log(f"mesh: {case.ne} elements, {case.np_} nodes, dt {case.params.dtdp:.4f} s "
         f"({time.perf_counter() - t0:.1f} s)")
+    ne = case.ne
+    model = cpu_model()
+    threads = len(os.sched_getaffinity(0))
+    ref = None
+    if not args.no_cpu:
+        # the reference's own serial solver on one core (steps differenced), before any GPU work
+        n1, n2 = (2, 8) if ne >= 500_000 else (5, 45)
+        log(f"cpu baseline: dgswem_serial, 1 core, {n1} and {n2} steps ({model})")
+        ref = reference_serial_rate(case, n1, n2)
+        case.params.nt = args.steps + args.warmup
     g = gpu.GpuSolver(case)
     log("library initialised")
     g.step(args.warmup)
@@ -251,7 +306,9 @@ def main() -> int:
     else:
         block = g.autotune()
     log(f"stage kernel block size {block} ({'set' if args.block else 'autotuned'})")
-    g.profile(True, every=PROFILE_EVERY)
+    # time the launches of every n-th step, with at least 20 stage launches sampled
+    every = max(1, min(PROFILE_EVERY, (args.steps * case.dg.rk_stage) // 40))
+    g.profile(True, every=every)
     g.synchronize()
     t1 = time.perf_counter()
     g.step(args.steps, sync=False)
@@ -269,6 +326,7 @@ def main() -> int:
     avg_s = tot_ms / 1e3 / max(launches, 1)
     alg_bytes = B_PER_ELEM_STAGE[args.p] * ne
     achieved = alg_bytes / avg_s / 1e9
+    step_gbs = alg_bytes * nrk * args.steps / wall / 1e9
     kernels = {}
     for k, (nm, ms, cnt) in stats.items():
         if cnt:
@@ -277,15 +335,25 @@ def main() -> int:
     log(f"timed {args.steps} steps: {wall * 1e3:.1f} ms, stage kernel {avg_s * 1e6:.1f} us/launch, "
         f"{achieved:.0f} GB/s algorithmic, finite={finite}")
 
-    cpu = None
+    cpu = port = None
     if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0))
+        # the OpenMP oracle on every core this process may use, beside dgswem_serial's rate
+        if ref is not None:
+            rate, per_step, secs = ref
+            cpu = {"value": rate, "unit": "DOF-stage updates/s", "cores": 1, "kind": "reference",
+                   "sample": f"dgswem_serial (built from the reference's sources) on the same {ne}-element "
+                             f"mesh, runs of {n1} and {n2} steps differenced ({per_step:.3f} s/step, "
+                             f"{secs:.1f} s in all)", "cpu_model": model, "host_threads": threads,
+                   "full_host_estimate": rate * threads}
         steps_cpu = args.cpu_steps or max(2, int(150 * 1_000_000 / max(ne, 1)))
-        log(f"cpu baseline: oracle, {threads} threads, {steps_cpu} steps")
-        rate, secs = cpu_baseline(case, steps_cpu, threads)
-        cpu = {"value": rate, "unit": "DOF-stage updates/s", "cores": threads, "kind": "port",
-               "sample": f"{steps_cpu} steps of the same {ne}-element mesh ({secs:.1f} s) with the "
-                         f"OpenMP oracle (bit-exact restatement of dgswem_serial's step)"}
+        log(f"cpu port: oracle, {threads} threads, {steps_cpu} steps")
+        prate, psecs = cpu_baseline(case, steps_cpu, threads)
+        port = {"value": prate, "unit": "DOF-stage updates/s", "cores": threads, "kind": "port",
+                "sample": f"{steps_cpu} steps of the same {ne}-element mesh ({psecs:.1f} s) with the "
+                          f"OpenMP oracle (bit-exact restatement of dgswem_serial's step)",
+                "cpu_model": model}
+        if cpu is None:
+            cpu, port = port, None
 
     g.close()
     ens = None if args.no_ensemble else ensemble_leg(gpu, mesh, args.ens_members, args.ens_steps,
@@ -317,7 +385,10 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(args.p),
                      "kernel": name, "avg_launch_us": avg_s * 1e6, "timed_launches": launches,
-                     "algorithmic_bytes_per_launch": alg_bytes},
+                     "algorithmic_bytes_per_launch": alg_bytes,
+                     # the metric's own definition: B_p per element-stage over the whole timed loop
+                     "step_level": {"achieved": step_gbs, "frac": step_gbs / HBM_PEAK_GBS},
+                     "limited_by": "FP64 VALU issue (HBM traffic at the algorithmic minimum)"},
         "kernels": kernels,
         "cpu_baseline": cpu,
         "ensemble": ens,
@@ -326,6 +397,11 @@ def main() -> int:
     }
     if cpu:
         out["speedup_vs_cpu_baseline"] = value / cpu["value"]
+        if cpu.get("full_host_estimate"):
+            out["speedup_vs_full_host_estimate"] = value / cpu["full_host_estimate"]
+    if port:
+        out["cpu_port"] = port
+        out["speedup_vs_cpu_port"] = value / port["value"]
     print(json.dumps(out), flush=True)
     return 0
 

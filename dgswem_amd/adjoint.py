@@ -134,22 +134,29 @@ def _make_function():
     import torch
 
     class GaugeMisfit(torch.autograd.Function):
+        """Forward: when a gradient will be asked for, one dg_adj_gradient call runs the
+        checkpointed forward sweep and the reverse sweep together and returns the loss with the
+        gradient, which backward then only scales; otherwise the plain forward model."""
+
         @staticmethod
-        def forward(ctx, manning, dp, problem: MisfitProblem):
+        def forward(ctx, manning, dp, problem: MisfitProblem, want_grad: bool):
             mn = manning.detach().to("cpu", torch.float64).numpy()
             d = dp.detach().to("cpu", torch.float64).numpy()
-            ctx.problem = problem
-            ctx.params = (mn.copy(), d.copy())
             ctx.devices = (manning.device, dp.device)
-            return torch.tensor(problem.loss(mn, d), dtype=torch.float64)
+            if want_grad:
+                J, gn, gdp = problem.gradient(mn, d)
+                ctx.grads = (gn, gdp)
+            else:
+                J = problem.loss(mn, d)
+                ctx.grads = None
+            return torch.tensor(J, dtype=torch.float64)
 
         @staticmethod
         def backward(ctx, grad_out):
-            mn, d = ctx.params
-            _, gn, gdp = ctx.problem.gradient(mn, d)
+            gn, gdp = ctx.grads
             g = float(grad_out)
             return (torch.from_numpy(g * gn).to(ctx.devices[0]),
-                    torch.from_numpy(g * gdp).to(ctx.devices[1]), None)
+                    torch.from_numpy(g * gdp).to(ctx.devices[1]), None, None)
 
     return GaugeMisfit
 
@@ -159,7 +166,9 @@ _FN = None
 
 def gauge_misfit(manning, dp, problem: MisfitProblem):
     """torch front end: differentiable loss of torch float64 tensors `manning` and `dp` (per node)."""
+    import torch
     global _FN
     if _FN is None:
         _FN = _make_function()
-    return _FN.apply(manning, dp, problem)
+    want = torch.is_grad_enabled() and (manning.requires_grad or dp.requires_grad)
+    return _FN.apply(manning, dp, problem, want)

@@ -491,21 +491,11 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
                     hbar = bar[6] + bar[7];
                     if (eirik) hbar += (hb_in > 0.0 ? 1.0 : (hb_in < 0.0 ? -1.0 : 0.0)) * bar[1];
                 } else {
-                    double qx_ex = 0.0, qy_ex = 0.0, qn_ext = 0.0;
-                    const double xe = T.xegp[I];
-                    for (int jj = 0; jj < a.fo.nffr; jj++) {
-                        const size_t t = ((size_t)sid * a.fo.rows + row) * a.fo.nffr + jj;
-                        const double argj = a.fo.fargj[t];
-                        const double rff = a.fo.frff[t];
-                        const size_t o = ((size_t)nb * a.fo.nffr + jj) * 2;
-                        const double q1 = a.fo.qnam[o], q2 = a.fo.qnam[o + 1];
-                        const double p1 = a.fo.qnph[o], p2 = a.fo.qnph[o + 1];
-                        const double qnam_gp = 0.5 * (q1 + q2) + 0.5 * (q2 - q1) * xe;
-                        const double qnph_gp = 0.5 * (p1 + p2) + 0.5 * (p2 - p1) * xe;
-                        qn_ext = qn_ext + qnam_gp * rff * cos(argj - qnph_gp);
-                        qx_ex = -(ty * qn_ext) / (nx * ty - ny * tx);
-                        qy_ex = -(-tx * qn_ext) / (nx * ty - ny * tx);
-                    }
+                    // the step's specified normal flux at this point (flow_kernel)
+                    const double qn_ext = a.fo.fz[(size_t)a.irk * a.fo.fzn + (size_t)nb * NEGP + I];
+                    const double qt_ext = 0.0;
+                    const double qx_ex = -(ty * qn_ext - ny * qt_ext) / (nx * ty - ny * tx);
+                    const double qy_ex = -(-tx * qn_ext + nx * qt_ext) / (nx * ty - ny * tx);
                     llf_flux_adj<NL>(ph, ze_in, ze_in, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_in, sf, sf, nx, ny, ph.g, fb, bar);
                     zb = bar[0] + bar[1];
                     xb = bar[2];

@@ -85,6 +85,25 @@ struct Member {
     bool met = false;
     HollandTrack track;
     std::vector<double> mxc, myc, mcosy, mcor;  // [np] node lon/lat (deg), cos(lat), Coriolis
+    // tidal potential (NTIP != 0): constituents, node data, SAL terms [ntif][np] (empty: zero)
+    int ntif = 0;
+    double reftim = 0.0;
+    std::vector<double> tpk, amigt, etrf, fft, facet, pert, tslam, tccs, ts2s, tsamp, tspha;
+    std::vector<int> tna;
+    // non-periodic boundaries: fort.19 / fort.20 records, the ocean edges' end nodes (NEDNO order),
+    // the NVEL boundary codes, and the replay state of the reference's record reading
+    std::vector<int> onode;                     // [needs][2] 0-based
+    bool eser = false, fser = false;
+    int neta = 0, nvel = 0, nflux = 0, e_nrec = 0, f_nrec = 0;
+    double etiminc = 0.0, ftiminc = 0.0;
+    std::vector<int> nbd, lbcodei;
+    std::vector<double> esbin, qnin;            // [nrec][neta], [nrec][nflux]
+    struct Series {
+        int it = 0;                             // steps whose stages have been applied
+        double t1 = 0.0, t2 = 0.0;
+        int next = 2;
+        std::vector<double> v1, v2;
+    } es, fs;
 };
 
 struct Ctx {
@@ -136,6 +155,19 @@ struct Ctx {
     int tzn = 0;                                 // tidal terms per stage
     int hstep = 0;                               // the same index for eager (uncaptured) launches
     int rows = 0;                                // table rows per scenario
+    // specified normal flux of the current step at every flux-edge point, and each flux edge's
+    // scenario (flow_kernel)
+    DevBuf<double> fz;
+    DevBuf<uint16_t> fsid;
+    int fzn = 0;
+    // non-periodic boundaries (single runs): the ramped end-node values of every stage row
+    bool eser = false, fser = false;
+    DevBuf<double> ets, qts;
+    // tidal potential (single runs): per-row constituent terms, node data and TIP2 of the step
+    bool tip = false;
+    int ntif = 0;
+    DevBuf<double> tiprow, tslam, tccs, ts2s, tsamp, tspha, tip2;
+    DevBuf<int> tna;
     // NWS=8 meteorological forcing
     bool met = false;
     DevBuf<double> ramp, mrow, mxc, myc, mcosy, mcor, mwx, mwy, mwm, mcd, mprn;
@@ -264,7 +296,22 @@ Forcing forcing(Ctx& c, bool captured = false) {
     F.argj = c.argj.p; F.rff = c.rff.p; F.fargj = c.fargj.p; F.frff = c.frff.p;
     F.ramp = c.ramp.p;
     F.tz = c.tz.p; F.tzn = c.tzn;
+    F.fz = c.fz.p; F.fzn = c.fzn;
+    F.ets = c.eser ? c.ets.p : nullptr;
+    F.qts = c.fser ? c.qts.p : nullptr;
+    F.nfeds = c.foff[c.S];
     return F;
+}
+
+Tip tip_args(Ctx& c) {
+    Tip t{};
+    t.np = c.np; t.ntif = c.ntif;
+    t.row = c.tiprow.p; t.na = c.tna.p;
+    t.slam = c.tslam.p; t.ccsfea = c.tccs.p; t.s2sfea = c.ts2s.p;
+    t.salt_amp = c.tsamp.n ? c.tsamp.p : nullptr;
+    t.salt_pha = c.tspha.n ? c.tspha.p : nullptr;
+    t.tip2 = c.tip2.p;
+    return t;
 }
 
 Met met_args(Ctx& c) {
@@ -397,19 +444,9 @@ int build_member(Member& m, const dg_params* prm, const RefTables& T, int np, in
                 }
         }
     }
-    // ---- node -> element CSR in element order (entries: element*4 + local vertex)
-    m.n2e_off.assign(np + 1, 0);
-    m.n2e.resize((size_t)3 * ne);
-    for (int e = 0; e < ne; e++) for (int v = 0; v < 3; v++) m.n2e_off[m.nm[(size_t)v * ne + e] + 1]++;
-    for (int i = 0; i < np; i++) m.n2e_off[i + 1] += m.n2e_off[i];
-    {
-        std::vector<int> fill(np, 0);
-        for (int e = 0; e < ne; e++)
-            for (int v = 0; v < 3; v++) {
-                const int n = m.nm[(size_t)v * ne + e];
-                m.n2e[m.n2e_off[n] + fill[n]++] = (e << 2) | v;
-            }
-    }
+    // ---- node -> element CSR (entries: element*4 + local vertex) in the reference's ELETAB order,
+    // the order modal2nodal sums in; the limiter's node min/max does not depend on it
+    eletab_order(np, ne, m.nm.data(), x, y, m.n2e_off, m.n2e);
     // ---- cold start: ZE=QX=QY=0 except the initial dry check (src/prep_DG.F:1214-1300)
     m.ze0.assign((size_t)dof * ne, 0.0);
     m.wd0.assign(ne, 1);
@@ -484,7 +521,19 @@ int upload_forcing(Ctx& c) {
         HIPCHK(c.osid.upload(os.data(), os.size(), c.stream));
         HIPCHK(c.tz.alloc((size_t)c.rk.nrk * c.tzn));
     }
+    c.fzn = tot_f * c.negp;
+    if (c.fzn > 0) {
+        std::vector<uint16_t> fsd(tot_f);
+        for (int s = 0; s < c.S; s++)
+            for (int L = c.foff[s]; L < c.foff[s + 1]; L++) fsd[L] = (uint16_t)s;
+        HIPCHK(c.fsid.upload(fsd.data(), fsd.size(), c.stream));
+        HIPCHK(c.fz.alloc((size_t)c.rk.nrk * c.fzn));
+    }
     const size_t rows = (size_t)kTableChunk * c.rk.nrk;
+    c.eser = c.fser = false;
+    for (auto& m : c.mem) { c.eser = c.eser || m.eser; c.fser = c.fser || m.fser; }
+    if (c.eser) HIPCHK(c.ets.alloc((size_t)c.S * rows * std::max(c.tzn, 1)));
+    if (c.fser) HIPCHK(c.qts.alloc((size_t)c.S * rows * tot_f * 2));
     HIPCHK(c.argj.alloc((size_t)c.S * rows * std::max(c.nbfr, 1)));
     HIPCHK(c.rff.alloc((size_t)c.S * rows * std::max(c.nbfr, 1)));
     HIPCHK(c.fargj.alloc((size_t)c.S * rows * std::max(c.nffr, 1)));
@@ -712,23 +761,95 @@ int upload_initial_state(Ctx& c) {
     return 0;
 }
 
+// The non-periodic boundary records the reference reads as time goes on: its state (the two
+// records in use and their times) before step 1, and the advance of one stage.  An advance
+// happens at a stage whose TIME_A is past the second record's time (src/DG_hydro_timestep.F:153-160
+// for fort.19, 174-185 for fort.20); at most one per stage, as in the reference.
+void series_reset(const Member& m, Member::Series& S, bool elev) {
+    const int n = elev ? m.neta : m.nflux;
+    const double* rec = elev ? m.esbin.data() : m.qnin.data();
+    S.it = 0;
+    S.t1 = m.prm.statim * 86400.0;
+    S.t2 = S.t1 + (elev ? m.etiminc : m.ftiminc);
+    S.next = 2;
+    if (elev) {
+        S.v1.assign(rec, rec + n);
+        S.v2.assign(rec + n, rec + 2 * n);
+    } else {
+        // QNIN1/QNIN2 over the NVEL nodes (zero where the code is not a flux code), one slot past
+        // NVEL for the edge after the last flux node
+        S.v1.assign(m.nvel + 1, 0.0);
+        S.v2.assign(m.nvel + 1, 0.0);
+        for (int j = 0, k = 0; j < m.nvel; j++)
+            if (m.lbcodei[j] == 2 || m.lbcodei[j] == 12 || m.lbcodei[j] == 22) { S.v1[j] = rec[k]; S.v2[j] = rec[n + k]; k++; }
+    }
+}
+
+int series_advance(const Member& m, Member::Series& S, bool elev, double time_a) {
+    if (!(time_a > S.t2)) return 0;
+    const int nrec = elev ? m.e_nrec : m.f_nrec;
+    S.t1 = S.t2;
+    S.t2 = S.t1 + (elev ? m.etiminc : m.ftiminc);
+    if (S.next >= nrec)
+        return fail(std::string(elev ? "fort.19" : "fort.20") + ": the time series ends before the run");
+    if (elev) {
+        S.v1 = S.v2;
+        const double* r = m.esbin.data() + (size_t)S.next * m.neta;
+        S.v2.assign(r, r + m.neta);
+    } else {
+        const double* r = m.qnin.data() + (size_t)S.next * m.nflux;
+        for (int j = 0, k = 0; j < m.nvel; j++) {
+            S.v1[j] = S.v2[j];
+            S.v2[j] = 0.0;
+            if (m.lbcodei[j] == 2 || m.lbcodei[j] == 12 || m.lbcodei[j] == 22) S.v2[j] = r[k++];
+        }
+    }
+    S.next++;
+    return 0;
+}
+
+// Bring a series to the state after step `it` (replayed from the start when an earlier step is
+// asked for, as the adjoint's recomputation and the autotuner do).
+int series_seek(Member& m, Member::Series& S, bool elev, int it, int nrk) {
+    if (S.it > it || S.v1.empty()) series_reset(m, S, elev);
+    while (S.it < it) {
+        const double time_a = (S.it + 1) * m.prm.dtdp + m.prm.statim * 86400.0;
+        for (int irk = 0; irk < nrk; irk++)
+            if (series_advance(m, S, elev, time_a)) return 1;
+        S.it++;
+    }
+    return 0;
+}
+
 // Per-stage forcing rows for steps it..it+n-1 of every member (host arithmetic of
-// src/DG_hydro_timestep.F:79-100, src/ocean_edge_hydro.F:59-75, src/flow_edge_hydro.F:45-62).
+// src/DG_hydro_timestep.F:79-200, src/ocean_edge_hydro.F:59-75, src/flow_edge_hydro.F:45-62,
+// src/tidal_potential.F:26-48).
 int build_stage_tables(Ctx& c, int it, int n) {
-    if (c.nbfr == 0 && c.nffr == 0 && !c.met) return 0;
+    if (c.nbfr == 0 && c.nffr == 0 && !c.met && !c.tip) return 0;
     if (n > kTableChunk) return fail("internal: table chunk overflow");
     const int nrk = c.rk.nrk, rows = c.rows;
+    const int tot_f = c.foff[c.S];
+    const bool ramped = c.met || c.tip;
     std::vector<double> aj((size_t)c.S * rows * std::max(c.nbfr, 1), 0.0), rf(aj.size(), 0.0);
     std::vector<double> fa((size_t)c.S * rows * std::max(c.nffr, 1), 0.0), fr(fa.size(), 0.0);
-    std::vector<double> rp, mr;
-    if (c.met) { rp.assign((size_t)c.S * rows, 0.0); mr.assign((size_t)c.S * kTableChunk * MET_ROW, 0.0); }
+    std::vector<double> rp, mr, et, qt, tr;
+    if (ramped) rp.assign((size_t)c.S * rows, 0.0);
+    if (c.met) mr.assign((size_t)c.S * kTableChunk * MET_ROW, 0.0);
+    if (c.eser) et.assign((size_t)c.S * rows * std::max(c.tzn, 1), 0.0);
+    if (c.fser) qt.assign((size_t)c.S * rows * tot_f * 2, 0.0);
+    if (c.tip) tr.assign((size_t)c.S * rows * c.ntif * 3, 0.0);
+    std::vector<double> eta_bc;
     std::string err;
     for (int sc = 0; sc < c.S; sc++) {
-        const Member& m = c.mem[sc];
+        Member& m = c.mem[sc];
+        if (m.eser && series_seek(m, m.es, true, it - 1, nrk)) return 1;
+        if (m.fser && series_seek(m, m.fs, false, it - 1, nrk)) return 1;
+        if (m.eser) eta_bc.assign(m.np, 0.0);
         for (int s = 0; s < n; s++) {
             const int IT = it + s;
             const double itdt = IT * m.prm.dtdp;
             const double time_a = itdt + m.prm.statim * 86400.0;
+            const double timeh = itdt + (m.prm.statim - m.reftim) * 86400.0;
             // storm scalars at TIME_A (MET_FORCING is called with TIME_A at every stage)
             if (c.met && !holland_row(m.track, time_a, m.prm.g, &mr[((size_t)sc * kTableChunk + s) * MET_ROW], err))
                 return fail(err);
@@ -738,18 +859,60 @@ int build_stage_tables(Ctx& c, int it, int n) {
                 double ramp = 1.0;
                 if (m.prm.nramp == 1)
                     ramp = std::tanh((2.0 * ((IT - 1) + c.rk.dtvd[irk]) * m.prm.dtdp / 86400.0) / m.prm.dramp);
-                if (c.met) rp[row] = ramp;
-                for (int jj = 0; jj < m.nbfr; jj++) {
-                    const int ncyc = (m.per[jj] == 0.0) ? 0 : (int)(timedg / m.per[jj]);
-                    aj[row * c.nbfr + jj] = m.amig[jj] * (timedg - ncyc * m.per[jj]) + m.face[jj];
-                    rf[row * c.nbfr + jj] = m.ff[jj] * ramp;
+                if (ramped) rp[row] = ramp;
+                if (m.ntif > 0) {
+                    const double timeh_dg = timeh - m.prm.dtdp + c.rk.dtvd[irk] * m.prm.dtdp;
+                    for (int j = 0; j < m.ntif; j++) {
+                        const int ncyc = (m.pert[j] == 0.0) ? 0 : (int)(timeh_dg / m.pert[j]);
+                        double* t = &tr[(row * c.ntif + j) * 3];
+                        t[0] = m.amigt[j] * (timeh_dg - ncyc * m.pert[j]) + m.facet[j];
+                        t[1] = ramp * m.etrf[j] * m.tpk[j] * m.fft[j];
+                        t[2] = ramp * m.fft[j];
+                    }
                 }
-                for (int jj = 0; jj < m.nffr; jj++) {
-                    const int ncyc = (m.fper[jj] == 0.0) ? 0 : (int)(timedg / m.fper[jj]);
-                    fa[row * c.nffr + jj] = m.famig[jj] * (timedg - ncyc * m.fper[jj]) + m.fface[jj];
-                    fr[row * c.nffr + jj] = m.fff[jj] * ramp;
+                if (m.eser) {
+                    if (series_advance(m, m.es, true, time_a)) return 1;
+                    const double etratio = (timedg - m.es.t1) / m.etiminc;
+                    for (int i = 0; i < m.neta; i++)
+                        eta_bc[m.nbd[i]] = ramp * (m.es.v1[i] + etratio * (m.es.v2[i] - m.es.v1[i]));
+                    for (int L = 0; L < m.needs; L++)
+                        for (int k = 0; k < 2; k++)
+                            et[row * c.tzn + (size_t)(c.ooff[sc] + L) * 2 + k] = eta_bc[m.onode[2 * L + k]];
+                } else {
+                    for (int jj = 0; jj < m.nbfr; jj++) {
+                        const int ncyc = (m.per[jj] == 0.0) ? 0 : (int)(timedg / m.per[jj]);
+                        aj[row * c.nbfr + jj] = m.amig[jj] * (timedg - ncyc * m.per[jj]) + m.face[jj];
+                        rf[row * c.nbfr + jj] = m.ff[jj] * ramp;
+                    }
+                }
+                if (m.fser) {
+                    // one term: ARGJ = 0, RFF = RAMPDG (src/flow_edge_hydro.F:147-149)
+                    fa[row * c.nffr] = 0.0;
+                    fr[row * c.nffr] = ramp;
+                    if (series_advance(m, m.fs, false, time_a)) return 1;
+                    const double qtratio = (timedg - m.fs.t1) / m.ftiminc;
+                    int nqeds = 0;
+                    for (int i = 0; i < m.nvel; i++) {
+                        const int code = m.lbcodei[i];
+                        if (!(code == 2 || code == 12 || code == 22)) continue;
+                        nqeds++;
+                        if (nqeds > m.nfeds) continue;
+                        const double* v1 = m.fs.v1.data();
+                        const double* v2 = m.fs.v2.data();
+                        double* q = &qt[(row * tot_f + c.foff[sc] + nqeds - 1) * 2];
+                        q[0] = ramp * (v1[i] + qtratio * (v2[i] - v1[i]));
+                        q[1] = ramp * (v1[i + 1] + qtratio * (v2[i + 1] - v1[i + 1]));
+                    }
+                } else {
+                    for (int jj = 0; jj < m.nffr; jj++) {
+                        const int ncyc = (m.fper[jj] == 0.0) ? 0 : (int)(timedg / m.fper[jj]);
+                        fa[row * c.nffr + jj] = m.famig[jj] * (timedg - ncyc * m.fper[jj]) + m.fface[jj];
+                        fr[row * c.nffr + jj] = m.fff[jj] * ramp;
+                    }
                 }
             }
+            if (m.eser) m.es.it = IT;
+            if (m.fser) m.fs.it = IT;
         }
     }
     // the previous chunk's launches may still read the tables
@@ -762,10 +925,11 @@ int build_stage_tables(Ctx& c, int it, int n) {
         HIPCHK(hipMemcpyAsync(c.fargj.p, fa.data(), fa.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
         HIPCHK(hipMemcpyAsync(c.frff.p, fr.data(), fr.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
     }
-    if (c.met) {
-        HIPCHK(hipMemcpyAsync(c.ramp.p, rp.data(), rp.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
-        HIPCHK(hipMemcpyAsync(c.mrow.p, mr.data(), mr.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
-    }
+    if (ramped) HIPCHK(hipMemcpyAsync(c.ramp.p, rp.data(), rp.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    if (c.met) HIPCHK(hipMemcpyAsync(c.mrow.p, mr.data(), mr.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    if (c.eser) HIPCHK(hipMemcpyAsync(c.ets.p, et.data(), et.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    if (c.fser) HIPCHK(hipMemcpyAsync(c.qts.p, qt.data(), qt.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    if (c.tip) HIPCHK(hipMemcpyAsync(c.tiprow.p, tr.data(), tr.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
     HIPCHK(hipStreamSynchronize(c.stream));
     return 0;
 }
@@ -857,14 +1021,31 @@ int launch_stage_v(Ctx& c, const StageArgs& a0, bool sph, bool fuse) {
     return 0;
 }
 
-// The step's tidal terms for all stages (before the step's stage launches; the forcing rows of the
-// step must be in the tables).
-int launch_tide(Ctx& c, bool capturing) {
-    if (c.tzn == 0) return 0;
-    const int n = c.rk.nrk * c.tzn;
-    hipLaunchKernelGGL(tide_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, forcing(c, capturing),
-                       (const uint16_t*)(c.S > 1 ? c.osid.p : nullptr), c.tz.p);
-    HIPCHK(hipGetLastError());
+// The step's boundary and potential forcing for all stages, before the step's stage launches (the
+// forcing rows of the step must be in the tables): the open-boundary elevation terms (tide_kernel),
+// the specified normal flux (flow_kernel) and the tidal potential at the nodes (tip_kernel).
+int launch_forcing(Ctx& c, bool capturing) {
+    const Forcing F = forcing(c, capturing);
+    if (c.tzn > 0) {
+        const int n = c.rk.nrk * c.tzn;
+        hipLaunchKernelGGL(tide_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, F,
+                           (const uint16_t*)(c.S > 1 ? c.osid.p : nullptr), c.tz.p);
+        HIPCHK(hipGetLastError());
+    }
+    if (c.fzn > 0) {
+        const int n = c.rk.nrk * c.fzn;
+        const uint16_t* fs = c.S > 1 ? c.fsid.p : nullptr;
+        if (c.prm.p == 1)
+            hipLaunchKernelGGL(flow_kernel<1>, dim3((n + 255) / 256), dim3(256), 0, c.stream, F, fs, c.fz.p);
+        else
+            hipLaunchKernelGGL(flow_kernel<2>, dim3((n + 255) / 256), dim3(256), 0, c.stream, F, fs, c.fz.p);
+        HIPCHK(hipGetLastError());
+    }
+    if (c.tip) {
+        const int n = c.rk.nrk * c.np;
+        hipLaunchKernelGGL(tip_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, tip_args(c), F);
+        HIPCHK(hipGetLastError());
+    }
     return 0;
 }
 
@@ -914,6 +1095,7 @@ StageArgs stage_args(Ctx& c, int irk, const int slot[], bool captured = false) {
     a.rout = c.keep_r[irk] ? c.r[irk].p : nullptr;
     a.irk = irk;
     a.wd_active = c.wd_active;
+    a.tip = c.tip ? c.tip2.p + (size_t)irk * c.np : nullptr;
     return a;
 }
 
@@ -947,7 +1129,7 @@ int launch_limiter(Ctx& c, double* u, const ElemData& ed, bool capturing) {
 // limiter is on) and the device step-counter increment.
 template <int P>
 int launch_step(Ctx& c, bool capturing) {
-    if (launch_tide(c, capturing)) return 1;
+    if (launch_forcing(c, capturing)) return 1;
     const int np = c.np;
     const int B = 256;
     const int gn = (np + B - 1) / B;
@@ -1004,7 +1186,7 @@ int launch_step_explicit(Ctx& c, double* const* st, double* const* wsave = nullp
     const bool sph = c.prm.ics == 2;
     const bool wdon = c.prm.nolifa >= 2, lim = c.prm.slopeflag != 0, fuse = wdon && !lim;
     const size_t ssz = (size_t)3 * c.dof * c.ne;
-    if (launch_tide(c, false)) return 1;
+    if (launch_forcing(c, false)) return 1;
     if (c.met) {
         hipLaunchKernelGGL(met_kernel, dim3((c.np + 255) / 256), dim3(256), 0, c.stream, met_args(c), nullptr, c.hstep);
         HIPCHK(hipGetLastError());
@@ -1027,6 +1209,7 @@ int launch_step_explicit(Ctx& c, double* const* st, double* const* wsave = nullp
         a.rout = c.keep_r[irk] ? c.r[irk].p : nullptr;
         a.irk = irk;
         a.wd_active = c.wd_active;
+        a.tip = c.tip ? c.tip2.p + (size_t)irk * c.np : nullptr;
         if (launch_stage_nt<P>(c, a, sph, fuse, c.block)) return 1;
         HIPCHK(hipGetLastError());
         if (fuse) c.wdcur = 1 - c.wdcur;
@@ -1079,6 +1262,8 @@ int run_chunk(Ctx& c, int n) {
 // ---- discrete adjoint driver ------------------------------------------------------------
 int adj_check(Ctx& c) {
     if (c.ng == 0) return fail("no gauges set (dg_adj_set_gauges)");
+    if (c.tip || c.eser || c.fser)
+        return fail("the adjoint does not cover the tidal potential or non-periodic boundary forcing");
     return 0;
 }
 
@@ -1215,7 +1400,7 @@ int adj_backward(Ctx& c, int it0, int nsteps, int interval, double weight) {
         }
         for (int k = k1; k > k0; k--) {
             c.hstep = k - k0 - 1;
-            if (launch_tide(c, false)) return 1;   // the step's tidal terms for the adjoint stages
+            if (launch_forcing(c, false)) return 1;   // the step's tidal terms for the adjoint stages
             if (c.met) {   // the step's node forcing for the adjoint stages
                 hipLaunchKernelGGL(met_kernel, dim3((c.np + 255) / 256), dim3(256), 0, c.stream, met_args(c), nullptr,
                                    c.hstep);
@@ -1270,7 +1455,7 @@ int validate(const dg_params* p) {
     // inputs whose terms the step does not have: refused, never run without them
     if (p->nws != 0 && p->nws != 8)
         return fail("NWS=" + std::to_string(p->nws) + ": only NWS=0 and NWS=8 (Holland vortex) are supported");
-    if (p->ntip != 0) return fail("NTIP=" + std::to_string(p->ntip) + ": tidal potential forcing is not supported");
+    if (p->ntip < 0 || p->ntip > 2) return fail("NTIP=" + std::to_string(p->ntip) + ": only NTIP 0, 1 and 2 are supported");
     if (p->artdif != 0) return fail("artdif=" + std::to_string(p->artdif) + ": LDG artificial diffusion is not supported");
     if (p->padapt != 0) return fail("padapt=" + std::to_string(p->padapt) + ": p-adaptivity is not supported");
     if (p->pl != p->p)
@@ -1329,7 +1514,7 @@ int dg_gpu_init(const dg_params* prm, int np, int ne, const double* x, const dou
                 const int* nedno, const int* nedel, const int* nedsd, int nieds, const int* niedn,
                 int nleds, const int* nledn, int needs, const int* needn, int nfeds, const int* nfedn,
                 int* handle) {
-    (void)sfac; (void)nedno;
+    (void)sfac;
     if (validate(prm)) return 1;
     if (np <= 0 || ne <= 0) return fail("empty mesh");
     if ((long long)ne * 4 >= (1LL << 31)) return fail("mesh too large for 32-bit indices");
@@ -1340,6 +1525,19 @@ int dg_gpu_init(const dg_params* prm, int np, int ne, const double* x, const dou
     if (build_member(c->mem[0], prm, c->T, np, ne, x, y, dp, corif, manning, nm, nedges, nedel, nedsd,
                      nieds, niedn, nleds, nledn, needs, needn, nfeds, nfedn))
         return 1;
+    {
+        // end nodes of the ocean edges in NEDNO order (non-periodic elevation boundaries)
+        Member& m = c->mem[0];
+        m.onode.resize((size_t)2 * needs);
+        for (int L = 0; L < needs; L++) {
+            const int ged = needn[L] - 1;
+            for (int k = 0; k < 2; k++) {
+                const int nd = nedno[2 * ged + k] - 1;
+                if (nd < 0 || nd >= np) return fail("ocean edge with a node out of range");
+                m.onode[2 * L + k] = nd;
+            }
+        }
+    }
     if (upload_mesh(*c) || upload_initial_state(*c)) return 1;
     return register_ctx(std::move(c), handle);
 }
@@ -1351,7 +1549,7 @@ int dg_gpu_set_tide(int h, int nbfr, const double* amig, const double* ff, const
     if (c->S != 1) return fail("the forcing of an ensemble is set on its members before dg_ens_create");
     Member& m = c->mem[0];
     if (nbfr < 0) return fail("nbfr < 0");
-    if (m.needs > 0 && nbfr == 0) return fail("non-periodic elevation boundary forcing is not supported");
+    if (m.needs > 0 && nbfr == 0) return fail("NBFR=0: a non-periodic elevation boundary is set with dg_gpu_set_elev_series");
     m.nbfr = nbfr;
     m.amig.assign(amig, amig + nbfr); m.ff.assign(ff, ff + nbfr);
     m.face.assign(face, face + nbfr); m.per.assign(per, per + nbfr);
@@ -1367,12 +1565,118 @@ int dg_gpu_set_flow(int h, int nffr, const double* famig, const double* fff, con
     if (c->S != 1) return fail("the forcing of an ensemble is set on its members before dg_ens_create");
     Member& m = c->mem[0];
     if (nffr < 0) return fail("nffr < 0");
-    if (m.nfeds > 0 && nffr == 0) return fail("time-series flux boundary forcing is not supported");
+    if (m.nfeds > 0 && nffr == 0) return fail("NFFR=0: a non-periodic normal flux is set with dg_gpu_set_flow_series");
     m.nffr = nffr;
     m.famig.assign(famig, famig + nffr); m.fff.assign(fff, fff + nffr);
     m.fface.assign(fface, fface + nffr); m.fper.assign(fper, fper + nffr);
     unpack_cols(nffr, m.nfeds, qnam_dg, m.qnam);
     unpack_cols(nffr, m.nfeds, qnph_dg, m.qnph);
+    return upload_forcing(*c);
+}
+
+int dg_gpu_set_tidal_potential(int h, int ntif, const double* tpk, const double* amigt, const double* etrf,
+                               const double* fft, const double* facet, const double* saltamp,
+                               const double* saltpha, const double* slam, const double* sfea, double reftim) {
+    Ctx* c = get(h);
+    if (!c) return 1;
+    if (c->S != 1) return fail("the forcing of an ensemble is set on its members before dg_ens_create");
+    Member& m = c->mem[0];
+    if (m.prm.ntip == 0) return fail("NTIP=0: the run was set up without tidal potential");
+    if (ntif <= 0) return fail("NTIF must be positive");
+    if (m.prm.ntip == 2 && (!saltamp || !saltpha)) return fail("NTIP=2 needs SALTAMP and SALTPHA");
+    const double pi = 3.141592653589793;     // PI of src/global.F:72
+    m.ntif = ntif;
+    m.reftim = reftim;
+    m.tpk.assign(tpk, tpk + ntif); m.amigt.assign(amigt, amigt + ntif); m.etrf.assign(etrf, etrf + ntif);
+    m.fft.assign(fft, fft + ntif); m.facet.assign(facet, facet + ntif);
+    m.pert.resize(ntif); m.tna.resize(ntif);
+    for (int j = 0; j < ntif; j++) {
+        m.pert[j] = amigt[j] == 0.0 ? 0.0 : 2.0 * pi / amigt[j];               // src/read_input.F:1870-1874
+        m.tna[j] = amigt[j] == 0.0 ? 0 : (int)std::lround(0.00014 / amigt[j]); // NINT(0.00014/AMIGT)
+    }
+    const int np = m.np;
+    m.tslam.assign(slam, slam + np);
+    m.tccs.resize(np); m.ts2s.resize(np);
+    for (int i = 0; i < np; i++) {
+        const double cc = std::cos(sfea[i]);
+        m.tccs[i] = cc * cc;
+        m.ts2s[i] = std::sin(2.0 * sfea[i]);
+    }
+    m.tsamp.clear(); m.tspha.clear();
+    if (m.prm.ntip == 2) {   // SALTAMP(MNTIF,MNP), column-major -> [ntif][np]
+        m.tsamp.resize((size_t)ntif * np); m.tspha.resize((size_t)ntif * np);
+        for (int i = 0; i < np; i++)
+            for (int j = 0; j < ntif; j++) {
+                m.tsamp[(size_t)j * np + i] = saltamp[(size_t)j + (size_t)ntif * i];
+                m.tspha[(size_t)j * np + i] = saltpha[(size_t)j + (size_t)ntif * i];
+            }
+    }
+    c->drop_graphs();
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->ntif = ntif;
+    HIPCHK(c->tna.upload(m.tna.data(), ntif, c->stream));
+    HIPCHK(c->tslam.upload(m.tslam.data(), np, c->stream));
+    HIPCHK(c->tccs.upload(m.tccs.data(), np, c->stream));
+    HIPCHK(c->ts2s.upload(m.ts2s.data(), np, c->stream));
+    if (m.prm.ntip == 2) {
+        HIPCHK(c->tsamp.upload(m.tsamp.data(), m.tsamp.size(), c->stream));
+        HIPCHK(c->tspha.upload(m.tspha.data(), m.tspha.size(), c->stream));
+    }
+    HIPCHK(c->tiprow.alloc((size_t)c->rows * ntif * 3));
+    HIPCHK(c->tip2.alloc((size_t)c->rk.nrk * np));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->tip = true;
+    return 0;
+}
+
+int dg_gpu_set_elev_series(int h, int neta, const int* nbd, double etiminc, int nrec, const double* esbin) {
+    Ctx* c = get(h);
+    if (!c) return 1;
+    if (c->S != 1) return fail("the forcing of an ensemble is set on its members before dg_ens_create");
+    Member& m = c->mem[0];
+    if (m.needs == 0) return fail("the mesh has no elevation-specified (ocean) edges");
+    if (neta <= 0 || nrec < 2) return fail("fort.19 needs NETA > 0 and at least two records");
+    if (!(etiminc > 0.0)) return fail("ETIMINC must be positive");
+    for (int i = 0; i < neta; i++)
+        if (nbd[i] < 1 || nbd[i] > m.np) return fail("NBD refers to a node out of range");
+    m.neta = neta;
+    m.nbd.resize(neta);
+    for (int i = 0; i < neta; i++) m.nbd[i] = nbd[i] - 1;
+    m.etiminc = etiminc;
+    m.e_nrec = nrec;
+    m.esbin.assign(esbin, esbin + (size_t)nrec * neta);
+    m.eser = true;
+    m.es = Member::Series{};
+    // one pseudo-constituent whose end-node values come from the series (tide_kernel copies them)
+    m.nbfr = 1;
+    m.amig.assign(1, 0.0); m.ff.assign(1, 0.0); m.face.assign(1, 0.0); m.per.assign(1, 0.0);
+    m.emo.assign((size_t)m.needs * 2, 0.0); m.efa.assign((size_t)m.needs * 2, 0.0);
+    return upload_forcing(*c);
+}
+
+int dg_gpu_set_flow_series(int h, int nvel, const int* lbcodei, double ftiminc, int nrec, const double* qnin) {
+    Ctx* c = get(h);
+    if (!c) return 1;
+    if (c->S != 1) return fail("the forcing of an ensemble is set on its members before dg_ens_create");
+    Member& m = c->mem[0];
+    if (m.nfeds == 0) return fail("the mesh has no flux-specified edges");
+    if (nvel <= 0 || nrec < 2) return fail("fort.20 needs NVEL > 0 and at least two records");
+    if (!(ftiminc > 0.0)) return fail("FTIMINC must be positive");
+    int nflux = 0;
+    for (int j = 0; j < nvel; j++) if (lbcodei[j] == 2 || lbcodei[j] == 12 || lbcodei[j] == 22) nflux++;
+    if (nflux == 0) return fail("LBCODEI has no flux-specified node");
+    m.nvel = nvel;
+    m.lbcodei.assign(lbcodei, lbcodei + nvel);
+    m.nflux = nflux;
+    m.ftiminc = ftiminc;
+    m.f_nrec = nrec;
+    m.qnin.assign(qnin, qnin + (size_t)nrec * nflux);
+    m.fser = true;
+    m.fs = Member::Series{};
+    // one term with ARGJ = 0 and RFF = RAMPDG, its end values from the series each stage
+    m.nffr = 1;
+    m.famig.assign(1, 0.0); m.fff.assign(1, 1.0); m.fface.assign(1, 0.0); m.fper.assign(1, 0.0);
+    m.qnam.assign((size_t)m.nfeds * 2, 0.0); m.qnph.assign((size_t)m.nfeds * 2, 0.0);
     return upload_forcing(*c);
 }
 
@@ -1458,6 +1762,7 @@ int dg_gpu_step(int h, int it, int nsteps) {
         if (m.needs > 0 && m.nbfr == 0) return fail("tidal forcing not set (dg_gpu_set_tide)");
         if (m.nfeds > 0 && m.nffr == 0) return fail("flux forcing not set (dg_gpu_set_flow)");
         if (m.prm.nws == 8 && !m.met) return fail("NWS=8 forcing not set (dg_gpu_set_holland)");
+        if (m.prm.ntip != 0 && m.ntif == 0) return fail("tidal potential not set (dg_gpu_set_tidal_potential)");
     }
     if (ensure_staging(*c, stage_nt(*c, c->block))) return 1;
     for (int s0 = 0; s0 < nsteps; s0 += kTableChunk) {
@@ -1804,6 +2109,8 @@ int dg_ens_create(int nmembers, const int* handles, int* ens_handle) {
     }
     for (int i = 0; i < nmembers; i++)
         for (const Member& m : src[i]->mem) {
+            if (m.ntif > 0 || m.eser || m.fser)
+                return fail("ensemble members with tidal potential or non-periodic boundary forcing are not supported");
             if (m.needs > 0 && m.nbfr == 0) return fail("member without tidal forcing (dg_gpu_set_tide)");
             if (m.nfeds > 0 && m.nffr == 0) return fail("member without flux forcing (dg_gpu_set_flow)");
         }
@@ -1864,7 +2171,7 @@ int dg_gpu_autotune(int h, int* block) {
     const bool sph = c->prm.ics == 2, fuse = c->prm.nolifa >= 2 && c->prm.slopeflag == 0;
     const std::vector<int> cand = c->prm.p == 1 ? std::vector<int>{64, 128, 256} : std::vector<int>{64, 128};
     if (build_stage_tables(*c, c->next_it, 1)) return 1;
-    if (launch_tide(*c, false)) return 1;
+    if (launch_forcing(*c, false)) return 1;
     if (c->met) {
         hipLaunchKernelGGL(met_kernel, dim3((c->np + 255) / 256), dim3(256), 0, c->stream, met_args(*c),
                            nullptr, 0);

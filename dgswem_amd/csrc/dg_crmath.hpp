@@ -97,11 +97,13 @@ constexpr double INV_FACT[28][2] = {
     {0x1.d1ab1c2dccea3p-94, 0x1.054d0c78aea14p-149},
 };
 
-// pi/2 = P1 + P2 + P3 + O(1e-37): P1 and P2 carry 33 significant bits, so k*P1 and k*P2 are exact
-// for |k| < 2^20; 2/pi rounded to a double picks the quadrant.
+// pi/2 = P1 + P2 + P3 + P4 + O(2^-176): P1 and P2 carry 33 significant bits, so k*P1 and k*P2 are
+// exact for |k| < 2^20; k*P3 is formed exactly as a two-product and k*P4 (about k*2^-123) needs no
+// more than its rounding.  2/pi rounded to a double picks the quadrant.
 constexpr double PIO2_1 = 0x1.921fb54400000p+0;
 constexpr double PIO2_2 = 0x1.0b4611a600000p-34;
 constexpr double PIO2_3 = 0x1.3198a2e037073p-69;
+constexpr double PIO2_4 = 0x1.129024e088a68p-123;
 constexpr double TWO_OVER_PI = 0x1.45f306dc9c883p-1;
 
 // sum_{j=0..13} (-1)^j z^j / (2j+s)!  (s = 0: the cosine series, s = 1: the sine series over r),
@@ -117,15 +119,20 @@ DG_CRM_FN dd even_series(dd z, int s) {
     return p;
 }
 
-// Correctly rounded cos(x) for |x| < 1e6 (tidal arguments are below 10); other arguments, NaN and
-// infinities go to the library's cos.
+// Correctly rounded cos(x) for |x| < 2^16 (the step's arguments -- tidal and flux boundary phases,
+// tidal potential arguments -- are below 30); other arguments, NaN and infinities go to the
+// library's cos.  The reduced argument r = x - k*pi/2 is a double-double whose absolute error is
+// below k*2^-170 (x - k*P1 and k*P2 exact, k*P3 an exact two-product, k*P4 rounded once), so even
+// for the doubles closest to a multiple of pi/2 (|r| above 2^-62 for |x| < 2^16) its relative
+// error stays below 2^-90, inside the series' accuracy.
 DG_CRM_FN double cr_cos(double x) {
-    if (!(std::fabs(x) < 1.0e6)) return std::cos(x);
+    if (!(std::fabs(x) < 65536.0)) return std::cos(x);
     const double k = std::rint(x * TWO_OVER_PI);
-    // r = x - k*pi/2 as a double-double: x - k*P1 is exact (Sterbenz), k*P1 and k*P2 are exact
-    const double a = x - k * PIO2_1;
-    const dd s = two_sum(a, -(k * PIO2_2));
-    const dd r = fast_two_sum(s.hi, s.lo - k * PIO2_3);
+    const double a = x - k * PIO2_1;                 // exact (Sterbenz)
+    const dd s = two_sum(a, -(k * PIO2_2));          // k*P2 exact
+    const dd p3 = two_prod(k, PIO2_3);
+    const dd u = dd_add(s, {-p3.hi, -p3.lo});
+    const dd r = fast_two_sum(u.hi, u.lo - k * PIO2_4);
     const dd z = dd_mul(r, r);
     const int q = (int)((long long)k & 3);
     // even quadrants: cos r; odd: sin r = r * (the sine series)

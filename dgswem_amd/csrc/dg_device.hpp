@@ -100,6 +100,30 @@ struct Forcing {
     // (tzn = needs*nbfr*2 per stage), written by tide_kernel before the step's stages
     const double* __restrict__ tz;
     int tzn;
+    // specified normal flux Q_N_EXT at every flux-edge quadrature point of the current step,
+    // [stage][nfeds][negp] (fzn = nfeds*negp per stage), written by flow_kernel
+    const double* __restrict__ fz;
+    int fzn;
+    // non-periodic boundaries (NBFR = 0, NFFR = 0): the ramped end-node values of every stage row,
+    // [scenario*rows + row][needs][2] (elevation; nbfr is 1) and [scenario*rows + row][nfeds][2]
+    // (normal flux; nffr is 1), or nullptr
+    const double* __restrict__ ets;
+    const double* __restrict__ qts;
+    int nfeds;
+};
+
+// Tidal potential (NTIP != 0): per-stage-row constituent terms and node data for tip_kernel, which
+// writes TIP2 of every stage at the nodes once per step.
+struct Tip {
+    int np, ntif;
+    const double* __restrict__ row;      // [scenario*rows + row][ntif][3]: ARGT, TPMUL, SALTMUL
+    const int* __restrict__ na;          // [ntif]: 1 semidiurnal, 2 diurnal, else no term
+    const double* __restrict__ slam;     // [np] longitude (rad)
+    const double* __restrict__ ccsfea;   // [np] COS(SFEA)**2
+    const double* __restrict__ s2sfea;   // [np] SIN(2*SFEA)
+    const double* __restrict__ salt_amp; // [ntif][np], or nullptr (NTIP = 1: zero)
+    const double* __restrict__ salt_pha; // [ntif][np] radians
+    double* __restrict__ tip2;           // [stage][np]
 };
 
 // Step index within the current forcing tables.

@@ -58,6 +58,7 @@ struct StageArgs {
     const int* __restrict__ snbl;
     int ns;                              // staged elements per block (LDS row length)
     int* __restrict__ wd_active;         // set to 1 when WETDRY changes an element, or nullptr
+    const double* __restrict__ tip;      // TIP2 of this stage at the nodes (tidal potential), or nullptr
 };
 
 // ------------------------------------------------------------------------------------------
@@ -456,22 +457,12 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
                     if ((ze_ex * ph.ifnlfa + hb_in) <= 0) ze_ex = fabs(hb_in) + sc.h0;
                     llf<NL>(ph, ze_in, ze_ex, qx_in, qx_in, qy_in, qy_in, hb_in, hb_in, sf, sf, nx, ny, ph.g, f0, f1, f2);
                 } else {
-                    double qn_ext = 0.0, qx_ex = 0.0, qy_ex = 0.0;
-                    const double xe = T.xegp[I];
-                    for (int jj = 0; jj < a.fo.nffr; jj++) {
-                        const size_t t = ((size_t)sid * a.fo.rows + row) * a.fo.nffr + jj;
-                        const double argj = a.fo.fargj[t];
-                        const double rff = a.fo.frff[t];
-                        const size_t o = ((size_t)nb * a.fo.nffr + jj) * 2;
-                        const double q1 = a.fo.qnam[o], q2 = a.fo.qnam[o + 1];
-                        const double p1 = a.fo.qnph[o], p2 = a.fo.qnph[o + 1];
-                        const double qnam_gp = 0.5 * (q1 + q2) + 0.5 * (q2 - q1) * xe;
-                        const double qnph_gp = 0.5 * (p1 + p2) + 0.5 * (p2 - p1) * xe;
-                        qn_ext = qn_ext + qnam_gp * rff * cos(argj - qnph_gp);
-                        const double qt_ext = 0.0;
-                        qx_ex = -(ty * qn_ext - ny * qt_ext) / (nx * ty - ny * tx);
-                        qy_ex = -(-tx * qn_ext + nx * qt_ext) / (nx * ty - ny * tx);
-                    }
+                    // the specified normal flux at this point (flow_kernel); the reference forms the
+                    // exterior momentum after every constituent, so the last, full sum is what counts
+                    const double qn_ext = a.fo.fz[(size_t)a.irk * a.fo.fzn + (size_t)nb * NEGP + I];
+                    const double qt_ext = 0.0;
+                    const double qx_ex = -(ty * qn_ext - ny * qt_ext) / (nx * ty - ny * tx);
+                    const double qy_ex = -(-tx * qn_ext + nx * qt_ext) / (nx * ty - ny * tx);
                     llf<NL>(ph, ze_in, ze_in, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_in, sf, sf, nx, ny, ph.g, f0, f1, f2);
                 }
 #pragma unroll
@@ -509,6 +500,11 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
             const int n = E.nm[(size_t)v * ne + e];
             mcd[v] = a.mt.cd[n]; mwm[v] = a.mt.wm[n]; mwx[v] = a.mt.wx[n]; mwy[v] = a.mt.wy[n]; mpr[v] = a.mt.prn[n];
         }
+    }
+    double tipv[3] = {0.0, 0.0, 0.0};
+    if (a.tip) {
+#pragma unroll
+        for (int v = 0; v < 3; v++) tipv[v] = a.tip[E.nm[(size_t)v * ne + e]];
     }
     double sfqv[NAGP];
 #pragma unroll
@@ -561,6 +557,20 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
             }
             mw[6] = pdx;
             mw[7] = pdy;
+        }
+        // tidal potential (src/rhs_dg_hydro.F:117-122,378-386): the sums DPSIDX(i)*TIP2(Ni) are
+        // the same at every area point; RAMPDG multiplies them once more at each point
+        double tdx = 0.0, tdy = 0.0, tramp = 0.0;
+        if (a.tip) {
+            const double drpsi[3] = {-1.0 / 2.0, 1.0 / 2.0, 0.0}, dspsi[3] = {-1.0 / 2.0, 0.0, 1.0 / 2.0};
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                const double dpsidx = drpsi[v] * drdx + dspsi[v] * dsdx;
+                const double dpsidy = drpsi[v] * drdy + dspsi[v] * dsdy;
+                if (v == 0) { tdx = dpsidx * tipv[v]; tdy = dpsidy * tipv[v]; }
+                else { tdx = tdx + dpsidx * tipv[v]; tdy = tdy + dpsidy * tipv[v]; }
+            }
+            tramp = a.fo.ramp[(size_t)sid * a.fo.rows + row];
         }
         // hybrid friction factor (1 + (HBREAK/H_c)^FTHETA)^(FGAMMA/FTHETA) from the element-mean
         // depth (src/rhs_dg_hydro.F:306-309): the same at every area point, evaluated once and only
@@ -632,6 +642,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
                 const double fw = 1.0 / f1;
                 sx = sx + (NL ? wsx : fw * wsx) - ph.g * sfq * depth * mw[6];
                 sy = sy + (NL ? wsy : fw * wsy) - ph.g * depth * mw[7];
+            }
+            if (a.tip) {
+                sx = sx + tramp * ph.g * depth * sfq * tdx;
+                sy = sy + tramp * ph.g * depth * tdy;
             }
             R[1][0] = R[1][0] + T.srfac[I][0] * sx;
             R[2][0] = R[2][0] + T.srfac[I][0] * sy;
@@ -714,8 +728,83 @@ __global__ void __launch_bounds__(256) tide_kernel(Forcing fo, const uint16_t* _
     const int jj = (o >> 1) % fo.nbfr, edge = (o >> 1) / fo.nbfr;
     const int sid = osid ? (int)osid[edge] : 0;
     const int row = table_step(fo) * fo.nrk + irk;
+    if (fo.ets) {   // non-periodic: the stage's ramped end-node values (nbfr is 1)
+        tz[i] = fo.ets[((size_t)sid * fo.rows + row) * fo.tzn + o];
+        return;
+    }
     const size_t t = ((size_t)sid * fo.rows + row) * fo.nbfr + jj;
     tz[i] = fo.emo[o] * fo.rff[t] * crm::cr_cos(fo.argj[t] - fo.efa[o]);
+}
+
+// ------------------------------------------------------------------------------------------
+// Specified normal flux of one step (src/flow_edge_hydro.F:143-173): for every stage, flux edge
+// and edge quadrature point Q_N_EXT = sum over constituents of QNAM_GP*RFF*COS(ARGJ - QNPH_GP),
+// with the cosine correctly rounded.  For NFFR = 0 the one term has ARGJ = 0, RFF = RAMPDG and the
+// end values of the time series (already ramped once, src/DG_hydro_timestep.F:188-195).
+// ------------------------------------------------------------------------------------------
+template <int P>
+__global__ void __launch_bounds__(256) flow_kernel(Forcing fo, const uint16_t* __restrict__ fsid,
+                                                   double* __restrict__ fz) {
+    constexpr int NEGP = P + 2;
+    const DevTables& T = tab<P>();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fo.nrk * fo.fzn) return;
+    const int irk = i / fo.fzn, o = i - irk * fo.fzn;       // o = edge*NEGP + point
+    const int edge = o / NEGP, I = o - edge * NEGP;
+    const int sid = fsid ? (int)fsid[edge] : 0;
+    const int row = table_step(fo) * fo.nrk + irk;
+    const double xe = T.xegp[I];
+    double qn = 0.0;
+    for (int jj = 0; jj < fo.nffr; jj++) {
+        const size_t t = ((size_t)sid * fo.rows + row) * fo.nffr + jj;
+        const double argj = fo.fargj[t];
+        const double rff = fo.frff[t];
+        double q1, q2;
+        if (fo.qts) {
+            const size_t q = (((size_t)sid * fo.rows + row) * fo.nfeds + edge) * 2;
+            q1 = fo.qts[q]; q2 = fo.qts[q + 1];
+        } else {
+            const size_t q = ((size_t)edge * fo.nffr + jj) * 2;
+            q1 = fo.qnam[q]; q2 = fo.qnam[q + 1];
+        }
+        const size_t q = ((size_t)edge * fo.nffr + jj) * 2;
+        const double p1 = fo.qnph[q], p2 = fo.qnph[q + 1];
+        const double qnam_gp = 0.5 * (q1 + q2) + 0.5 * (q2 - q1) * xe;
+        const double qnph_gp = 0.5 * (p1 + p2) + 0.5 * (p2 - p1) * xe;
+        qn = qn + qnam_gp * rff * crm::cr_cos(argj - qnph_gp);
+    }
+    fz[i] = qn;
+}
+
+// ------------------------------------------------------------------------------------------
+// TIDAL_POTENTIAL (src/tidal_potential.F:15-80) for every stage of one step: TIP2 at every node,
+// with the cosines of the time-dependent arguments correctly rounded (COS(SFEA)**2 and
+// SIN(2*SFEA) come from the host's libm).
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) tip_kernel(Tip tp, Forcing fo) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fo.nrk * tp.np) return;
+    const int irk = i / tp.np, n = i - irk * tp.np;
+    const int row = table_step(fo) * fo.nrk + irk;
+    const double* r = tp.row + (size_t)row * tp.ntif * 3;
+    double tip = 0.0;
+    for (int j = 0; j < tp.ntif; j++) {
+        const double argt = r[3 * j], tpmul = r[3 * j + 1], saltmul = r[3 * j + 2];
+        const double sa = tp.salt_amp ? tp.salt_amp[(size_t)j * tp.np + n] : 0.0;
+        const double sp = tp.salt_pha ? tp.salt_pha[(size_t)j * tp.np + n] : 0.0;
+        const int na = tp.na[j];
+        if (na == 1) {
+            const double argtp = argt + 2.0 * tp.slam[n];
+            const double argsalt = argt - sp;
+            tip = tip + tpmul * tp.ccsfea[n] * crm::cr_cos(argtp) + saltmul * sa * crm::cr_cos(argsalt);
+        }
+        if (na == 2) {
+            const double argtp = argt + tp.slam[n];
+            const double argsalt = argt - sp;
+            tip = tip + tpmul * tp.s2sfea[n] * crm::cr_cos(argtp) + saltmul * sa * crm::cr_cos(argsalt);
+        }
+    }
+    tp.tip2[(size_t)irk * tp.np + n] = tip;
 }
 
 // ------------------------------------------------------------------------------------------

@@ -14,6 +14,7 @@
 //                      1964-2061 (HollandGet's storm-wide scalars)
 #include "dg_setup.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <unordered_map>
@@ -433,6 +434,86 @@ bool holland_row(const HollandTrack& t, double time, double g, double* row, std:
     row[MR_BLADJ] = t.bladj;
     row[MR_RHOG] = 1000.0 * g;
     return true;
+}
+
+}  // namespace dg
+
+namespace dg {
+
+void eletab_order(int np, int ne, const int* nm, const double* x, const double* y,
+                  std::vector<int>& off, std::vector<int>& row) {
+    // incident elements in element order (NNOEL's content; its order does not matter below, an
+    // element being fixed by its three nodes)
+    off.assign(np + 1, 0);
+    for (int e = 0; e < ne; e++) for (int v = 0; v < 3; v++) off[nm[(size_t)v * ne + e] + 1]++;
+    for (int i = 0; i < np; i++) off[i + 1] += off[i];
+    row.assign((size_t)3 * ne, 0);
+    {
+        std::vector<int> fill(np, 0);
+        for (int e = 0; e < ne; e++)
+            for (int v = 0; v < 3; v++) {
+                const int n = nm[(size_t)v * ne + e];
+                row[off[n] + fill[n]++] = (e << 2) | v;
+            }
+    }
+    // NEIGHB: neighbours in insertion order (pairs 1-2, 1-3, 2-3 of every element, each once)
+    std::vector<std::vector<int>> nb(np);
+    for (int e = 0; e < ne; e++) {
+        const int en[3] = {nm[e], nm[(size_t)ne + e], nm[2 * (size_t)ne + e]};
+        const int pr[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+        for (auto& q : pr) {
+            const int a = en[q[0]], b = en[q[1]];
+            bool found = false;
+            for (int m : nb[a]) if (m == b) { found = true; break; }
+            if (found) continue;
+            nb[a].push_back(b);
+            nb[b].push_back(a);
+        }
+    }
+    const double rad2deg = 45.0 / std::atan(1.0);
+    std::vector<double> ang;
+    std::vector<int> sorted, tmp;
+    for (int i = 0; i < np; i++) {
+        const int nn = (int)nb[i].size();
+        ang.assign(nn, 0.0);
+        for (int j = 0; j < nn; j++) {
+            const int m = nb[i][j];
+            const double delx = x[m] - x[i], dely = y[m] - y[i];
+            const double dist = std::sqrt(delx * delx + dely * dely);
+            if (dely != 0.0) {
+                ang[j] = rad2deg * std::acos(delx / dist);
+                if (dely > 0.0) ang[j] = 360.0 - ang[j];
+            } else {
+                if (delx > 0.0) ang[j] = 0.0;
+                if (delx < 0.0) ang[j] = 180.0;
+            }
+        }
+        // selection by increasing angle, as NEIGHB does it
+        sorted.assign(nn + 1, i);
+        double more = -1.0;
+        int jlow = 0;
+        for (int jj = 0; jj < nn; jj++) {
+            double low = 400.0;
+            for (int j = 0; j < nn; j++)
+                if (ang[j] < low && ang[j] > more) { low = ang[j]; jlow = j; }
+            sorted[1 + jj] = nb[i][jlow];
+            more = low;
+        }
+        // ELETAB: the element of each consecutive (cyclic) neighbour pair
+        const int cnt = off[i + 1] - off[i], nneigh = nn + 1;
+        tmp.clear();
+        for (int k = 1; k <= nneigh - 1; k++) {
+            const int j2 = sorted[k], j3 = k < nneigh - 1 ? sorted[k + 1] : sorted[1];
+            for (int j = off[i]; j < off[i + 1]; j++) {
+                const int e = row[j] >> 2;
+                const int v0 = nm[e], v1 = nm[(size_t)ne + e], v2 = nm[2 * (size_t)ne + e];
+                const bool h2 = v0 == j2 || v1 == j2 || v2 == j2;
+                const bool h3 = v0 == j3 || v1 == j3 || v2 == j3;
+                if (h2 && h3) { tmp.push_back(row[j]); break; }
+            }
+        }
+        if ((int)tmp.size() == cnt) std::copy(tmp.begin(), tmp.end(), row.begin() + off[i]);
+    }
 }
 
 }  // namespace dg

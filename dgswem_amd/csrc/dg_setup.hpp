@@ -51,6 +51,14 @@ bool create_edge_data(int np, int ne, const int* nm_colmajor_1based, int nope, c
                       const int* nbdv, int nbou, const int* ibtype, const int* nvell,
                       const int* nbvv, EdgeData& ed, std::string& err);
 
+// Node-to-element table in PREP_DG's ELETAB order (src/prep_DG.F:292-330): around each node the
+// elements between consecutive neighbours, the neighbours sorted by NEIGHB by clockwise angle from
+// east (src/dgswem.F:478-570).  row[off[i]..off[i+1]) holds element*4 + local vertex; a node whose
+// neighbour fan does not yield all its elements keeps them in ascending element order.
+// nm: [3][ne] 0-based (vertex-major), x/y: the projected node coordinates.
+void eletab_order(int np, int ne, const int* nm, const double* x, const double* y,
+                  std::vector<int>& off, std::vector<int>& row);
+
 // NWS=8 storm track after the first-call processing of GetHollandStormData
 // (src/wind.F:2185-2410): repeated lines dropped, forecasts given the nowcast's pressure and radii,
 // SI units, cast times relative to the wind reference time, translation velocities.

@@ -89,6 +89,17 @@ class Params:
     fper: np.ndarray = field(default_factory=lambda: np.zeros(0))
     qnam: np.ndarray = field(default_factory=lambda: np.zeros((0, 0)))  # [nffr, n flux nodes]
     qnph: np.ndarray = field(default_factory=lambda: np.zeros((0, 0)))
+    # tidal potential constituents (NTIP = 1, 2; src/read_input.F:1856-1923)
+    ntif: int = 0
+    tpk: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    amigt: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    etrf: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    fft: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    facet: np.ndarray = field(default_factory=lambda: np.zeros(0))   # radians
+    pert: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    tipotag: List[str] = field(default_factory=list)
+    saltamp: Optional[np.ndarray] = None     # [ntif, np] (NTIP = 2, fort.24)
+    saltpha: Optional[np.ndarray] = None     # [ntif, np] radians
     # global elevation output (fort.63)
     noutge: int = 0
     toutsge: float = 0.0
@@ -133,6 +144,20 @@ class Case:
     manning: Optional[np.ndarray] = None         # ManningsN per node when loaded from fort.13
     storm: Optional["HollandStorm"] = None        # NWS=8 track and control line
     lonlat: Optional[np.ndarray] = None          # [NP,2] fort.14 lon/lat in degrees (ICS=2)
+    # non-periodic boundary forcing: fort.19 (NBFR = 0 with open boundaries) as
+    # (ETIMINC, [nrec, NETA]) and fort.20 (NFFR = 0 with flux boundaries) as (FTIMINC, [nrec, flux nodes])
+    elev_series: Optional[Tuple[float, np.ndarray]] = None
+    flow_series: Optional[Tuple[float, np.ndarray]] = None
+
+    def lbcodei(self) -> np.ndarray:
+        """LBCODEI(NVEL): the boundary type of every land-boundary node, in NVEL order."""
+        return np.concatenate([np.full(len(n), t, np.int32) for t, n in self.land_segs]) \
+            if self.land_segs else np.zeros(0, np.int32)
+
+    def nbd(self) -> np.ndarray:
+        """NBD(NETA): the open-boundary nodes in order (1-based)."""
+        return np.concatenate(self.open_segs).astype(np.int32) if self.open_segs \
+            else np.zeros(0, np.int32)
 
     @property
     def ne(self) -> int:
@@ -403,12 +428,20 @@ def read_case(dirname: str) -> Case:
     else:
         corif = coriolis(sfea)
 
-    # ---- tidal potential constituents (read and skipped: NTIP=0 on the configured paths)
-    p_ntif = f15.ints(1)
-    for _ in range(p_ntif):
-        f15.text(); f15.reals(5)
-    if p.ntip != 0:
-        raise NotImplementedError("NTIP != 0")
+    # ---- tidal potential constituents (src/read_input.F:1856-1923)
+    p.ntif = f15.ints(1)
+    cons = []
+    for _ in range(p.ntif):
+        p.tipotag.append(f15.text()[:5].strip())
+        cons.append(f15.reals(5))
+    cons = np.array(cons, dtype=np.float64).reshape(p.ntif, 5)
+    p.tpk, p.amigt, p.etrf, p.fft = cons[:, 0].copy(), cons[:, 1].copy(), cons[:, 2].copy(), cons[:, 3].copy()
+    p.facet = cons[:, 4] * DEG2RAD
+    p.pert = np.array([0.0 if a == 0.0 else 2.0 * PI / a for a in p.amigt])
+    if p.ntip not in (0, 1, 2):
+        raise NotImplementedError(f"NTIP={p.ntip}")
+    if p.ntip == 2:
+        p.saltamp, p.saltpha = read_fort24(os.path.join(dirname, "fort.24"), p.ntif, n_nodes)
 
     # ---- open-boundary constituents and segments (src/read_input.F:1979-2111)
     p.nbfr = f15.ints(1)
@@ -474,6 +507,14 @@ def read_case(dirname: str) -> Case:
                 qnam[i, j] = a; qnph[i, j] = b * DEG2RAD
         p.qnam, p.qnph = qnam, qnph
 
+    # ---- non-periodic boundary forcing files (src/cstart.F:80-124)
+    elev_series = flow_series = None
+    if open_segs and p.nbfr == 0:
+        elev_series = read_series(os.path.join(dirname, "fort.19"), sum(len(s) for s in open_segs))
+    nflux = sum(len(n) for t, n in land_segs if t in FLOW_TYPES)
+    if nflux and p.nffr == 0:
+        flow_series = read_series(os.path.join(dirname, "fort.20"), nflux)
+
     # ---- station output specs (skipped) and global elevation output
     for _ in range(2):                                # elevation, velocity stations
         f15.values(4)
@@ -495,7 +536,41 @@ def read_case(dirname: str) -> Case:
 
     return Case(title=title, params=p, dg=dg, x=x, y=y, slam=slam, sfea=sfea, dp=dp, sfac=sfac,
                 corif=corif, nm=nm, open_segs=open_segs, land_segs=land_segs, manning=manning,
-                barrier_back=barrier_back, storm=storm, lonlat=lonlat)
+                barrier_back=barrier_back, storm=storm, lonlat=lonlat, elev_series=elev_series,
+                flow_series=flow_series)
+
+
+def read_series(path: str, nvals: int) -> Tuple[float, np.ndarray]:
+    """fort.19 / fort.20: the time increment, then records of `nvals` values, one per line
+    (list-directed READs, src/cstart.F:89-94,109-121, src/DG_hydro_timestep.F:158,181)."""
+    r = _Records(path)
+    inc = r.reals(1)
+    vals = []
+    while True:
+        try:
+            vals.append(r.reals(1))
+        except (EOFError, IndexError, ValueError):
+            break
+    nrec = len(vals) // nvals
+    if nrec < 2:
+        raise ValueError(f"{path}: fewer than two records of {nvals} values")
+    return inc, np.array(vals[:nrec * nvals], dtype=np.float64).reshape(nrec, nvals)
+
+
+def read_fort24(path: str, ntif: int, n_nodes: int) -> Tuple[np.ndarray, np.ndarray]:
+    """fort.24 self-attraction/load tide (src/read_input.F:1880-1889): per constituent a
+    four-line header, then JJ, SALTAMP, SALTPHA (degrees) for every node."""
+    r = _Records(path)
+    amp = np.zeros((ntif, n_nodes)); pha = np.zeros((ntif, n_nodes))
+    for i in range(ntif):
+        for _ in range(4):
+            r.text()
+        for _ in range(n_nodes):
+            t = r.values(3)
+            jj = int(t[0])
+            amp[i, jj - 1] = _freal(t[1])
+            pha[i, jj - 1] = _freal(t[2]) * DEG2RAD
+    return amp, pha
 
 
 def spherical_coords(lonlat: np.ndarray, slam0: float, sfea0: float):

@@ -5,8 +5,10 @@
 ! modal2nodal with dg_gpu_nodal_update, then calls the reference's own WRITE_RESULTS, so the run
 ! still reads fort.14/fort.15/fort.dg(/fort.13/fort.22) and writes fort.6x, maxele.63 and the DG
 ! hot start.  The first call hands the module arrays built by READ_INPUT and PREP_DG to the
-! library; inputs whose terms the library does not have (NWS other than 0 and 8, NTIP, artdif,
-! p-adaptivity, a per-node quadratic friction attribute) make dg_gpu_init fail and the run STOP.
+! library, with the boundary time series of fort.19/fort.20 and the tidal potential constituents
+! when the run has them; inputs whose terms the library does not have (NWS other than 0 and 8,
+! artdif, p-adaptivity, a per-node quadratic friction attribute) make dg_gpu_init fail and the run
+! STOP.
 !
 ! The nodal values (ETA2, UU2, VV2, NODECODE), the running maximum ETAMAX and the modal state stay
 ! on the GPU; they are copied to the host only on the steps at which WRITE_RESULTS reads them: an
@@ -26,6 +28,8 @@ subroutine DG_TIMESTEP(IT)
        DGHOTSPOOL, NEDGES, NEDNO, NEDEL, NEDSD, NIEDS, NIEDN, NLEDS, NLEDN, NEEDS, NEEDN, NFEDS, &
        NFEDN, EMO_DG, EFA_DG, QNAM_DG, QNPH_DG, ZE, QX, QY, WDFLG
   use global, only : NFFR, FAMIG, FFF, FFACE, FPER
+  use global, only : NTIF, TPK, AMIGT, ETRF, FFT, FACET, SALTAMP, SALTPHA, REFTIM, NETA, NBD, &
+       ESBIN1, ESBIN2, ETIMINC, NVEL, LBCODEI, QNIN1, QNIN2, FTIMINC
   use NodalAttributes, only : NoLiBF, CF, HBreak, FTheta, FGamma, LoadManningsN, ManningsN, &
        LoadQuadraticFric
   use wind, only : StormNumber, BLAdj
@@ -78,13 +82,19 @@ subroutine DG_TIMESTEP(IT)
              NEDNO, NEDEL, NEDSD, NIEDS, NIEDN, NLEDS, NLEDN, NEEDS, NEEDN, NFEDS, NFEDN, handle), &
              'dg_gpu_init')
      end if
-     if (NEEDS > 0) then
+     if (NEEDS > 0 .and. NBFR > 0) then
         call dg_gpu_check(dg_gpu_set_tide(handle, NBFR, AMIG, FF, FACE, PER, &
              EMO_DG(1:NBFR, 1:NEEDS, 1:2), EFA_DG(1:NBFR, 1:NEEDS, 1:2)), 'dg_gpu_set_tide')
      end if
-     if (NFEDS > 0) then
+     if (NEEDS > 0 .and. NBFR == 0) call set_elevation_series()
+     if (NFEDS > 0 .and. NFFR > 0) then
         call dg_gpu_check(dg_gpu_set_flow(handle, NFFR, FAMIG, FFF, FFACE, FPER, &
              QNAM_DG(1:NFFR, 1:NFEDS, 1:2), QNPH_DG(1:NFFR, 1:NFEDS, 1:2)), 'dg_gpu_set_flow')
+     end if
+     if (NFEDS > 0 .and. NFFR == 0) call set_flow_series()
+     if (NTIP /= 0) then
+        call dg_gpu_check(dg_gpu_set_tidal_potential(handle, NTIF, TPK, AMIGT, ETRF, FFT, FACET, &
+             SALTAMP, SALTPHA, SLAM, SFEA, REFTIM), 'dg_gpu_set_tidal_potential')
      end if
      if (NWS == 8) call set_holland_track()
      ! modal2nodal visits each node's elements in PREP_DG's order (ELETAB holds them as reals)
@@ -136,6 +146,75 @@ contains
     integer, intent(in) :: nout, ntcys, ntcyf, nscou, nspool
     fires = (nout /= 0) .and. (IT > ntcys) .and. (IT <= ntcyf) .and. (nscou + 1 == nspool)
   end function fires
+
+  ! fort.19 for NBFR = 0: cold start read the first two records into ESBIN1/ESBIN2 (src/cstart.F:
+  ! 88-97) and left unit 19 open; the stages would read the rest one record at a time
+  ! (src/DG_hydro_timestep.F:158), so the whole remainder goes to the library now
+  subroutine set_elevation_series()
+    real(c_double), allocatable :: rec(:), tmp(:)
+    real(c_double) :: v
+    integer :: n, ios, nrec
+    allocate(rec(2 * NETA + 1024))
+    rec(1:NETA) = ESBIN1(1:NETA)
+    rec(NETA + 1:2 * NETA) = ESBIN2(1:NETA)
+    n = 2 * NETA
+    do
+       read(19, *, iostat=ios) v
+       if (ios /= 0) exit
+       if (n == size(rec)) then
+          allocate(tmp(2 * size(rec)))
+          tmp(1:n) = rec(1:n)
+          call move_alloc(tmp, rec)
+       end if
+       n = n + 1
+       rec(n) = v
+    end do
+    nrec = n / NETA
+    call dg_gpu_check(dg_gpu_set_elev_series(handle, NETA, NBD, ETIMINC, nrec, rec), &
+         'dg_gpu_set_elev_series')
+    deallocate(rec)
+  end subroutine set_elevation_series
+
+  ! fort.20 for NFFR = 0: QNIN1/QNIN2 hold the first two records over the NVEL nodes (zero where
+  ! the code is not a flux code, src/cstart.F:109-121); the rest is read from unit 20 the same way
+  subroutine set_flow_series()
+    real(c_double), allocatable :: rec(:), tmp(:)
+    real(c_double) :: v
+    integer :: n, ios, nrec, nflux, j
+    nflux = 0
+    do j = 1, NVEL
+       if (LBCODEI(j) == 2 .or. LBCODEI(j) == 12 .or. LBCODEI(j) == 22) nflux = nflux + 1
+    end do
+    allocate(rec(2 * nflux + 1024))
+    n = 0
+    do j = 1, NVEL
+       if (LBCODEI(j) == 2 .or. LBCODEI(j) == 12 .or. LBCODEI(j) == 22) then
+          n = n + 1
+          rec(n) = QNIN1(j)
+       end if
+    end do
+    do j = 1, NVEL
+       if (LBCODEI(j) == 2 .or. LBCODEI(j) == 12 .or. LBCODEI(j) == 22) then
+          n = n + 1
+          rec(n) = QNIN2(j)
+       end if
+    end do
+    do
+       read(20, *, iostat=ios) v
+       if (ios /= 0) exit
+       if (n == size(rec)) then
+          allocate(tmp(2 * size(rec)))
+          tmp(1:n) = rec(1:n)
+          call move_alloc(tmp, rec)
+       end if
+       n = n + 1
+       rec(n) = v
+    end do
+    nrec = n / nflux
+    call dg_gpu_check(dg_gpu_set_flow_series(handle, NVEL, LBCODEI, FTIMINC, nrec, rec), &
+         'dg_gpu_set_flow_series')
+    deallocate(rec)
+  end subroutine set_flow_series
 
   ! fort.22 (ATCF track) for NWS=8, read with the fixed columns GetHollandStormData reads
   ! (src/wind.F:2224-2227, FORMAT 14 at src/wind.F:2440): 12 integers per record for the library

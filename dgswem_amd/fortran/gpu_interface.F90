@@ -108,6 +108,33 @@ module dg_gpu_interface
        real(c_double), intent(in) :: slam(*), sfea(*)
      end function dg_gpu_set_holland
 
+     integer(c_int) function dg_gpu_set_elev_series(handle, neta, nbd, etiminc, nrec, esbin) &
+          bind(C, name='dg_gpu_set_elev_series')
+       import :: c_int, c_double
+       integer(c_int), value :: handle, neta, nrec
+       integer(c_int), intent(in) :: nbd(*)
+       real(c_double), value :: etiminc
+       real(c_double), intent(in) :: esbin(*)
+     end function dg_gpu_set_elev_series
+
+     integer(c_int) function dg_gpu_set_flow_series(handle, nvel, lbcodei, ftiminc, nrec, qnin) &
+          bind(C, name='dg_gpu_set_flow_series')
+       import :: c_int, c_double
+       integer(c_int), value :: handle, nvel, nrec
+       integer(c_int), intent(in) :: lbcodei(*)
+       real(c_double), value :: ftiminc
+       real(c_double), intent(in) :: qnin(*)
+     end function dg_gpu_set_flow_series
+
+     integer(c_int) function dg_gpu_set_tidal_potential(handle, ntif, tpk, amigt, etrf, fft, facet, &
+          saltamp, saltpha, slam, sfea, reftim) bind(C, name='dg_gpu_set_tidal_potential')
+       import :: c_int, c_double
+       integer(c_int), value :: handle, ntif
+       real(c_double), intent(in) :: tpk(*), amigt(*), etrf(*), fft(*), facet(*)
+       real(c_double), intent(in) :: saltamp(*), saltpha(*), slam(*), sfea(*)
+       real(c_double), value :: reftim
+     end function dg_gpu_set_tidal_potential
+
      type(c_ptr) function dg_gpu_last_error() bind(C, name='dg_gpu_last_error')
        import :: c_ptr
      end function dg_gpu_last_error

@@ -29,7 +29,8 @@ EXPORTED = ["dg_create_edge_data", "dg_gpu_init", "dg_gpu_set_tide", "dg_gpu_set
             "dg_gpu_set_state", "dg_gpu_get_state", "dg_gpu_step", "dg_gpu_synchronize",
             "dg_gpu_modal2nodal", "dg_gpu_nodal_update", "dg_gpu_get_nodal",
             "dg_gpu_set_etamax", "dg_gpu_set_eletab", "dg_gpu_mass", "dg_gpu_profile", "dg_gpu_kernel_stats",
-            "dg_gpu_set_holland", "dg_holland_storm_at", "dg_gpu_set_manning", "dg_gpu_set_depth",
+            "dg_gpu_set_holland", "dg_holland_storm_at", "dg_gpu_set_tidal_potential",
+            "dg_gpu_set_elev_series", "dg_gpu_set_flow_series", "dg_gpu_set_manning", "dg_gpu_set_depth",
             "dg_adj_set_gauges", "dg_adj_gauges", "dg_adj_gradient", "dg_ens_create", "dg_ens_layout", "dg_gpu_use_graph", "dg_gpu_autotune",
             "dg_gpu_set_block", "dg_gpu_last_error", "dg_gpu_finalize"]
 
@@ -83,6 +84,10 @@ def load_library(path: str = LIB_PATH):
     L.dg_gpu_set_holland.argtypes = [C.c_int, C.c_int, _ip, _ip, C.c_int, C.c_double, _dp, _dp]
     L.dg_holland_storm_at.argtypes = [C.c_int, _ip, _ip, C.c_int, C.c_double, C.c_double, C.c_int,
                                       _dp, _dp]
+    L.dg_gpu_set_tidal_potential.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_void_p,
+                                             C.c_void_p, _dp, _dp, C.c_double]
+    L.dg_gpu_set_elev_series.argtypes = [C.c_int, C.c_int, _ip, C.c_double, C.c_int, _dp]
+    L.dg_gpu_set_flow_series.argtypes = [C.c_int, C.c_int, _ip, C.c_double, C.c_int, _dp]
     L.dg_gpu_set_manning.argtypes = [C.c_int, _dp]
     L.dg_gpu_set_depth.argtypes = [C.c_int, _dp]
     L.dg_adj_set_gauges.argtypes = [C.c_int, C.c_int, _ip, _dp, _dp]
@@ -364,6 +369,23 @@ class GpuSolver(_Handle):
             qa, qp = qnam_dg(case)
             _check(L.dg_gpu_set_flow(self.h, p.nffr, _f(p.famig), _f(p.fff), _f(p.fface),
                                      _f(p.fper), _f(qa), _f(qp)))
+        if case.elev_series is not None:
+            inc, vals = case.elev_series
+            _check(L.dg_gpu_set_elev_series(self.h, vals.shape[1], _i(case.nbd()), float(inc),
+                                            vals.shape[0], _f(vals.reshape(-1))))
+        if case.flow_series is not None:
+            inc, vals = case.flow_series
+            lb = _i(case.lbcodei())
+            _check(L.dg_gpu_set_flow_series(self.h, len(lb), lb, float(inc), vals.shape[0],
+                                            _f(vals.reshape(-1))))
+        if p.ntip != 0:
+            # SALTAMP(MNTIF,MNP) column-major = [node][constituent] row-major
+            sa = _f(p.saltamp.T.reshape(-1)) if p.saltamp is not None else None
+            sp = _f(p.saltpha.T.reshape(-1)) if p.saltpha is not None else None
+            _check(L.dg_gpu_set_tidal_potential(
+                self.h, p.ntif, _f(p.tpk), _f(p.amigt), _f(p.etrf), _f(p.fft), _f(p.facet),
+                sa.ctypes.data if sa is not None else None, sp.ctypes.data if sp is not None else None,
+                _f(case.slam), _f(case.sfea), float(p.reftim)))
         if p.nws == 8:
             st = case.storm
             _check(L.dg_gpu_set_holland(self.h, len(st.records), _i(st.records.reshape(-1)),

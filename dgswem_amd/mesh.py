@@ -356,7 +356,7 @@ def write_run_dir(d: str, case: Case, dghot: int = 0, dghotspool: int = 0,
     a(str(1 if case.manning is not None else 0))
     if case.manning is not None:
         a("mannings_n_at_sea_floor")
-    a(str(p.ncor)); a("0"); a(str(p.nws)); a(str(p.nramp))
+    a(str(p.ncor)); a(str(p.ntip)); a(str(p.nws)); a(str(p.nramp))
     a(_r(p.g)); a(_r(p.tau0)); a(_r(p.dtdp)); a(_r(p.statim)); a(_r(p.reftim))
     if p.nws == 8:
         st = case.storm
@@ -373,7 +373,11 @@ def write_run_dir(d: str, case: Case, dghot: int = 0, dghotspool: int = 0,
         a(_r(p.cf))
     else:
         a(f"{_r(p.cf)} {_r(p.hbreak)} {_r(p.ftheta)} {_r(p.fgamma)}")
-    a(_r(p.eslm)); a(_r(p.cori)); a("0")
+    a(_r(p.eslm)); a(_r(p.cori))
+    a(str(p.ntif))
+    for i in range(p.ntif):
+        a(p.tipotag[i] if i < len(p.tipotag) else f"T{i}")
+        a(f"{_r(p.tpk[i])} {_r(p.amigt[i])} {_r(p.etrf[i])} {_r(p.fft[i])} {_r(p.facet[i] / DEG2RAD)}")
     a(str(p.nbfr))
     for i in range(p.nbfr):
         a(f"C{i}"); a(f"{_r(p.amig[i])} {_r(p.ff[i])} {_r(p.face[i] / DEG2RAD)}")
@@ -403,6 +407,19 @@ def write_run_dir(d: str, case: Case, dghot: int = 0, dghotspool: int = 0,
     if p.nws == 8:
         from .holland import write_fort22
         write_fort22(os.path.join(d, "fort.22"), case.storm.records)
+    if p.ntip == 2:
+        with open(os.path.join(d, "fort.24"), "w") as f:
+            for i in range(p.ntif):
+                f.write(f"SAL {i}\n1.0\n1\n{p.tipotag[i] if i < len(p.tipotag) else 'T'}\n")
+                for j in range(case.np_):
+                    f.write(f"{j + 1} {_r(p.saltamp[i, j])} {_r(p.saltpha[i, j] / DEG2RAD)}\n")
+    for name, ser in (("fort.19", case.elev_series), ("fort.20", case.flow_series)):
+        if ser is not None:
+            inc, vals = ser
+            with open(os.path.join(d, name), "w") as f:
+                f.write(_r(inc) + "\n")
+                for v in vals.reshape(-1):
+                    f.write(_r(v) + "\n")
     if case.manning is not None:
         with open(os.path.join(d, "fort.13"), "w") as f:
             f.write("attrs\n" + f"{case.np_}\n1\nmannings_n_at_sea_floor\nunitless\n1\n0.0\n")

@@ -35,7 +35,7 @@ typedef struct dg_params {
     /* Inputs the step does not implement, passed so that the library can refuse them instead of
      * running without their terms (each must be as stated, else dg_gpu_init fails):             */
     int nws;          /* NWS: 0, or 8 (then dg_gpu_set_holland before the first step)             */
-    int ntip;         /* NTIP: 0 (tidal potential forcing is not implemented)                      */
+    int ntip;         /* NTIP: 0, 1 or 2 (then dg_gpu_set_tidal_potential before the first step) */
     int artdif;       /* fort.dg artdif: 0 (no LDG artificial diffusion)                           */
     int pl, padapt;   /* fort.dg pl and padapt: pl == p and padapt == 0 (no p-adaptivity)          */
     int load_quadratic_fric; /* LoadQuadraticFric: 0 (FRIC must be the uniform CF)                */
@@ -94,6 +94,39 @@ int dg_gpu_set_tide(int handle, int nbfr, const double* amig, const double* ff,
 int dg_gpu_set_flow(int handle, int nffr, const double* famig, const double* fff,
                     const double* fface, const double* fper, const double* qnam_dg,
                     const double* qnph_dg);
+
+/* Non-periodic elevation boundary (NBFR = 0 with open boundaries): the fort.19 time series
+ * (src/cstart.F:80-97), in place of dg_gpu_set_tide.  Each stage sets the open-boundary nodes'
+ * ETA2 to RAMPDG*(ESBIN1 + ETRATIO*(ESBIN2-ESBIN1)) and the ocean edges interpolate it between their
+ * end nodes (src/DG_hydro_timestep.F:153-169, src/ocean_edge_hydro.F:180-184); a record is consumed
+ * whenever TIME_A passes the later record's time, as the reference reads unit 19.
+ *   neta, nbd   NETA and NBD(NETA), the open-boundary nodes (1-based)
+ *   etiminc     ETIMINC (s); record k holds the values at STATIM*86400 + k*ETIMINC
+ *   esbin       nrec records of NETA values, record-major (the file's order); nrec >= 2, and the
+ *               run fails with an error when it needs a record past the last */
+int dg_gpu_set_elev_series(int handle, int neta, const int* nbd, double etiminc, int nrec,
+                           const double* esbin);
+
+/* Non-periodic normal flux (NFFR = 0 with flux-specified boundaries): the fort.20 time series
+ * (src/cstart.F:101-124), in place of dg_gpu_set_flow.  Each stage forms QNAM_DG(1,L,1:2) from
+ * QNIN1/QNIN2 the way the reference does (src/DG_hydro_timestep.F:172-200, nodes I and I+1 of the
+ * NVEL list for the L-th flux node) and FLOW_EDGE_HYDRO applies it with RFF = RAMPDG.
+ *   nvel, lbcodei  NVEL and LBCODEI(NVEL): the boundary code of every land-boundary node
+ *   ftiminc        FTIMINC (s)
+ *   qnin           nrec records of one value per flux node (codes 2, 12, 22) in NVEL order */
+int dg_gpu_set_flow_series(int handle, int nvel, const int* lbcodei, double ftiminc, int nrec,
+                           const double* qnin);
+
+/* Tidal potential forcing (NTIP = 1 or 2; TIDAL_POTENTIAL, src/tidal_potential.F:15-80, and its
+ * source term in RHS_DG_HYDRO, src/rhs_dg_hydro.F:117-122,378-386).  The constituents as READ_INPUT
+ * holds them (src/read_input.F:1856-1923): TPK, AMIGT (rad/s), ETRF, FFT, FACET (radians), NTIF
+ * each; saltamp/saltpha SALTAMP(MNTIF,MNP) and SALTPHA (radians, column-major) for NTIP = 2, NULL
+ * for NTIP = 1; slam/sfea SLAM, SFEA (radians); reftim REFTIM (days).  TIP2 is evaluated at every
+ * node once per step for all stages; single runs only. */
+int dg_gpu_set_tidal_potential(int handle, int ntif, const double* tpk, const double* amigt,
+                               const double* etrf, const double* fft, const double* facet,
+                               const double* saltamp, const double* saltpha, const double* slam,
+                               const double* sfea, double reftim);
 
 /* Replace / read the stage-1 state ZE(DOFH,NE,1), QX, QY (column-major: [elem][mode]) and
  * WDFLG(NE) -- the arrays DG_HYDRO_TIMESTEP advances (src/dg.F:286-313). */

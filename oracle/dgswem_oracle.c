@@ -26,7 +26,12 @@
  *   RHS_DG_HYDRO .......... src/rhs_dg_hydro.F:18-473
  *   SLOPELIMITER5 ......... src/slopelimiter.F:3247-3783
  *   WETDRY ................ src/wetdry.F:17-553
- *   modal2nodal ........... src/write_results.F:14-174
+ *   modal2nodal ........... src/write_results.F:14-174 (with ETAMAX, :164)
+ *   TIDAL_POTENTIAL ....... src/tidal_potential.F:15-80 (+ its source, src/rhs_dg_hydro.F:378-386)
+ *   non-periodic elevation and flux boundaries (fort.19, fort.20)
+ *                           src/DG_hydro_timestep.F:153-200, src/cstart.F:80-124,
+ *                           src/ocean_edge_hydro.F:180-184, src/flow_edge_hydro.F:143-173
+ *   NWS=8 Holland vortex .. src/wind.F:116-141,1964-2450, src/met_forcing.F:479-520
  *
  * Deliberate differences from the reference (all only where the reference is broken for p=2):
  *   - internal edges trace and accumulate all DOFH modes (the reference hard-codes 3,
@@ -36,8 +41,12 @@
  * For p=1 both are identical to the reference.
  *
  * Parity status: p=1 is pinned against dgswem_serial (FP64 hot-start dumps and the reference's
- * fort.63.true fixtures, see tests/).  p=2 has no working reference and is pinned only by
+ * fort.63.true fixtures, see tests/), including the tidal potential (NTIP 1 and 2), both time-series
+ * boundaries and NWS=8, all bit for bit.  p=2 has no working reference and is pinned only by
  * invariants (lake at rest, mass conservation, symmetry); see tests/test_oracle_invariants.py.
+ * modal2nodal visits each node's elements in the reference's ELETAB order (NEIGHB's angular sort,
+ * src/dgswem.F:478-570, and src/prep_DG.F:292-330) and raises ETAMAX after every wet contribution,
+ * so ETA2 and ETAMAX are pinned against dgswem_serial's fort.63 and maxele.63 as well.
  */
 #include <math.h>
 #include <stdio.h>
@@ -117,6 +126,25 @@ typedef struct oracle {
     double *wsx2, *wsy2, *pr2;
     double drpsi[3], dspsi[3];
     int met_error;
+    /* tidal potential (NTIP = 1, 2): constituents, the SAL terms [ntif][np] and the stage's TIP2 */
+    int ntip, ntif;
+    double reftim, *tpk, *amigt, *etrf, *fft, *facet, *pert, *saltamp, *saltpha, *tip2;
+    double *tslam, *tsfea;
+    /* non-periodic elevation boundary (fort.19): records [nrec][neta], the two in use, their times,
+       and ETA2 at the boundary nodes (the reference writes these into its nodal ETA2 array) */
+    int eser, eser_nrec, eser_next, *nbd;
+    double etiminc, etime1, etime2, *esbin, *esbin1, *esbin2, *eta2bc;
+    /* non-periodic normal flux (fort.20): boundary codes of the NVEL land nodes, records of the
+       flux-node values, the two in use (QNIN1/QNIN2 over all NVEL nodes) and their times */
+    int fser, fser_nrec, fser_next, nvel, nfluxnodes, *lbcodei;
+    double ftiminc, qtime1, qtime2, *qnin, *qnin1, *qnin2;
+    int series_error;
+    /* frozen-branch SLOPELIMITER5 (a check of the adjoint): mode 1 records the branches every
+       limiter call takes, mode 2 replays recorded branches instead of taking them */
+    int lim_mode, lim_cap, lim_pos;
+    int *lim_node;   /* [call][np][6]: elements attaining each field's min and max bound, -1 none */
+    int *lim_elem;   /* [call][ne][3]: per field, the clamp (2 bits per vertex), pass signs, positive
+                        differences and min(redfac, redmax) choices, packed */
 } oracle_t;
 
 #define ZE(s, l, k) o->ze[((size_t)(s) * o->ne + (l)) * o->dofh + (k)]
@@ -212,6 +240,65 @@ void oracle_set_flow(oracle_t *o, int nffr, int nflownodes, const double *famig,
     o->famig = dup_d(famig, nffr); o->fff = dup_d(fff, nffr); o->fface = dup_d(fface, nffr);
     o->fper = dup_d(fper, nffr);
     o->qnam = dup_d(qnam, (size_t)nffr * nflownodes); o->qnph = dup_d(qnph, (size_t)nffr * nflownodes);
+}
+
+/* Tidal potential constituents (src/read_input.F:1856-1923: FACET in radians, PERT = 2*PI/AMIGT)
+   and, for NTIP = 2, the self-attraction/load terms SALTAMP, SALTPHA (radians) as [ntif][np]. */
+void oracle_set_tidal_potential(oracle_t *o, int ntip, int ntif, const double *tpk, const double *amigt,
+                                const double *etrf, const double *fft, const double *facet,
+                                const double *pert, const double *saltamp, const double *saltpha,
+                                const double *slam, const double *sfea, double reftim) {
+    o->ntip = ntip; o->ntif = ntif; o->reftim = reftim;
+    o->tpk = dup_d(tpk, ntif); o->amigt = dup_d(amigt, ntif); o->etrf = dup_d(etrf, ntif);
+    o->fft = dup_d(fft, ntif); o->facet = dup_d(facet, ntif); o->pert = dup_d(pert, ntif);
+    o->saltamp = (double *)xcalloc((size_t)ntif * o->np + 1, sizeof(double));
+    o->saltpha = (double *)xcalloc((size_t)ntif * o->np + 1, sizeof(double));
+    if (saltamp) memcpy(o->saltamp, saltamp, (size_t)ntif * o->np * sizeof(double));
+    if (saltpha) memcpy(o->saltpha, saltpha, (size_t)ntif * o->np * sizeof(double));
+    o->tslam = dup_d(slam, o->np); o->tsfea = dup_d(sfea, o->np);
+    o->tip2 = (double *)xcalloc(o->np, sizeof(double));
+}
+
+/* fort.19 (src/cstart.F:80-97): ETIMINC and nrec records of the NETA open-boundary node values;
+   nbd: the open-boundary nodes in NBD order (0-based). */
+int oracle_set_elev_series(oracle_t *o, int neta, const int *nbd, double etiminc, int nrec,
+                           const double *esbin) {
+    if (nrec < 2) return 1;
+    o->eser = 1; o->eser_nrec = nrec; o->etiminc = etiminc;
+    o->nbd = (int *)xcalloc(neta, sizeof(int));
+    memcpy(o->nbd, nbd, neta * sizeof(int));
+    o->esbin = dup_d(esbin, (size_t)nrec * neta);
+    o->esbin1 = dup_d(esbin, neta);
+    o->esbin2 = dup_d(esbin + neta, neta);
+    o->eser_next = 2;
+    o->etime1 = o->statim * 86400.0;
+    o->etime2 = o->etime1 + etiminc;
+    o->eta2bc = (double *)xcalloc(o->np, sizeof(double));
+    o->neta = neta;
+    return 0;
+}
+
+/* fort.20 (src/cstart.F:101-124): FTIMINC, the NVEL land-boundary node codes LBCODEI and nrec
+   records of one value per flux node (codes 2, 12, 22) in NVEL order. */
+int oracle_set_flow_series(oracle_t *o, int nvel, const int *lbcodei, double ftiminc, int nrec,
+                           const double *qnin) {
+    if (nrec < 2) return 1;
+    o->fser = 1; o->fser_nrec = nrec; o->ftiminc = ftiminc; o->nvel = nvel;
+    o->lbcodei = (int *)xcalloc(nvel, sizeof(int));
+    memcpy(o->lbcodei, lbcodei, nvel * sizeof(int));
+    int nf = 0;
+    for (int j = 0; j < nvel; j++) if (lbcodei[j] == 2 || lbcodei[j] == 12 || lbcodei[j] == 22) nf++;
+    o->nfluxnodes = nf;
+    o->qnin = dup_d(qnin, (size_t)nrec * nf);
+    /* QNIN has one slot past NVEL: the edge of the last flux node reads QNIN(I+1) */
+    o->qnin1 = (double *)xcalloc(nvel + 1, sizeof(double));
+    o->qnin2 = (double *)xcalloc(nvel + 1, sizeof(double));
+    for (int j = 0, k = 0; j < nvel; j++)
+        if (lbcodei[j] == 2 || lbcodei[j] == 12 || lbcodei[j] == 22) { o->qnin1[j] = qnin[k]; o->qnin2[j] = qnin[nf + k]; k++; }
+    o->fser_next = 2;
+    o->qtime1 = o->statim * 86400.0;
+    o->qtime2 = o->qtime1 + ftiminc;
+    return 0;
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -505,6 +592,77 @@ static int create_edges(oracle_t *o) {
 /* ------------------------------------------------------------------------------------------ */
 /* PREP_DG                                                                                     */
 /* ------------------------------------------------------------------------------------------ */
+/* NEIGHB (src/dgswem.F:478-570): each node's neighbours in insertion order, then sorted by
+   increasing clockwise angle from east; PREP_DG's ELETAB (src/prep_DG.F:292-330): for consecutive
+   neighbour pairs (J2, J3) around node I, cyclically, the element holding I, J2 and J3.  The
+   node's segment of o->eletab (filled in element order) is rewritten in that order; a node whose
+   fan does not give all its elements keeps the element order. */
+static void eletab_reference_order(oracle_t *o) {
+    int np = o->np, ne = o->ne;
+    int *cnt = (int *)xcalloc(np, sizeof(int));
+    int cap = 0;
+    for (int i = 0; i < np; i++) if (o->el_count[i] + 2 > cap) cap = o->el_count[i] + 2;
+    int *nb = (int *)xcalloc((size_t)np * cap, sizeof(int));
+    for (int n = 0; n < ne; n++) {
+        int en[3] = {o->nm[3 * n], o->nm[3 * n + 1], o->nm[3 * n + 2]};
+        const int pr[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+        for (int q = 0; q < 3; q++) {
+            int a = en[pr[q][0]], b = en[pr[q][1]], found = 0;
+            for (int j = 0; j < cnt[a]; j++) if (nb[(size_t)a * cap + j] == b) { found = 1; break; }
+            if (found) continue;
+            if (cnt[a] >= cap || cnt[b] >= cap) continue;
+            nb[(size_t)a * cap + cnt[a]++] = b;
+            nb[(size_t)b * cap + cnt[b]++] = a;
+        }
+    }
+    const double rad2deg = 45.0 / atan(1.0);
+    double *ang = (double *)xcalloc(cap, sizeof(double));
+    int *sorted = (int *)xcalloc(cap + 1, sizeof(int));
+    int *tmp = (int *)xcalloc(cap + 1, sizeof(int));
+    for (int i = 0; i < np; i++) {
+        int nn = cnt[i];
+        for (int j = 0; j < nn; j++) {
+            int m = nb[(size_t)i * cap + j];
+            double delx = o->x[m] - o->x[i], dely = o->y[m] - o->y[i];
+            double dist = sqrt(delx * delx + dely * dely);
+            ang[j] = 0.0;
+            if (dely != 0.0) {
+                ang[j] = rad2deg * acos(delx / dist);
+                if (dely > 0.0) ang[j] = 360.0 - ang[j];
+            }
+            if (dely == 0.0) {
+                if (delx > 0.0) ang[j] = 0.0;
+                if (delx < 0.0) ang[j] = 180.0;
+            }
+        }
+        double more = -1.0;
+        int jlow = 0;
+        for (int jj = 0; jj < nn; jj++) {
+            double low = 400.0;
+            for (int j = 0; j < nn; j++)
+                if (ang[j] < low && ang[j] > more) { low = ang[j]; jlow = j; }
+            sorted[1 + jj] = nb[(size_t)i * cap + jlow];
+            more = low;
+        }
+        sorted[0] = i;
+        int nneigh = nn + 1, kk = 0;
+        const int *seg = o->eletab + o->eletab_off[i];
+        for (int k = 1; k <= nneigh - 1; k++) {
+            int j2 = sorted[k], j3 = (k < nneigh - 1) ? sorted[k + 1] : sorted[1];
+            for (int j = 0; j < o->el_count[i]; j++) {
+                int el = seg[j];
+                const int *v = &o->nm[3 * el];
+                int h1 = v[0] == i || v[1] == i || v[2] == i;
+                int h2 = v[0] == j2 || v[1] == j2 || v[2] == j2;
+                int h3 = v[0] == j3 || v[1] == j3 || v[2] == j3;
+                if (h1 && h2 && h3) { if (kk < cap) tmp[kk] = el; kk++; break; }
+            }
+        }
+        if (kk == o->el_count[i]) memcpy(o->eletab + o->eletab_off[i], tmp, kk * sizeof(int));
+    }
+    free(cnt); free(nb); free(ang); free(sorted); free(tmp);
+}
+
 int oracle_prep(oracle_t *o) {
     int ne = o->ne, dofh = o->dofh, nagp = o->nagp, negp = o->negp, p = o->p;
     if (p < 1 || p > 2) return -1;
@@ -534,12 +692,10 @@ int oracle_prep(oracle_t *o) {
             jj++;
         }
     }
-    if (o->needs > 0 && o->nbfr == 0) { fprintf(stderr, "oracle: non-periodic elevation BCs not supported\n"); return -1; }
     /* flow edges (src/prep_DG.F:211-250) */
     int nffr = o->nffr;
     o->qnam_dg = (double *)xcalloc((size_t)o->nfeds * (nffr ? nffr : 1) * 2, sizeof(double));
     o->qnph_dg = (double *)xcalloc((size_t)o->nfeds * (nffr ? nffr : 1) * 2, sizeof(double));
-    if (o->nfeds > 0 && nffr == 0) { fprintf(stderr, "oracle: time-series flux BCs not supported\n"); return -1; }
     for (int i = 0; i < nffr; i++) {
         int ii = 0, jj = 0;
         for (int j = 0; j < o->nbou; j++) {
@@ -556,7 +712,7 @@ int oracle_prep(oracle_t *o) {
         }
     }
 
-    /* node -> element table (ELETAB; order only affects modal2nodal round-off) */
+    /* node -> element table: ELETAB, in PREP_DG's angular order (modal2nodal's summation order) */
     o->etamax = (double *)xcalloc(o->np, sizeof(double));
     for (int i = 0; i < o->np; i++) o->etamax[i] = -99999.0;      /* src/prep_DG.F:295 */
     o->el_count = (int *)xcalloc(o->np, sizeof(int));
@@ -569,6 +725,7 @@ int oracle_prep(oracle_t *o) {
         for (int e = 0; e < ne; e++) for (int k = 0; k < 3; k++) { int n = o->nm[3 * e + k]; o->eletab[o->eletab_off[n] + f[n]++] = e; }
         free(f);
     }
+    eletab_reference_order(o);
 
     /* quadrature and basis tables */
     tri_rule(p, o->xagp, o->yagp, o->wagp);
@@ -805,6 +962,11 @@ static void ocean_edges(oracle_t *o) {
                 double zf = 0.5 * (zg[0] + zg[1]) + 0.5 * (zg[1] - zg[0]) * o->xegp[I];
                 ze_ex = ze_ex + zf;
             }
+            if (o->nbfr == 0 && o->eser) {
+                /* non-periodic elevation: the boundary nodes' values of the stage */
+                int n1 = o->nedno[2 * ged], n2 = o->nedno[2 * ged + 1];
+                ze_ex = 0.5 * (o->eta2bc[n1] + o->eta2bc[n2]) + 0.5 * (o->eta2bc[n2] - o->eta2bc[n1]) * o->xegp[I];
+            }
             for (int K = 1; K < dofh; K++) {
                 double ph = o->phi_edge[led][I][K];
                 ze_in = ze_in + ZE(irk, el, K) * ph;
@@ -867,12 +1029,18 @@ static void flow_edges(oracle_t *o) {
             double ze_in = 0.0, qx_in = 0.0, qy_in = 0.0, qx_ex = 0.0, qy_ex = 0.0, hb_in = 0.0;
             double sfac_in = SFACED(el, led, I);
             double qn_ext = 0.0, qt_ext = 0.0;
-            for (int jj = 0; jj < nffr; jj++) {
-                int ncyc = (o->fper[jj] == 0.0) ? 0 : (int)(o->timedg / o->fper[jj]);
-                double argj = o->famig[jj] * (o->timedg - ncyc * o->fper[jj]) + o->fface[jj];
-                double rff = o->fff[jj] * o->rampdg;
-                const double *qa = &o->qnam_dg[((size_t)L * nffr + jj) * 2];
-                const double *qp = &o->qnph_dg[((size_t)L * nffr + jj) * 2];
+            /* NFFR = 0: one term from the time series, ARGJ = 0 and RFF = RAMPDG (the values are
+               ramped once more where they are interpolated) */
+            int nterm = o->fser ? 1 : nffr, nf1 = o->fser ? 1 : nffr;
+            for (int jj = 0; jj < nterm; jj++) {
+                double argj = 0.0, rff = o->rampdg;
+                if (!o->fser) {
+                    int ncyc = (o->fper[jj] == 0.0) ? 0 : (int)(o->timedg / o->fper[jj]);
+                    argj = o->famig[jj] * (o->timedg - ncyc * o->fper[jj]) + o->fface[jj];
+                    rff = o->fff[jj] * o->rampdg;
+                }
+                const double *qa = &o->qnam_dg[((size_t)L * nf1 + jj) * 2];
+                const double *qp = &o->qnph_dg[((size_t)L * nf1 + jj) * 2];
                 double qnam_gp = 0.5 * (qa[0] + qa[1]) + 0.5 * (qa[1] - qa[0]) * o->xegp[I];
                 double qnph_gp = 0.5 * (qp[0] + qp[1]) + 0.5 * (qp[1] - qp[0]) * o->xegp[I];
                 double arg = argj - qnph_gp;
@@ -1197,6 +1365,91 @@ static void met_forcing(oracle_t *o, double time) {
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* tidal potential and non-periodic boundary forcing of a stage                                */
+/* ------------------------------------------------------------------------------------------ */
+/* TIDAL_POTENTIAL (src/tidal_potential.F:15-80): TIP2 at every node for the stage. */
+static void tidal_potential(oracle_t *o, int it) {
+    double timeh = it * o->dtdp + (o->statim - o->reftim) * 86400.0;
+    double timeh_dg = timeh - o->dtdp + o->dtvd[o->irk] * o->dtdp;
+#pragma omp parallel for num_threads(o->nthreads) schedule(static)
+    for (int i = 0; i < o->np; i++) {
+        double tip = 0.0;
+        for (int j = 0; j < o->ntif; j++) {
+            int ncyc = (o->pert[j] == 0.0) ? 0 : (int)(timeh_dg / o->pert[j]);
+            double argt = o->amigt[j] * (timeh_dg - ncyc * o->pert[j]) + o->facet[j];
+            double tpmul = o->rampdg * o->etrf[j] * o->tpk[j] * o->fft[j];
+            double saltmul = o->rampdg * o->fft[j];
+            long na = lround(0.00014 / o->amigt[j]);
+            double sa = o->saltamp[(size_t)j * o->np + i], sp = o->saltpha[(size_t)j * o->np + i];
+            if (na == 1) {
+                double argtp = argt + 2.0 * o->tslam[i];
+                double argsalt = argt - sp;
+                double ccsfea = cos(o->tsfea[i]);
+                ccsfea = ccsfea * ccsfea;
+                tip = tip + tpmul * ccsfea * cos(argtp) + saltmul * sa * cos(argsalt);
+            }
+            if (na == 2) {
+                double argtp = argt + o->tslam[i];
+                double argsalt = argt - sp;
+                double s2sfea = sin(2.0 * o->tsfea[i]);
+                tip = tip + tpmul * s2sfea * cos(argtp) + saltmul * sa * cos(argsalt);
+            }
+        }
+        o->tip2[i] = tip;
+    }
+}
+
+/* Non-periodic elevation (src/DG_hydro_timestep.F:153-169) and normal flux
+   (src/DG_hydro_timestep.F:172-200) boundary values of the stage. */
+static void boundary_series(oracle_t *o, int it) {
+    double time_a = it * o->dtdp + o->statim * 86400.0;
+    if (o->eser) {
+        if (time_a > o->etime2) {
+            o->etime1 = o->etime2;
+            o->etime2 = o->etime1 + o->etiminc;
+            if (o->eser_next >= o->eser_nrec) { o->series_error = 1; return; }
+            memcpy(o->esbin1, o->esbin2, o->neta * sizeof(double));
+            memcpy(o->esbin2, o->esbin + (size_t)o->eser_next * o->neta, o->neta * sizeof(double));
+            o->eser_next++;
+        }
+        double etratio = (o->timedg - o->etime1) / o->etiminc;
+        for (int i = 0; i < o->neta; i++)
+            o->eta2bc[o->nbd[i]] = o->rampdg * (o->esbin1[i] + etratio * (o->esbin2[i] - o->esbin1[i]));
+    }
+    if (o->fser) {
+        if (time_a > o->qtime2) {
+            o->qtime1 = o->qtime2;
+            o->qtime2 = o->qtime1 + o->ftiminc;
+            if (o->fser_next >= o->fser_nrec) { o->series_error = 1; return; }
+            const double *rec = o->qnin + (size_t)o->fser_next * o->nfluxnodes;
+            for (int j = 0, k = 0; j < o->nvel; j++) {
+                o->qnin1[j] = o->qnin2[j];
+                o->qnin2[j] = 0.0;
+                int c = o->lbcodei[j];
+                if (c == 2 || c == 12 || c == 22) o->qnin2[j] = rec[k++];
+            }
+            o->fser_next++;
+        }
+        double qtratio = (o->timedg - o->qtime1) / o->ftiminc;
+        int nqeds = 0;
+        for (int i = 0; i < o->nvel; i++) {
+            int c = o->lbcodei[i];
+            if (c == 2 || c == 12 || c == 22) {
+                nqeds++;
+                if (nqeds <= o->nfeds) {
+                    double *qa = &o->qnam_dg[(size_t)(nqeds - 1) * 2];
+                    double *qp = &o->qnph_dg[(size_t)(nqeds - 1) * 2];
+                    qa[0] = o->rampdg * (o->qnin1[i] + qtratio * (o->qnin2[i] - o->qnin1[i]));
+                    qp[0] = 0.0;
+                    qa[1] = o->rampdg * (o->qnin1[i + 1] + qtratio * (o->qnin2[i + 1] - o->qnin1[i + 1]));
+                    qp[1] = 0.0;
+                }
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* volume RHS                                                                                  */
 /* ------------------------------------------------------------------------------------------ */
 static void rhs_volume(oracle_t *o) {
@@ -1267,6 +1520,18 @@ static void rhs_volume(oracle_t *o) {
                 sy = sy + fw * (o->wsy2[n1] * o->psi1[I] + o->wsy2[n2] * o->psi2[I] + o->wsy2[n3] * o->psi3[I]) -
                      o->g * depth * (o->pr2[n1] * dpsidy[0] + o->pr2[n2] * dpsidy[1] + o->pr2[n3] * dpsidy[2]);
             }
+            if (o->ntip != 0) {
+                /* tidal potential (src/rhs_dg_hydro.F:378-386) */
+                double dpsidx[3], dpsidy[3];
+                for (int k = 0; k < 3; k++) {
+                    dpsidx[k] = o->drpsi[k] * o->drdx[L] + o->dspsi[k] * o->dsdx[L];
+                    dpsidy[k] = o->drpsi[k] * o->drdy[L] + o->dspsi[k] * o->dsdy[L];
+                }
+                sx = sx + o->rampdg * o->g * depth * sfq *
+                              (dpsidx[0] * o->tip2[n1] + dpsidx[1] * o->tip2[n2] + dpsidx[2] * o->tip2[n3]);
+                sy = sy + o->rampdg * o->g * depth *
+                              (dpsidy[0] * o->tip2[n1] + dpsidy[1] * o->tip2[n2] + dpsidy[2] * o->tip2[n3]);
+            }
             size_t t0 = ((size_t)L * nagp + I) * dofh;
             RQX(irk, L, 0) = RQX(irk, L, 0) + o->srfac[t0] * sx;
             RQY(irk, L, 0) = RQY(irk, L, 0) + o->srfac[t0] * sy;
@@ -1298,7 +1563,127 @@ static void rk_update(oracle_t *o) {
         }
 }
 
+/* Branch bits of one field of one element (the order the adjoint kernel records them in):
+   bits 0-5 vertex clamp (0 none, 1 to the max bound, 2 to the min bound; two bits per vertex),
+   bits 6-8 the sign of SUMDIF of passes 1-3 (1 for +1), bits 9-17 DIF(K) > 0 of pass ll at bit
+   9+3ll+k, bits 18-26 REDMAX chosen over REDFAC at bit 18+3ll+k. */
+static void limiter_element(const oracle_t *o, const double zec[3], const double vmin[3],
+                            const double vmax[3], int mode, int *bits, double out[2]) {
+    double v[3] = {zec[0], zec[0], zec[0]};
+    for (int kk = 1; kk < 3; kk++)
+        for (int c = 0; c < 3; c++) v[c] = v[c] + o->phi_corner[c][kk] * zec[kk];
+    int b = mode == 2 ? *bits : 0;
+    for (int c = 0; c < 3; c++) {
+        int sel;
+        if (mode == 2) sel = (b >> (2 * c)) & 3;
+        else sel = !(v[c] <= vmax[c]) ? 1 : (fmin(v[c], vmax[c]) < vmin[c] ? 2 : 0);
+        v[c] = sel == 1 ? vmax[c] : (sel == 2 ? vmin[c] : v[c]);
+        /* the free forward: fmax(fmin(v, vmax), vmin), the same value */
+        if (mode != 2) b |= sel << (2 * c);
+    }
+    for (int ll = 0; ll < 3; ll++) {
+        double sumloc = (v[0] + v[1] + v[2]) / 3.0;
+        double sumdif = (sumloc - zec[0]) * 3.0;
+        double signdif;
+        if (mode == 2) signdif = (b >> (6 + ll)) & 1 ? 1.0 : -1.0;
+        else { signdif = copysign(1.0, sumdif); if (signdif > 0) b |= 1 << (6 + ll); }
+        double dif[3];
+        int posb = 0;
+        for (int c = 0; c < 3; c++) {
+            dif[c] = (v[c] - zec[0]) * signdif;
+            if (mode == 2) posb |= ((b >> (9 + 3 * ll + c)) & 1) << c;
+            else if (dif[c] > 0) { posb |= 1 << c; b |= 1 << (9 + 3 * ll + c); }
+        }
+        int kdp = (posb & 1) + ((posb >> 1) & 1) + ((posb >> 2) & 1);
+        for (int k = 0; k < 3; k++) {
+            double div = fmax(1.0, (double)kdp);
+            double redfac = 0.0;
+            if ((posb >> k) & 1) { redfac = sumdif * signdif / div; kdp = kdp - 1; }
+            double redmax = (signdif > 0) ? v[k] - vmin[k] : vmax[k] - v[k];
+            int usemax;
+            if (mode == 2) usemax = (b >> (18 + 3 * ll + k)) & 1;
+            else { usemax = !(redfac <= redmax); if (usemax) b |= 1 << (18 + 3 * ll + k); }
+            if (usemax) redfac = redmax;
+            sumdif = sumdif - redfac * signdif;
+            v[k] = v[k] - redfac * signdif;
+        }
+    }
+    if (mode != 2) *bits = b;
+    out[0] = -(1.0 / 6.0 * (v[0] + v[1])) + 1.0 / 3.0 * v[2];
+    out[1] = -(0.5 * v[0]) + 0.5 * v[1];
+}
+
+/* SLOPELIMITER5 with its branches recorded (mode 1) or replayed (mode 2): the node bounds come
+   from the recorded bound-attaining elements, the element pass from limiter_element.  The
+   recorded map is what the adjoint differentiates; replayed at the recording point it gives the
+   free limiter's result bit for bit. */
+static void slopelimiter5_frozen(oracle_t *o) {
+    int s = o->irk + 1, call = o->lim_pos++;
+    if (call >= o->lim_cap) { o->series_error = 1; return; }
+    int *nrec = o->lim_node + (size_t)call * o->np * 6;
+    int *erec = o->lim_elem + (size_t)call * o->ne * 3;
+    double *fld[3] = {o->ze, o->qx, o->qy};
+    double *mnv[3] = {o->zemin, o->qxmin, o->qymin}, *mxv[3] = {o->zemax, o->qxmax, o->qymax};
+    for (int i = 0; i < o->np; i++) {
+        for (int f = 0; f < 3; f++) {
+            int alo = -1, ahi = -1;
+            double lo = 99999., hi = -99999.;
+            if (o->lim_mode == 2) {
+                alo = nrec[(size_t)i * 6 + 2 * f]; ahi = nrec[(size_t)i * 6 + 2 * f + 1];
+                if (alo >= 0) lo = fld[f][((size_t)s * o->ne + alo) * o->dofh];
+                if (ahi >= 0) hi = fld[f][((size_t)s * o->ne + ahi) * o->dofh];
+            } else {
+                for (int j = o->eletab_off[i]; j < o->eletab_off[i + 1]; j++) {
+                    int e = o->eletab[j];
+                    double c = fld[f][((size_t)s * o->ne + e) * o->dofh];
+                    if (c < lo) { lo = c; alo = e; }
+                    if (c > hi) { hi = c; ahi = e; }
+                }
+                nrec[(size_t)i * 6 + 2 * f] = alo; nrec[(size_t)i * 6 + 2 * f + 1] = ahi;
+            }
+            mnv[f][i] = lo; mxv[f][i] = hi;
+        }
+    }
+    for (int I = 0; I < o->ne; I++) {
+        int nn[3] = {o->nm[3 * I], o->nm[3 * I + 1], o->nm[3 * I + 2]};
+        for (int f = 0; f < 3; f++) {
+            double *u = &fld[f][((size_t)s * o->ne + I) * o->dofh];
+            double zec[3] = {u[0], u[1], u[2]};
+            double vmax[3] = {mxv[f][nn[0]], mxv[f][nn[1]], mxv[f][nn[2]]};
+            double vmin[3] = {mnv[f][nn[0]], mnv[f][nn[1]], mnv[f][nn[2]]};
+            double out[2];
+            limiter_element(o, zec, vmin, vmax, o->lim_mode, &erec[(size_t)I * 3 + f], out);
+            u[1] = out[0];
+            u[2] = out[1];
+        }
+    }
+}
+
+/* Record the limiter branches of the next ncalls limiter calls (NRK per step). */
+void oracle_limiter_record(oracle_t *o, int ncalls) {
+    free(o->lim_node); free(o->lim_elem);
+    o->lim_node = (int *)xcalloc((size_t)ncalls * o->np * 6 + 1, sizeof(int));
+    o->lim_elem = (int *)xcalloc((size_t)ncalls * o->ne * 3 + 1, sizeof(int));
+    o->lim_cap = ncalls; o->lim_pos = 0; o->lim_mode = 1;
+}
+/* Copy out / in the records of ncalls calls ([call][np][6] and [call][ne][3]). */
+int oracle_limiter_get(const oracle_t *o, int *node, int *elem) {
+    if (!o->lim_node) return 1;
+    memcpy(node, o->lim_node, (size_t)o->lim_cap * o->np * 6 * sizeof(int));
+    memcpy(elem, o->lim_elem, (size_t)o->lim_cap * o->ne * 3 * sizeof(int));
+    return 0;
+}
+/* Replay: the next ncalls limiter calls take the given branches; WETDRY is left out (the adjoint
+   takes it as the identity). */
+void oracle_limiter_replay(oracle_t *o, int ncalls, const int *node, const int *elem) {
+    oracle_limiter_record(o, ncalls);
+    memcpy(o->lim_node, node, (size_t)ncalls * o->np * 6 * sizeof(int));
+    memcpy(o->lim_elem, elem, (size_t)ncalls * o->ne * 3 * sizeof(int));
+    o->lim_mode = 2;
+}
+
 static void slopelimiter5(oracle_t *o) {
+    if (o->lim_mode != 0) { slopelimiter5_frozen(o); return; }
     int s = o->irk + 1;
 #pragma omp parallel for num_threads(o->nthreads) schedule(static)
     for (int i = 0; i < o->np; i++) {
@@ -1482,6 +1867,13 @@ void oracle_rhs(oracle_t *o, int it, int irk) {
     o->irk = irk;
     stage_time(o, it, irk);
     if (o->nws == 8) met_forcing(o, it * o->dtdp + o->statim * 86400.0);
+    if (o->ntip != 0) tidal_potential(o, it);
+    /* NBFR = 0 with open boundaries, NFFR = 0 with flux boundaries: the time series must be set */
+    if ((o->needs > 0 && o->nbfr == 0 && !o->eser) || (o->nfeds > 0 && o->nffr == 0 && !o->fser)) {
+        o->series_error = 1;
+        return;
+    }
+    if (o->eser || o->fser) boundary_series(o, it);
     if (o->needs > 0) ocean_edges(o);
     if (o->nleds > 0) land_edges(o);
     if (o->nfeds > 0) flow_edges(o);
@@ -1495,7 +1887,7 @@ void oracle_step(oracle_t *o, int it) {
         oracle_rhs(o, it, irk);
         rk_update(o);
         if (o->slopeflag != 0) slopelimiter5(o);
-        if (o->nolifa >= 2) wetdry(o);
+        if (o->nolifa >= 2 && o->lim_mode != 2) wetdry(o);
     }
     size_t n = (size_t)o->ne * o->dofh;
     memcpy(o->ze, o->ze + (size_t)o->nrk * n, n * sizeof(double));
@@ -1618,6 +2010,8 @@ void oracle_get_tables(const oracle_t *o, double *phi_area, double *phi_edge, do
 }
 /* Nonzero once a met-forcing time fell outside the track (the reference STOPs there). */
 int oracle_met_error(const oracle_t *o) { return o->met_error; }
+int oracle_series_error(const oracle_t *o) { return o->series_error; }
+void oracle_get_tip2(const oracle_t *o, double *tip2) { memcpy(tip2, o->tip2, o->np * sizeof(double)); }
 
 /* The stage's node forcing WSX2, WSY2, PR2 (after the last oracle_rhs). */
 void oracle_get_met(const oracle_t *o, double *wsx2, double *wsy2, double *pr2) {
@@ -1646,7 +2040,10 @@ void oracle_free(oracle_t *o) {
                     o->dbathdy, o->sfac_elem, o->xfac, o->yfac, o->srfac, o->bathed, o->sfaced, o->edgeq,
                     o->dp_node, o->el_count, o->eletab_off, o->eletab, o->ze, o->qx, o->qy, o->rhs_ze,
                     o->rhs_qx, o->rhs_qy, o->ze_edge, o->qx_edge, o->qy_edge, o->wdflg, o->nodecode,
-                    o->zemin, o->zemax, o->qxmin, o->qxmax, o->qymin, o->qymax, o->etamax};
+                    o->zemin, o->zemax, o->qxmin, o->qxmax, o->qymin, o->qymax, o->etamax,
+                    o->tpk, o->amigt, o->etrf, o->fft, o->facet, o->pert, o->saltamp, o->saltpha,
+                    o->tip2, o->tslam, o->tsfea, o->nbd, o->esbin, o->esbin1, o->esbin2, o->eta2bc,
+                    o->lbcodei, o->qnin, o->qnin1, o->qnin2, o->lim_node, o->lim_elem};
     for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; k++) free(ptrs[k]);
     free(o);
 }

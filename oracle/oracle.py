@@ -45,6 +45,16 @@ def lib():
         L.oracle_clear_rhs.argtypes = [C.c_void_p]
         L.oracle_get_nodecode.argtypes = [C.c_void_p, _ip]
         L.oracle_get_etamax.argtypes = [C.c_void_p, _dp]
+        L.oracle_set_tidal_potential.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp,
+                                                 _dp, _dp, C.c_void_p, C.c_void_p, _dp, _dp,
+                                                 C.c_double]
+        L.oracle_set_elev_series.argtypes = [C.c_void_p, C.c_int, _ip, C.c_double, C.c_int, _dp]
+        L.oracle_set_flow_series.argtypes = [C.c_void_p, C.c_int, _ip, C.c_double, C.c_int, _dp]
+        L.oracle_series_error.argtypes = [C.c_void_p]
+        L.oracle_get_tip2.argtypes = [C.c_void_p, _dp]
+        L.oracle_limiter_record.argtypes = [C.c_void_p, C.c_int]
+        L.oracle_limiter_get.argtypes = [C.c_void_p, _ip, _ip]
+        L.oracle_limiter_replay.argtypes = [C.c_void_p, C.c_int, _ip, _ip]
         L.oracle_set_etamax.argtypes = [C.c_void_p, _dp]
         L.oracle_num_edges.argtypes = [C.c_void_p, _ip]
         L.oracle_num_edges.restype = C.c_int
@@ -114,6 +124,25 @@ class Oracle:
         rc = L.oracle_prep(self.h)
         if rc != 0:
             raise RuntimeError("oracle_prep failed")
+        if p.ntip != 0:
+            sa = _f(p.saltamp.reshape(-1)) if p.saltamp is not None else None
+            sp = _f(p.saltpha.reshape(-1)) if p.saltpha is not None else None
+            self._keep += [sa, sp]
+            L.oracle_set_tidal_potential(self.h, p.ntip, p.ntif, _f(p.tpk), _f(p.amigt), _f(p.etrf),
+                                         _f(p.fft), _f(p.facet), _f(p.pert),
+                                         sa.ctypes.data if sa is not None else None,
+                                         sp.ctypes.data if sp is not None else None,
+                                         _f(case.slam), _f(case.sfea), p.reftim)
+        if case.elev_series is not None:
+            inc, vals = case.elev_series
+            if L.oracle_set_elev_series(self.h, vals.shape[1], _i(case.nbd() - 1), inc, vals.shape[0],
+                                        _f(vals.reshape(-1))) != 0:
+                raise RuntimeError("fort.19 needs at least two records")
+        if case.flow_series is not None:
+            inc, vals = case.flow_series
+            lb = _i(case.lbcodei())
+            if L.oracle_set_flow_series(self.h, len(lb), lb, inc, vals.shape[0], _f(vals.reshape(-1))) != 0:
+                raise RuntimeError("fort.20 needs at least two records")
         if p.nws == 8:
             st = case.storm
             rc = L.oracle_set_holland(self.h, len(st.records), _i(st.records.reshape(-1)),
@@ -133,6 +162,9 @@ class Oracle:
         self.it += n
         if lib().oracle_met_error(self.h):
             raise RuntimeError("the storm track does not cover the model time")
+        if lib().oracle_series_error(self.h):
+            raise RuntimeError("a boundary time series (fort.19/fort.20) or the limiter record "
+                               "ended before the run")
 
     def storm_at(self, time: float):
         """Storm scalars {lat, lon, cpress, deficit, spd, rmw, tvx, tvy, B} at `time` (call with
@@ -177,6 +209,26 @@ class Oracle:
         nc = np.empty(self.np_, dtype=np.int32)
         lib().oracle_get_nodecode(self.h, nc)
         return e, u, v, nc
+
+    def limiter_record(self, nsteps: int):
+        """Record the slope limiter's branches over the next `nsteps` steps."""
+        self._lim_calls = nsteps * self.nrk
+        lib().oracle_limiter_record(self.h, self._lim_calls)
+
+    def limiter_records(self):
+        node = np.empty(self._lim_calls * self.np_ * 6, np.int32)
+        elem = np.empty(self._lim_calls * self.ne * 3, np.int32)
+        if lib().oracle_limiter_get(self.h, node, elem) != 0:
+            raise RuntimeError("no limiter record")
+        return node, elem
+
+    def limiter_replay(self, records):
+        """The next steps take the recorded limiter branches (the frozen-branch map the adjoint
+        differentiates); WETDRY is left out, as the adjoint takes it as the identity."""
+        node, elem = records
+        n = node.size // (self.np_ * 6)
+        lib().oracle_limiter_replay(self.h, n, np.ascontiguousarray(node, np.int32),
+                                    np.ascontiguousarray(elem, np.int32))
 
     def etamax(self):
         """ETAMAX as modal2nodal has kept it so far (-99999 before the first call)."""

@@ -37,4 +37,5 @@ def load_case(name):
 GOLDEN_STATES = [("c1", 1), ("c1", 10), ("c1", 1000), ("c1", 8640), ("c1var", 200), ("delaunay4k", 100),
                  ("delaunay4k", 1000), ("quarter_annular", 100), ("quarter_annular", 2000),
                  ("wetdry", 50), ("wetdry", 500), ("mass_conservation", 1000), ("storm3k", 100),
-                 ("storm3k", 500), ("delaunay4k_nolim", 300)]
+                 ("storm3k", 500), ("delaunay4k_nolim", 300), ("tipot2k", 300), ("tipotsal2k", 200),
+                 ("c1eseries", 1000), ("c1fseries", 400)]

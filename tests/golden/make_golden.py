@@ -77,10 +77,11 @@ def generated_case(name: str, case, steps) -> None:
         case.params.nt = n
         with tempfile.TemporaryDirectory() as d:
             mesh.write_run_dir(d, case, dghot=1, dghotspool=n, nspoolge=n)
-            if name.startswith(("delaunay", "storm")) and n == steps[0]:
+            if name.startswith(("delaunay", "storm", "tipot", "c1eseries", "c1fseries")) and n == steps[0]:
                 keep = os.path.join(HERE, name)
                 os.makedirs(keep, exist_ok=True)
-                for f in ("fort.14", "fort.15", "fort.13", "fort.dg", "fort.22"):
+                for f in ("fort.14", "fort.15", "fort.13", "fort.dg", "fort.22", "fort.24", "fort.19",
+                          "fort.20"):
                     if os.path.exists(os.path.join(d, f)):
                         shutil.copy(os.path.join(d, f), keep)
             run_ref(d)
@@ -138,9 +139,80 @@ def maxele_case(name: str, case, nsteps: int) -> None:
         print(f"  {name}: maxele.63 after step {nsteps} written")
 
 
+# Tidal potential constituents (TPK, AMIGT rad/s, ETRF): the usual ADCIRC fort.15 values
+TIDAL_POTENTIAL = [("M2", 0.242334, 1.405189e-4, 0.693), ("S2", 0.112841, 1.454441e-4, 0.693),
+                   ("N2", 0.046398, 1.378797e-4, 0.693), ("K1", 0.141565, 7.292117e-5, 0.736),
+                   ("O1", 0.100514, 6.759774e-5, 0.695)]
+
+
+def tidal_potential_case(ntip: int, nsteps: int):
+    """A 2000-element spherical basin (1000 km x 600 km around 28 N, limiter and wetting/drying on)
+    under the five tidal potential constituents (two diurnal, three semidiurnal) besides its open
+    boundary tides; with ntip=2 also seeded self-attraction/load amplitudes and phases."""
+    case = mesh.delaunay_basin(n_elem=2000, seed=11, ics=2, nsteps=nsteps, depth_range=(5.0, 3000.0))
+    case.params.dramp = 0.05
+    p = case.params
+    rng = np.random.default_rng(12)
+    p.ntip = ntip
+    p.ntif = len(TIDAL_POTENTIAL)
+    p.tipotag = [t[0] for t in TIDAL_POTENTIAL]
+    p.tpk = np.array([t[1] for t in TIDAL_POTENTIAL])
+    p.amigt = np.array([t[2] for t in TIDAL_POTENTIAL])
+    p.etrf = np.array([t[3] for t in TIDAL_POTENTIAL])
+    p.fft = rng.uniform(0.9, 1.1, p.ntif)
+    p.facet = rng.uniform(0.0, 360.0, p.ntif) * fortio.DEG2RAD
+    p.pert = np.array([2.0 * fortio.PI / a for a in p.amigt])
+    if ntip == 2:
+        lat = case.sfea / fortio.DEG2RAD
+        p.saltamp = np.stack([0.01 + 0.02 * np.abs(np.sin(lat * (k + 1))) for k in range(p.ntif)])
+        p.saltpha = np.stack([(30.0 * k + 10.0 * np.cos(case.slam * 50.0)) for k in range(p.ntif)]) * fortio.DEG2RAD
+    return case
+
+
+def elev_series_case(nsteps: int):
+    """The config-1 basin with its open boundary driven by a fort.19 time series (NBFR = 0):
+    hourly records of a semidiurnal-like signal that varies along the boundary."""
+    case = mesh.structured_basin(nsteps=nsteps)
+    p = case.params
+    p.nbfr = 0
+    p.amig = p.ff = p.face = p.per = np.zeros(0)
+    p.emo = p.efa = np.zeros((0, 0))
+    neta = len(case.nbd())
+    t = np.arange(40) * 3600.0
+    along = np.linspace(0.0, 1.0, neta)
+    vals = 0.4 * np.sin(2 * np.pi * t[:, None] / 44712.0 + 0.3 * along[None, :]) + 0.05 * along[None, :]
+    case.elev_series = (3600.0, vals)
+    return case
+
+
+def flow_series_case(nsteps: int):
+    """The c1var basin (limiter, Coriolis, linear friction) with its flux side driven by a fort.20
+    time series (NFFR = 0): half-hourly records of an inflow that varies along the side."""
+    case = mesh.structured_basin(nx=24, ny=12, slopeflag=5, cori=1.0e-4, nolibf=0, manning=None,
+                                 flow_side=True, flow_amp=2.0, depth=8.0, nsteps=nsteps)
+    p = case.params
+    p.nffr = 0
+    p.famig = p.fff = p.fface = p.fper = np.zeros(0)
+    p.qnam = p.qnph = np.zeros((1, 0))
+    nflux = sum(len(n) for tp, n in case.land_segs if tp in fortio.FLOW_TYPES)
+    t = np.arange(30) * 1800.0
+    along = np.linspace(0.0, 1.0, nflux)
+    vals = -1.5 * np.sin(2 * np.pi * t[:, None] / 30000.0) * (1.0 + 0.2 * along[None, :])
+    case.flow_series = (1800.0, vals)
+    return case
+
+
 def delaunay4k():
     return mesh.delaunay_basin(n_elem=4000, seed=1, lx=6.0e4, ly=3.6e4, depth_range=(2.0, 30.0),
                                cfl=0.3)
+
+
+def forcing_cases() -> None:
+    """Tidal potential (NTIP = 1 and 2) and the non-periodic boundaries (fort.19, fort.20)."""
+    generated_case("tipot2k", tidal_potential_case(1, 300), [300])
+    generated_case("tipotsal2k", tidal_potential_case(2, 200), [200])
+    generated_case("c1eseries", elev_series_case(1000), [1000])
+    generated_case("c1fseries", flow_series_case(400), [400])
 
 
 def main(which=None) -> None:
@@ -151,6 +223,8 @@ def main(which=None) -> None:
             generated_case("c1", mesh.structured_basin(), [8640])
         if "maxele" in which:
             maxele_case("delaunay4k", delaunay4k(), 100)
+        if "forcing" in which:
+            forcing_cases()
         return
     generated_case("c1", mesh.structured_basin(), [1, 10, 1000, 8640])
     maxele_case("delaunay4k", delaunay4k(), 100)
@@ -167,6 +241,7 @@ def main(which=None) -> None:
                                                            slopeflag=0),
                    [300])
     generated_case("storm3k", storm_case(), [100, 500])
+    forcing_cases()
     reference_case("quarter_annular", [100, 2000])
     reference_case("wetdry", [50, 500])
     reference_case("mass_conservation", [1000])

@@ -92,6 +92,51 @@ def test_cos_near_multiples_of_half_pi(lib):
     assert math.isnan(lib.t_cos(float("nan")))
 
 
+def _pi_decimal() -> Decimal:
+    """pi to about 110 digits (Machin's formula in exact rationals)."""
+    def atan_inv(n, terms):
+        s, x = Fraction(0), Fraction(1, n)
+        t = x
+        for k in range(terms):
+            s += (t / (2 * k + 1)) * (-1) ** k
+            t *= x * x
+        return s
+    pi = 16 * atan_inv(5, 160) - 4 * atan_inv(239, 50)
+    return Decimal(pi.numerator) / Decimal(pi.denominator)
+
+
+def exact_cos_reduced(x: float) -> float:
+    """cos(x) for large |x|: reduced by pi/2 in 90-digit arithmetic first, then the series."""
+    xd, hp = _dec(x), _pi_decimal() / 2
+    k = int((xd / hp).to_integral_value())
+    r = xd - k * hp
+    cs, sn, t, j = Decimal(0), Decimal(0), Decimal(1), 0
+    while abs(t) > Decimal(10) ** -88:
+        if j % 2 == 0:
+            cs += t if j % 4 == 0 else -t
+        else:
+            sn += t if j % 4 == 1 else -t
+        j += 1
+        t = t * r / j
+    v = [cs, -sn, -cs, sn][k % 4]
+    return _nearest(v)
+
+
+def test_cos_near_multiples_of_half_pi_at_the_largest_accepted_arguments(lib):
+    """The reduction uses pi/2 in four parts with k*P3 an exact product: arguments next to
+    k*pi/2 keep their relative accuracy up to the largest k accepted (|x| < 2^16, k < 41722)."""
+    hp = _pi_decimal() / 2
+    for k in (41721, 40000, 33333, 2 ** 15, 12345, 1001):
+        x = float(k * hp)
+        assert abs(x) < 65536.0
+        for y in (x, math.nextafter(x, math.inf), math.nextafter(x, -math.inf), -x):
+            assert lib.t_cos(y) == exact_cos_reduced(y), (k, y)
+    rng = random.Random(5)
+    for _ in range(200):
+        y = rng.uniform(1000.0, 65535.0)
+        assert lib.t_cos(y) == exact_cos_reduced(y), y
+
+
 def test_pow_third_is_correctly_rounded(lib):
     rng = random.Random(11)
     args = [math.exp(rng.uniform(-9.0, 9.5)) for _ in range(1500)]

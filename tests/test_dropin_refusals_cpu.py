@@ -55,7 +55,7 @@ def test_read_case_refuses_fort_dg_options_it_does_not_run(tmp_path, key, value,
 
 
 @pytest.mark.parametrize("field,value,message", [
-    ("ntip", 1, "NTIP=1"), ("nws", 1, "NWS=1"), ("artdif", 1, "artdif=1"), ("padapt", 1, "padapt=1"),
+    ("ntip", 3, "NTIP=3"), ("nws", 1, "NWS=1"), ("artdif", 1, "artdif=1"), ("padapt", 1, "padapt=1"),
     ("pl", 2, "pl=2 differs from ph=1"), ("load_quadratic_fric", 1, "quadratic friction"),
 ])
 def test_library_refuses_parameters_before_touching_a_device(field, value, message):

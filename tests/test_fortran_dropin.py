@@ -80,18 +80,24 @@ def test_dropin_maxele_matches_reference(tmp_path):
     np.testing.assert_allclose(eta[-1], ref["eta"], rtol=0, atol=2e-8 * np.abs(ref["eta"]).max())
 
 
-def test_dropin_storm_matches_reference_fp64(tmp_path):
-    """NWS=8: the drop-in reads fort.22 itself (FORMAT 14) and hands the track to the library; the
-    DG hot start after 100 steps of the storm3k case matches dgswem_serial's."""
+@pytest.mark.parametrize("name", ["storm3k", "tipot2k", "tipotsal2k", "c1eseries", "c1fseries"])
+def test_dropin_forcing_matches_reference_fp64(tmp_path, name):
+    """Forcing the drop-in hands to the library from the host's modules: NWS=8 (it reads fort.22
+    itself, FORMAT 14), the tidal potential (NTIP=1, and NTIP=2 with fort.24's self-attraction
+    terms), a fort.19 elevation series (NBFR=0) and a fort.20 flux series (NFFR=0; the first two
+    records come from the host's QNIN1/QNIN2, the rest from unit 20).  The DG hot start at the end
+    matches dgswem_serial's."""
     d = str(tmp_path)
-    for f in ("fort.14", "fort.15", "fort.13", "fort.dg", "fort.22"):
-        shutil.copy(os.path.join(GOLDEN, "storm3k", f), d)
+    for f in ("fort.14", "fort.15", "fort.13", "fort.dg", "fort.22", "fort.24", "fort.19", "fort.20"):
+        if os.path.exists(os.path.join(GOLDEN, name, f)):
+            shutil.copy(os.path.join(GOLDEN, name, f), d)
     case = fortio.read_case(d)
-    assert case.params.nt == 100 and case.dg.dghot == 1 and case.dg.dghotspool == 100
-    print("dropin storm3k", flush=True)
+    steps = case.params.nt
+    assert case.dg.dghot == 1 and case.dg.dghotspool == steps
+    print(f"dropin {name} {steps} steps", flush=True)
     run(d)
     hs = fortio.read_dg_hotstart(d, case.dofh, case.ne)
-    ref = golden("storm3k_step100")
+    ref = golden(f"{name}_step{steps}")
     np.testing.assert_array_equal(hs["wdflg"], ref["wdflg"])
     for k in ("ze", "qx", "qy"):
         rel = np.linalg.norm(hs[k] - ref[k]) / np.linalg.norm(ref[k])

@@ -110,9 +110,22 @@ def test_torch_autograd_front_end(p1):
     case, prob, J, gn, gdp = p1
     m = torch.tensor(case.manning, dtype=torch.float64, requires_grad=True)
     d = torch.tensor(case.dp, dtype=torch.float64, requires_grad=True)
-    loss = adjoint.gauge_misfit(m, d, prob)
-    assert float(loss.detach()) == pytest.approx(J, rel=1e-13)
-    (2.0 * loss).backward()
+    # one forward-and-adjoint call serves both forward and backward (no second forward run)
+    calls = []
+    orig_g, orig_l = prob.gradient, prob.loss
+    prob.gradient = lambda *a, **k: (calls.append("gradient"), orig_g(*a, **k))[1]
+    prob.loss = lambda *a, **k: (calls.append("loss"), orig_l(*a, **k))[1]
+    try:
+        loss = adjoint.gauge_misfit(m, d, prob)
+        assert float(loss.detach()) == pytest.approx(J, rel=1e-13)
+        (2.0 * loss).backward()
+        assert calls == ["gradient"]
+        with torch.no_grad():
+            plain = adjoint.gauge_misfit(m, d, prob)
+        assert calls == ["gradient", "loss"]
+    finally:
+        prob.gradient, prob.loss = orig_g, orig_l
+    assert float(plain) == pytest.approx(J, rel=1e-13)
     np.testing.assert_array_equal(m.grad.numpy(), 2.0 * gn)
     np.testing.assert_array_equal(d.grad.numpy(), 2.0 * gdp)
 
@@ -242,3 +255,61 @@ def test_frozen_limiter_gradient_matches_central_differences(p, wetdry):
         fd = central_fd(prob, case, vn, vd, eps=1e-8)
         print(f"p={p} wetdry={wetdry}: adjoint {ad:.12e} fd {fd:.12e} rel {abs(fd - ad) / abs(ad):.2e}", flush=True)
         assert abs(fd - ad) <= TOL_LIMITER * abs(ad)
+
+
+def frozen_oracle_loss(case, prob, ng, seed, records):
+    """Loss of the oracle's forward model with the slope limiter's branches replayed from `records`
+    (the frozen-branch map the adjoint differentiates): a function of Manning n and depth."""
+    import copy
+    from oracle.oracle import Oracle, basis_at
+    elem, r, s = adjoint.seeded_gauges(case, ng, seed)
+    phi = np.stack([basis_at(case.dg.pl, r[g], s[g]) for g in range(len(elem))])
+
+    def loss(mn, dp):
+        c = copy.copy(case)
+        c.manning = mn
+        c.dp = dp
+        o = Oracle(c, threads=8)
+        o.set_state(*prob.state0)
+        o.limiter_replay(records)
+        J = 0.0
+        for k in range(prob.nsteps):
+            o.step(1)
+            eta = np.sum(o.state()[0][elem - 1] * phi, axis=1)
+            J += np.sum((eta - prob.data[k]) ** 2)
+        return prob.weight * J
+    return loss
+
+
+@pytest.mark.parametrize("p,wetdry", [(1, False), (1, True), (2, True)])
+def test_frozen_limiter_gradient_matches_frozen_branch_oracle(p, wetdry):
+    """The map the adjoint differentiates, evaluated on the CPU: the oracle's forward model with
+    every SLOPELIMITER5 branch (the clamped vertices and the bound-attaining elements, the signs,
+    counts and min choices of the redistribution passes) replayed from the unperturbed run.  Its
+    fourth-order central differences, in directions that move the bathymetry as well as Manning's
+    n, match the GPU gradient to 1e-6.  The step is 1e-5: the map is smooth but for the LLF wave
+    speed bound, a maximum that is continuous, whose kinks give the differences an O(step) error
+    (1e-4 relative at 1e-3, 4e-8 at 1e-5 on these runs)."""
+    from oracle.oracle import Oracle
+    nst = 40 if p == 1 else 60
+    case = limiter_case(p=p, wetdry=wetdry, nsteps=nst)
+    if p == 2:
+        case.params.dtdp = case.params.dtdp * 0.5
+    st = adjoint.hump_state(case, amplitude=0.8, width=0.15)
+    prob = adjoint.twin_problem(case, nsteps=nst, ng=40, seed=8, state0=st, perturb=0.5)
+    J, gn, gdp = prob.gradient(case.manning, case.dp)
+    o = Oracle(case, threads=8)
+    o.set_state(*prob.state0)
+    o.limiter_record(nst)
+    o.step(nst)
+    records = o.limiter_records()
+    assert np.mean((records[1] & 0x3F) != 0) > 0.01        # the limiter clamps on this run
+    loss = frozen_oracle_loss(case, prob, 40, 8, records)
+    assert loss(case.manning, case.dp) == pytest.approx(J, rel=1e-9)
+    for vn, vd in directions(case, 3, seed=13):
+        ad = float(gn @ vn + gdp @ vd)
+        fd = fd4(lambda t: loss(case.manning + t * vn, case.dp + t * vd), 1e-5)
+        rel = abs(fd - ad) / abs(ad)
+        print(f"p={p} wetdry={wetdry}: adjoint {ad:.12e} frozen-branch oracle fd {fd:.12e} rel {rel:.2e}", flush=True)
+        assert rel <= TOL
+    prob.solver.close()

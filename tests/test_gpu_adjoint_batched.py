@@ -8,7 +8,9 @@ bit for bit (the members are independent and every element's arithmetic is the s
 gradient matches fourth-order central differences of the batched GPU forward model (relative
 1e-6 in Manning-only directions, 1e-5 in directions that move the bathymetry, whose difference
 step must stay below the limiter's branch switches); member 0's gradient matches central
-differences of the CPU oracle in FP64 (the CPU reference gradient, relative 1e-6).
+differences of the CPU oracle in FP64 (the CPU reference gradient, relative 1e-6), and also
+in a direction that moves the bathymetry when the oracle replays the forward run's limiter
+branches (the frozen-branch map the adjoint differentiates, relative 1e-6).
 """
 import copy
 
@@ -105,3 +107,43 @@ def test_member_gradient_matches_cpu_oracle_differences(batch):
     print(f"oracle fd {fd:.12e} adjoint {ad:.12e}", flush=True)
     assert abs(fd - ad) <= TOL * abs(ad)
 
+
+def test_member_gradient_matches_frozen_branch_oracle(batch):
+    """Member 0 against the map the adjoint differentiates, on the CPU: the oracle's forward model
+    (Holland forcing included) with the slope limiter's branches replayed from the unperturbed
+    run, fourth-order central differences at step 1e-5 in a direction that moves the bathymetry
+    as well as Manning's n: relative 1e-6."""
+    cases, eprob, probs, mn, dp, J, gn, gdp = batch
+    c0, p0 = cases[0], probs[0]
+    elem, r, s = adjoint.seeded_gauges(c0, 10, 70)
+    phi = np.stack([basis_at(c0.dg.pl, r[g], s[g]) for g in range(len(elem))])
+    o = Oracle(c0, threads=8)
+    o.set_state(*p0.state0)
+    o.limiter_record(NSTEPS)
+    o.step(NSTEPS)
+    records = o.limiter_records()
+    assert np.mean((records[1] & 0x3F) != 0) > 0.001
+
+    def frozen_loss(m, d):
+        c = copy.copy(c0)
+        c.manning = m
+        c.dp = d
+        oc = Oracle(c, threads=8)
+        oc.set_state(*p0.state0)
+        oc.limiter_replay(records)
+        Jo = 0.0
+        for k in range(NSTEPS):
+            oc.step(1)
+            eta = np.sum(oc.state()[0][elem - 1] * phi, axis=1)
+            Jo += np.sum((eta - p0.data[k]) ** 2)
+        return p0.weight * Jo
+
+    rng = np.random.default_rng(12)
+    n0 = c0.np_
+    vn = rng.standard_normal(n0) * c0.manning
+    vd = rng.standard_normal(n0) * c0.dp
+    ad = float(gn[:n0] @ vn + gdp[:n0] @ vd)
+    fd = fd4(lambda t: frozen_loss(c0.manning + t * vn, c0.dp + t * vd), 1e-5)
+    rel = abs(fd - ad) / abs(ad)
+    print(f"frozen-branch oracle fd {fd:.12e} adjoint {ad:.12e} rel {rel:.2e}", flush=True)
+    assert rel <= TOL

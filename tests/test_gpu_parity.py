@@ -88,8 +88,9 @@ def test_modal2nodal_matches_oracle(name):
 def test_device_nodal_values_and_etamax_match_oracle(name):
     """modal2nodal kept on the device every step (dg_gpu_nodal_update) with ETAMAX raised there,
     copied once at the end (dg_gpu_get_nodal), against the oracle's modal2nodal run every step.
-    Both visit each node's elements in ascending element order and raise ETAMAX after every wet
-    contribution (so partial sums count, as in src/write_results.F:164)."""
+    Both visit each node's elements in the reference's ELETAB order and raise ETAMAX after every
+    wet contribution (so partial sums count, as in src/write_results.F:164).  The wetdry case's
+    states agree bit for bit, so its nodal values and ETAMAX must too."""
     case = load_case(name)
     steps = 60
     g = gpu.GpuSolver(case)
@@ -104,6 +105,9 @@ def test_device_nodal_values_and_etamax_match_oracle(name):
     oe, ou, ov, onc = o.nodal()      # one more call: the values are those of the last step
     oem = o.etamax()
     np.testing.assert_array_equal(gnc, onc)
+    if name == "wetdry":
+        for a, b in ((ge, oe), (gu, ou), (gv, ov), (gem, oem)):
+            np.testing.assert_array_equal(a, b)
     for a, b in ((ge, oe), (gu, ou), (gv, ov)):
         np.testing.assert_allclose(a, b, rtol=1e-11, atol=1e-13 * max(np.abs(b).max(), 1.0))
     # the oracle's extra call repeats the last step's partial sums, which cannot raise ETAMAX again
