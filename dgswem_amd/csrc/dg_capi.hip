@@ -29,6 +29,10 @@
 
 using namespace dg;
 
+// Working memory of dg_gpu_mass_noalloc (see scratch_ptr below).
+constexpr int kScratchDoubles = 65536;                   // 512 KiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 namespace {
 
 thread_local std::string g_err;
@@ -1482,6 +1486,28 @@ void unpack_cols(int nc, int nedge, const double* src, std::vector<double>& dst)
                 dst[((size_t)L * nc + j) * 2 + k] = src[(size_t)j + (size_t)nc * (L + (size_t)nedge * k)];
 }
 
+// ---- dg_gpu_mass_noalloc's working memory.  A file-scope device array: every GPU the library runs
+// on holds its own copy until the process ends, and every call on that GPU shares it (calls that
+// overlap on one GPU race).  Host code never passes the array's name to a kernel or a copy (on the
+// host that name is a stand-in, not a device address); it uses the address hipGetSymbolAddress
+// returns for the current device, looked up once per device.
+constexpr int kScratchMaxBlocks = kScratchDoubles - 1;   // partial sums; the last slot holds the total
+constexpr int kMaxDevices = 64;
+double* g_scratch[kMaxDevices] = {};
+
+int scratch_ptr(int dev, double** out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail("device index out of range");
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_scratch[dev]) {
+        void* ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail("hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch[dev] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch[dev];
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1886,6 +1912,34 @@ int dg_gpu_mass(int h, double* mass) {
     double m = 0.0;
     for (double v : part) m += v;
     *mass = m;
+    return 0;
+}
+
+int dg_gpu_mass_noalloc(int h, double* mass) {
+    Ctx* c = get(h);
+    if (!c) return 1;
+    if (!mass) return fail("null output pointer");
+    const int nb = (c->ne + 255) / 256;
+    if (nb > kScratchMaxBlocks)
+        return fail("dg_gpu_mass_noalloc: " + std::to_string(c->ne) + " elements exceed its limit of " +
+                    std::to_string((long long)kScratchMaxBlocks * 256));
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev != c->dev) return fail("dg_gpu_mass_noalloc: the handle's device is not the current device");
+    double* ws = nullptr;
+    if (scratch_ptr(dev, &ws)) return 1;
+    ElemData E = elem_data(*c);
+    if (c->prm.p == 1)
+        hipLaunchKernelGGL((mass_kernel<1>), dim3(nb), dim3(256), 0, c->stream, E, (const double*)c->u[c->ucur].p, ws);
+    else
+        hipLaunchKernelGGL((mass_kernel<2>), dim3(nb), dim3(256), 0, c->stream, E, (const double*)c->u[c->ucur].p, ws);
+    HIPCHK(hipGetLastError());
+    // the partial sums added in block order, as dg_gpu_mass adds them on the host
+    hipLaunchKernelGGL(sum_in_order_kernel, dim3(1), dim3(64), 0, c->stream, (const double*)ws, nb,
+                       ws + kScratchMaxBlocks);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(mass, ws + kScratchMaxBlocks, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
 

@@ -1076,4 +1076,13 @@ __global__ void __launch_bounds__(256) mass_kernel(ElemData E, const double* __r
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
+// out = ((0 + v[0]) + v[1]) + ... + v[n-1], the left-to-right order of a host loop.
+__global__ void __launch_bounds__(64) sum_in_order_kernel(const double* __restrict__ v, int n,
+                                                          double* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += v[i];
+    *out = s;
+}
+
 }  // namespace dg

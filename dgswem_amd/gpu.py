@@ -28,7 +28,7 @@ K_STAGE, K_MINMAX, K_LIMIT, K_M2N, K_MET = 0, 1, 2, 3, 4
 EXPORTED = ["dg_create_edge_data", "dg_gpu_init", "dg_gpu_set_tide", "dg_gpu_set_flow",
             "dg_gpu_set_state", "dg_gpu_get_state", "dg_gpu_step", "dg_gpu_synchronize",
             "dg_gpu_modal2nodal", "dg_gpu_nodal_update", "dg_gpu_get_nodal",
-            "dg_gpu_set_etamax", "dg_gpu_set_eletab", "dg_gpu_mass", "dg_gpu_profile", "dg_gpu_kernel_stats",
+            "dg_gpu_set_etamax", "dg_gpu_set_eletab", "dg_gpu_mass", "dg_gpu_mass_noalloc", "dg_gpu_profile", "dg_gpu_kernel_stats",
             "dg_gpu_set_holland", "dg_holland_storm_at", "dg_gpu_set_tidal_potential",
             "dg_gpu_set_elev_series", "dg_gpu_set_flow_series", "dg_gpu_set_manning", "dg_gpu_set_depth",
             "dg_adj_set_gauges", "dg_adj_gauges", "dg_adj_gradient", "dg_ens_create", "dg_ens_layout", "dg_gpu_use_graph", "dg_gpu_autotune",
@@ -78,6 +78,7 @@ def load_library(path: str = LIB_PATH):
     L.dg_gpu_set_etamax.argtypes = [C.c_int, _dp]
     L.dg_gpu_set_eletab.argtypes = [C.c_int, C.c_int, _ip, _ip]
     L.dg_gpu_mass.argtypes = [C.c_int, C.POINTER(C.c_double)]
+    L.dg_gpu_mass_noalloc.argtypes = [C.c_int, C.POINTER(C.c_double)]
     L.dg_gpu_profile.argtypes = [C.c_int, C.c_int]
     L.dg_gpu_kernel_stats.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_longlong), C.c_char_p, C.c_int]

@@ -171,6 +171,16 @@ int dg_gpu_set_eletab(int handle, int mnp, const int* el_count, const int* eleta
 /* Total water volume sum_e A_e*(ze_1 + hb_1) of the stage-1 state (mode 1 is the cell mean). */
 int dg_gpu_mass(int handle, double* mass);
 
+/* The same total as dg_gpu_mass, bit for bit, from an entry point that allocates nothing: no
+ * hipMalloc, hipFree or allocating library routine.  No counterpart in the reference; it is for
+ * callers that cannot allocate device memory (tight serving loops, harnesses that forbid it).
+ * Its partial sums and total live in a fixed 512 KiB device array that the library holds on each
+ * GPU until the process ends, so the mesh may have at most 65535*256 = 16,776,960 elements (more
+ * is refused with an error).  Every call on one GPU shares that array: calls that overlap on one
+ * GPU (two host threads, two handles) race and must be serialised by the caller.  The handle's
+ * device must be the current device. */
+int dg_gpu_mass_noalloc(int handle, double* mass);
+
 /* Kernel timing with HIP events on the handle's stream.  With enable=n>0 every kernel launch of
  * every n-th step of dg_gpu_step is bracketed by events (n=1: all steps; a larger n samples a
  * long run with little event overhead); enable=0 switches timing off.  dg_gpu_kernel_stats
