@@ -144,6 +144,14 @@ struct Ctx {
     std::map<int, Staging> stg;
     int max_lds = 65536;                         // LDS bytes per block the device allows
     DevBuf<double> nmm, outbuf, partial;
+    // slope limiter: the stage kernel's copy of the updated cell means [3][ne] (the limiter's node
+    // bounds read them there, never from the state it limits in place) and the per-tile node lists
+    DevBuf<double> cmean;
+    struct LimBufs {
+        DevBuf<int> vhoff, vhidx, qoff, qinc_off, elq;
+        DevBuf<uint16_t> qinc;
+        int nsl = 0, nql = 0;
+    } lim;
     DevBuf<int> nodecode;
     // modal2nodal: node rows in the host's ELETAB order (default: the node->element CSR) and the
     // running maximum ETAMAX, when the host has handed it over
@@ -578,6 +586,71 @@ int upload_met(Ctx& c) {
     return 0;
 }
 
+// The slope limiter's tiles of LIM_NT elements (LimTiles): each tile's vertex halo (ascending),
+// its nodes in order of first appearance, every node's incident elements as staged rows in the
+// node->element order `n2e` (entries elem<<2 | vertex), and each element's tile-local nodes.
+int build_limtiles(Ctx& c, const std::vector<int>& nm, const std::vector<int>& off, const std::vector<int>& n2e) {
+    const int ne = c.ne, nt = LIM_NT, ntiles = (ne + nt - 1) / nt;
+    std::vector<int> vhoff(ntiles + 1, 0), vhidx, qoff(ntiles + 1, 0), qinc_off(1, 0), elq(ne, 0), halo, nodes;
+    std::vector<uint16_t> qinc;
+    std::vector<int> qlocal(c.np, -1);
+    int nsl = nt, nql = 1;
+    for (int t = 0; t < ntiles; t++) {
+        const int b = t * nt, en = std::min(ne, b + nt);
+        nodes.clear();
+        halo.clear();
+        for (int e = b; e < en; e++) {
+            int packed = 0;
+            for (int v = 0; v < 3; v++) {
+                const int n = nm[(size_t)v * ne + e];
+                if (qlocal[n] < 0) { qlocal[n] = (int)nodes.size(); nodes.push_back(n); }
+                packed |= qlocal[n] << (10 * v);
+            }
+            elq[e] = packed;
+        }
+        for (int n : nodes)
+            for (int j = off[n]; j < off[n + 1]; j++) {
+                const int g = n2e[j] >> 2;
+                if (g < b || g >= en) halo.push_back(g);
+            }
+        std::sort(halo.begin(), halo.end());
+        halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+        if (nt + (int)halo.size() > 65535 || (int)nodes.size() > 1023)
+            return fail("slope limiter tiles: a tile touches too many nodes or elements");
+        for (int n : nodes) {
+            for (int j = off[n]; j < off[n + 1]; j++) {
+                const int g = n2e[j] >> 2;
+                const int r = (g >= b && g < en) ? g - b
+                                                 : nt + (int)(std::lower_bound(halo.begin(), halo.end(), g) - halo.begin());
+                qinc.push_back((uint16_t)r);
+            }
+            qinc_off.push_back((int)qinc.size());
+            qlocal[n] = -1;
+        }
+        nsl = std::max(nsl, nt + (int)halo.size());
+        nql = std::max(nql, (int)nodes.size());
+        vhidx.insert(vhidx.end(), halo.begin(), halo.end());
+        vhoff[t + 1] = (int)vhidx.size();
+        qoff[t + 1] = qoff[t] + (int)nodes.size();
+    }
+    if ((size_t)8 * (3 * nsl + 6 * nql) > (size_t)c.max_lds)
+        return fail("slope limiter tiles: a tile needs more LDS than a block may use");
+    if (vhidx.empty()) vhidx.push_back(0);
+    if (qinc.empty()) qinc.push_back(0);
+    Ctx::LimBufs& L = c.lim;
+    L.nsl = nsl;
+    L.nql = nql;
+    HIPCHK(L.vhoff.upload(vhoff.data(), vhoff.size(), c.stream));
+    HIPCHK(L.vhidx.upload(vhidx.data(), vhidx.size(), c.stream));
+    HIPCHK(L.qoff.upload(qoff.data(), qoff.size(), c.stream));
+    HIPCHK(L.qinc_off.upload(qinc_off.data(), qinc_off.size(), c.stream));
+    HIPCHK(L.qinc.upload(qinc.data(), qinc.size(), c.stream));
+    HIPCHK(L.elq.upload(elq.data(), elq.size(), c.stream));
+    HIPCHK(c.cmean.alloc((size_t)3 * ne));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return 0;
+}
+
 // Concatenate the members and upload everything except the state.
 int upload_mesh(Ctx& c) {
     const int S = c.S;
@@ -672,6 +745,7 @@ int upload_mesh(Ctx& c) {
         HIPCHK(hipStreamSynchronize(c.stream));
     }
     HIPCHK(c.nmm.alloc((size_t)6 * np));
+    if (c.prm.slopeflag != 0 && build_limtiles(c, nm, off, n2e)) return 1;
     HIPCHK(c.outbuf.alloc((size_t)3 * np));
     HIPCHK(c.nodecode.alloc(np));
     HIPCHK(c.partial.alloc((ne + 255) / 256));
@@ -1100,28 +1174,29 @@ StageArgs stage_args(Ctx& c, int irk, const int slot[], bool captured = false) {
     a.irk = irk;
     a.wd_active = c.wd_active;
     a.tip = c.tip ? c.tip2.p + (size_t)irk * c.np : nullptr;
+    a.cmout = c.prm.slopeflag != 0 ? c.cmean.p : nullptr;
     return a;
 }
 
-// SLOPELIMITER5 (+ WETDRY) on the stage output u: node min/max, then the element pass.
+// SLOPELIMITER5 (+ WETDRY) on the stage output u, in place; the stage kernel has left u's cell
+// means in c.cmean.
 template <int P>
 int launch_limiter(Ctx& c, double* u, const ElemData& ed, bool capturing) {
-    const int ne = c.ne, np = c.np, B = 256;
-    const int ge = (ne + B - 1) / B, gn = (np + B - 1) / B;
-    int ev = prof_begin(c, K_MINMAX, capturing);
-    hipLaunchKernelGGL((node_minmax_kernel<P>), dim3(gn), dim3(B), 0, c.stream, np, ne, (const int*)c.n2e_off.p,
-                       (const int*)c.n2e.p, (const double*)u, c.nmm.p);
-    prof_end(c, ev);
-    HIPCHK(hipGetLastError());
-    ev = prof_begin(c, K_LIMIT, capturing);
+    const int ne = c.ne;
+    const int ge = (ne + LIM_NT - 1) / LIM_NT;
+    if (!c.lim.elq.p || !c.cmean.p) return fail("internal: slope limiter tiles not built");
+    LimTiles lt{c.lim.vhoff.p, c.lim.vhidx.p, c.lim.qoff.p, c.lim.qinc_off.p, c.lim.qinc.p, c.lim.elq.p,
+                c.lim.nsl, c.lim.nql};
+    const size_t lds = (size_t)8 * (3 * lt.nsl + 6 * lt.nql);
+    int ev = prof_begin(c, K_LIMIT, capturing);
     if (c.prm.nolifa >= 2) {
-        hipLaunchKernelGGL((limit_wetdry_kernel<P, true, true>), dim3(ge), dim3(B), 0, c.stream, np, ed, u,
-                           (const double*)c.nmm.p, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p,
+        hipLaunchKernelGGL((limit_wetdry_kernel<P, true>), dim3(ge), dim3(LIM_NT), lds, c.stream, ed, u,
+                           (const double*)c.cmean.p, lt, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p,
                            c.wd_active);
         c.wdcur = 1 - c.wdcur;
     } else {
-        hipLaunchKernelGGL((limit_wetdry_kernel<P, true, false>), dim3(ge), dim3(B), 0, c.stream, np, ed, u,
-                           (const double*)c.nmm.p, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p,
+        hipLaunchKernelGGL((limit_wetdry_kernel<P, false>), dim3(ge), dim3(LIM_NT), lds, c.stream, ed, u,
+                           (const double*)c.cmean.p, lt, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p,
                            (int*)nullptr);
     }
     prof_end(c, ev);
@@ -1214,6 +1289,7 @@ int launch_step_explicit(Ctx& c, double* const* st, double* const* wsave = nullp
         a.irk = irk;
         a.wd_active = c.wd_active;
         a.tip = c.tip ? c.tip2.p + (size_t)irk * c.np : nullptr;
+        a.cmout = lim ? c.cmean.p : nullptr;
         if (launch_stage_nt<P>(c, a, sph, fuse, c.block)) return 1;
         HIPCHK(hipGetLastError());
         if (fuse) c.wdcur = 1 - c.wdcur;

@@ -149,6 +149,22 @@ struct Met {
     double* __restrict__ prn;           // [np] surface pressure (m of water)
 };
 
+// Block-local node bounds of the slope limiter.  Tile t (elements [t*NT, t*NT+NT)) stages the
+// cell means of its own elements (rows 0..NT-1) and of its vertex halo -- the elements outside
+// the tile that share a node with it, vhidx[vhoff[t]..vhoff[t+1]) (rows NT..) -- and forms the
+// bounds of its nodes qnode[qoff[t]..qoff[t+1]) from their incident elements, listed in the
+// node->element order as staged rows qinc[qinc_off[q]..qinc_off[q+1]).  elq packs each element's
+// three tile-local node numbers (10 bits each).
+struct LimTiles {
+    const int* __restrict__ vhoff;
+    const int* __restrict__ vhidx;
+    const int* __restrict__ qoff;
+    const int* __restrict__ qinc_off;
+    const uint16_t* __restrict__ qinc;
+    const int* __restrict__ elq;
+    int nsl, nql;                       // LDS rows: staged elements, nodes (largest tile)
+};
+
 // Local Lax-Friedrichs flux (src/numerical_flux.F:750-1022), operation order kept.
 __device__ __forceinline__ void llf_flux(const Phys& P, double ze_in, double ze_ex, double qx_in,
                                          double qx_ex, double qy_in, double qy_ex, double hb_in,

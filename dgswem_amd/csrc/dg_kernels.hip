@@ -59,6 +59,7 @@ struct StageArgs {
     int ns;                              // staged elements per block (LDS row length)
     int* __restrict__ wd_active;         // set to 1 when WETDRY changes an element, or nullptr
     const double* __restrict__ tip;      // TIP2 of this stage at the nodes (tidal potential), or nullptr
+    double* __restrict__ cmout;          // copy of the output cell means [3][ne] for the limiter, or nullptr
 };
 
 // ------------------------------------------------------------------------------------------
@@ -712,6 +713,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
     for (int f = 0; f < 3; f++)
 #pragma unroll
         for (int k = 0; k < DOF; k++) a.uout[(size_t)(f * DOF + k) * ne + e] = W[f][k];
+    if (a.cmout) {
+#pragma unroll
+        for (int f = 0; f < 3; f++) a.cmout[(size_t)f * ne + e] = W[f][0];
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -847,50 +852,112 @@ __global__ void __launch_bounds__(256) node_minmax_kernel(int np, int ne, const 
 }
 
 // ------------------------------------------------------------------------------------------
-// SLOPELIMITER5 part 2 (src/slopelimiter.F:3434-3676) then WETDRY (src/wetdry.F), per element.
+// SLOPELIMITER5 (src/slopelimiter.F:3290-3676) then WETDRY (src/wetdry.F), one thread per element.
+// The node bounds of part 1 are formed per tile in LDS (LimTiles): the tile stages the updated cell
+// means of its elements and of its vertex halo, each node of the tile takes the min/max over its
+// incident elements in the node->element order (as node_minmax_kernel does, so the bounds are the
+// same bit for bit), and each element then limits its modes 2-3 against the bounds of its nodes.
+// The cell means come from the stage kernel's copy `cmean` [3][ne]: WETDRY rewrites some cell
+// means of u in place, which other tiles' halos must not see.  Blocks of LIM_NT threads; the
+// launch passes 8*(3*nsl + 6*nql) bytes of dynamic LDS.
 // ------------------------------------------------------------------------------------------
-template <int P, bool LIMIT, bool WETDRY>
-__global__ void __launch_bounds__(256) limit_wetdry_kernel(int np, ElemData E, double* __restrict__ u,
-                                                           const double* __restrict__ mm,
-                                                           const uint8_t* __restrict__ wd_in,
-                                                           uint8_t* __restrict__ wd_out,
-                                                           int* __restrict__ wd_active) {
+constexpr int LIM_NT = 256;
+
+template <int P, bool WETDRY>
+__global__ void __launch_bounds__(LIM_NT) limit_wetdry_kernel(ElemData E, double* __restrict__ u,
+                                                              const double* __restrict__ cmean, LimTiles lt,
+                                                              const uint8_t* __restrict__ wd_in,
+                                                              uint8_t* __restrict__ wd_out,
+                                                              int* __restrict__ wd_active) {
     constexpr int DOF = (P + 1) * (P + 2) / 2;
     const DevTables& T = tab<P>();
     const int ne = E.ne;
-    const int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (e >= ne) return;
+    const int tid = threadIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int e = tile * LIM_NT + tid;
+    extern __shared__ double lds[];
+    const int nsl = lt.nsl, nql = lt.nql;
+    double* const cm = lds;                  // [3][nsl] staged cell means
+    double* const bnd = lds + 3 * nsl;       // [6][nql] zmin zmax qxmin qxmax qymin qymax
+    // the element's own loads first, so that their latency overlaps the staging below (lanes past
+    // the last element read element ne-1 and exit after the second barrier)
+    const int ec = e < ne ? e : ne - 1;
     double U[3][DOF];
 #pragma unroll
     for (int f = 0; f < 3; f++)
 #pragma unroll
-        for (int k = 0; k < DOF; k++) U[f][k] = u[(size_t)(f * DOF + k) * ne + e];
-    // the limiter keeps the cell means; wetting and drying changes them only for some elements,
-    // so a cell mean is written back only when its bits changed
-    const double c0in[3] = {U[0][0], U[1][0], U[2][0]};
-    // every load the element needs is issued here, before any arithmetic, so that the latencies
-    // overlap: the node bounds of all three fields and the wet/dry inputs
-    double bmin[3][3], bmax[3][3];
-    if (LIMIT) {
-        const int n0 = E.nm[e], n1 = E.nm[(size_t)ne + e], n2 = E.nm[2 * (size_t)ne + e];
-#pragma unroll
-        for (int iv = 0; iv < 3; iv++) {
-            const double* mn = mm + (size_t)(2 * iv) * np;
-            const double* mx = mm + (size_t)(2 * iv + 1) * np;
-            bmin[iv][0] = mn[n0]; bmin[iv][1] = mn[n1]; bmin[iv][2] = mn[n2];
-            bmax[iv][0] = mx[n0]; bmax[iv][1] = mx[n1]; bmax[iv][2] = mx[n2];
-        }
-    }
+        for (int k = 1; k < DOF; k++) U[f][k] = u[(size_t)(f * DOF + k) * ne + ec];
+    const int lq = lt.elq[ec];
     double D[3] = {0.0, 0.0, 0.0}, h0 = 0.0;
     int wdi = 0;
     if (WETDRY) {
 #pragma unroll
-        for (int v = 0; v < 3; v++) D[v] = E.dpv[(size_t)v * ne + e];
-        const int sid = E.sid ? (int)E.sid[e] : 0;
+        for (int v = 0; v < 3; v++) D[v] = E.dpv[(size_t)v * ne + ec];
+        const int sid = E.sid ? (int)E.sid[ec] : 0;
         h0 = E.scen[sid].h0;
-        wdi = wd_in[e];
+        wdi = wd_in[ec];
     }
-    if (LIMIT) {
+    {
+        // the tile's cell means and its vertex halo's, two entries per lane per pass with all their
+        // loads issued before any is stored
+        const int h0t = lt.vhoff[tile], nh = lt.vhoff[tile + 1] - h0t;
+        for (int j = tid; j < LIM_NT + nh; j += 2 * LIM_NT) {
+            const int j2 = j + LIM_NT;
+            const bool has2 = j2 < LIM_NT + nh;
+            int g = j < LIM_NT ? tile * LIM_NT + j : lt.vhidx[h0t + j - LIM_NT];
+            int g2 = has2 ? lt.vhidx[h0t + j2 - LIM_NT] : ne;
+            const bool ok = g < ne, ok2 = g2 < ne;
+            g = ok ? g : 0;
+            g2 = ok2 ? g2 : 0;
+            const double z = cmean[g], x = cmean[(size_t)ne + g], y = cmean[(size_t)2 * ne + g];
+            const double z2 = cmean[g2], x2 = cmean[(size_t)ne + g2], y2 = cmean[(size_t)2 * ne + g2];
+            if (ok) { cm[j] = z; cm[nsl + j] = x; cm[2 * nsl + j] = y; }
+            if (ok2) { cm[j2] = z2; cm[nsl + j2] = x2; cm[2 * nsl + j2] = y2; }
+        }
+    }
+    __syncthreads();
+    {
+        const int q0 = lt.qoff[tile], nq = lt.qoff[tile + 1] - q0;
+        for (int q = tid; q < nq; q += LIM_NT) {
+            double zmin = 99999., zmax = -99999., xmin = 99999., xmax = -99999., ymin = 99999., ymax = -99999.;
+            const int i1 = lt.qinc_off[q0 + q + 1];
+            // up to eight incidences per pass, their rows loaded together; past the node's last
+            // incidence the pass repeats it (min and max are idempotent, so a repeat changes nothing)
+            constexpr int B = 8;
+            for (int i = lt.qinc_off[q0 + q]; i < i1; i += B) {
+                int r[B];
+#pragma unroll
+                for (int b = 0; b < B; b++) r[b] = lt.qinc[min(i + b, i1 - 1)];
+#pragma unroll
+                for (int b = 0; b < B; b++) {
+                    const double z = cm[r[b]], x = cm[nsl + r[b]], y = cm[2 * nsl + r[b]];
+                    zmin = fmin(zmin, z); xmin = fmin(xmin, x); ymin = fmin(ymin, y);
+                    zmax = fmax(zmax, z); xmax = fmax(xmax, x); ymax = fmax(ymax, y);
+                }
+            }
+            bnd[q] = zmin; bnd[nql + q] = zmax; bnd[2 * nql + q] = xmin;
+            bnd[3 * nql + q] = xmax; bnd[4 * nql + q] = ymin; bnd[5 * nql + q] = ymax;
+        }
+    }
+    __syncthreads();
+    if (e >= ne) return;
+#pragma unroll
+    for (int f = 0; f < 3; f++) U[f][0] = cm[f * nsl + tid];
+    // the limiter keeps the cell means; wetting and drying changes them only for some elements,
+    // so a cell mean is written back only when its bits changed
+    const double c0in[3] = {U[0][0], U[1][0], U[2][0]};
+    double bmin[3][3], bmax[3][3];
+    {
+        const int q[3] = {lq & 1023, (lq >> 10) & 1023, (lq >> 20) & 1023};
+#pragma unroll
+        for (int iv = 0; iv < 3; iv++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                bmin[iv][c] = bnd[(2 * iv) * nql + q[c]];
+                bmax[iv][c] = bnd[(2 * iv + 1) * nql + q[c]];
+            }
+    }
+    {
 #pragma unroll
         for (int iv = 0; iv < 3; iv++) {
             const double* vmin = bmin[iv];
