@@ -8,8 +8,13 @@ no transcendental function is on the path, so the GPU state must equal the oracl
 * a basin whose every node lies above the datum with wetting/drying on: every element dry for
   the whole run (no internal flux at all);
 * a basin that dries and floods: land rising across the basin under a moving hump;
+* the same kind of basin with its elements and nodes numbered at random: every tile's neighbours
+  lie all over the mesh (the stage kernel's halos are as large as they get, and a 256-element
+  limiter tile touches up to 692 nodes and 2220 halo elements: 90 KB of LDS);
 each at p=1 and p=2 and at every stage block size.
 """
+import copy
+
 import numpy as np
 import pytest
 
@@ -118,3 +123,29 @@ def test_drying_and_flooding(p):
     ref = compare(c, 200, st, BLOCKS[p])
     np.testing.assert_array_equal(ref[3], prev)
     assert np.isfinite(ref[0]).all()
+
+
+def renumbered(c, seed):
+    """The case with its elements and nodes in a seeded random order."""
+    rng = np.random.default_rng(seed)
+    r = copy.deepcopy(c)
+    pn = rng.permutation(c.np_)                      # new node k is old node pn[k]
+    inv = np.empty_like(pn)
+    inv[pn] = np.arange(c.np_)
+    for f in ("x", "y", "slam", "sfea", "dp", "sfac", "corif"):
+        setattr(r, f, getattr(c, f)[pn])
+    if c.manning is not None:
+        r.manning = c.manning[pn]
+    r.nm = (inv[c.nm - 1] + 1).astype(np.int32)[rng.permutation(c.ne)]
+    r.open_segs = [(inv[s_ - 1] + 1).astype(np.int32) for s_ in c.open_segs]
+    r.land_segs = [(t, (inv[s_ - 1] + 1).astype(np.int32)) for t, s_ in c.land_segs]
+    return r
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_random_numbering(p):
+    c = renumbered(closed(60, 40, p, nolifa=2, slopeflag=5), seed=5)
+    c.dp = 4.0 - 0.0001 * c.x                      # 4 m deep in the west to -2 m in the east
+    print(f"random numbering p={p}: {c.ne} elements", flush=True)
+    ref = compare(c, 40, wetdry_state(c, 1.0), BLOCKS[p])
+    assert np.isfinite(ref[0]).all() and (ref[3] == 0).any()
