@@ -542,11 +542,10 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
 #pragma unroll
             for (int v = 0; v < 3; v++) {
                 const int n = E.nm[(size_t)v * ne + e];
-                const double* r = a.mt.nf + (size_t)n * MET_NF;
-                const double cd = r[NF_CD], wm = r[NF_WM];
-                mw[v] = ramp * 0.001293 * cd * r[NF_WX] * wm;
-                mw[3 + v] = ramp * 0.001293 * cd * r[NF_WY] * wm;
-                const double pr = ramp * r[NF_PR];
+                const double cd = a.mt.cd[n], wm = a.mt.wm[n];
+                mw[v] = ramp * 0.001293 * cd * a.mt.wx[n] * wm;
+                mw[3 + v] = ramp * 0.001293 * cd * a.mt.wy[n] * wm;
+                const double pr = ramp * a.mt.prn[n];
                 pdx += pr * (drpsi[v] * drdx + dspsi[v] * dsdx);
                 pdy += pr * (drpsi[v] * drdy + dspsi[v] * dsdy);
             }

@@ -130,7 +130,7 @@ struct Ctx {
     // mesh
     DevBuf<uint16_t> sid;
     DevBuf<Scen> scen;
-    DevBuf<double> xv, yv, dpv, nbar, cori_el, sfacq, sfaced;
+    DevBuf<double> xv, yv, dpv, nbar, cori_el, sfacq, sfaced, sfs;
     DevBuf<int> sinfo, snbr, nm, n2e_off, n2e;
     std::vector<int> h_sinfo, h_snbr;            // host copies (staging lists)
     bool has_cori_el = false;
@@ -182,7 +182,7 @@ struct Ctx {
     DevBuf<int> tna;
     // NWS=8 meteorological forcing
     bool met = false;
-    DevBuf<double> ramp, mrow, mxc, myc, mcosy, mcor, mnf;
+    DevBuf<double> ramp, mrow, mxc, myc, mcosy, mcor, mwx, mwy, mwm, mcd, mprn;
     DevBuf<uint16_t> nsid;
     int next_it = 1;                             // IT of the next step (for the autotuner's tables)
     // set while the adjoint's forward sweeps run: WETDRY raises it when it changes an element
@@ -292,6 +292,7 @@ ElemData elem_data(Ctx& c) {
     E.cori_el = c.has_cori_el ? c.cori_el.p : nullptr;
     E.sfacq = c.prm.ics == 2 ? c.sfacq.p : nullptr;
     E.sfaced = c.prm.ics == 2 ? c.sfaced.p : nullptr;
+    E.sfs = c.prm.ics == 2 ? c.sfs.p : nullptr;
     E.sinfo = c.sinfo.p; E.snbr = c.snbr.p; E.nm = c.nm.p;
     return E;
 }
@@ -331,7 +332,7 @@ Met met_args(Ctx& c) {
     m.row = c.mrow.p;
     m.nsid = c.S > 1 ? c.nsid.p : nullptr;
     m.xc = c.mxc.p; m.yc = c.myc.p; m.cosy = c.mcosy.p; m.cor = c.mcor.p;
-    m.nf = c.mnf.p;
+    m.wx = c.mwx.p; m.wy = c.mwy.p; m.wm = c.mwm.p; m.cd = c.mcd.p; m.prn = c.mprn.p;
     return m;
 }
 
@@ -579,7 +580,7 @@ int upload_met(Ctx& c) {
     HIPCHK(c.mcosy.upload(cy.data(), np, c.stream));
     HIPCHK(c.mcor.upload(co.data(), np, c.stream));
     if (c.S > 1) HIPCHK(c.nsid.upload(ns.data(), np, c.stream));
-    HIPCHK(c.mnf.alloc((size_t)MET_NF * np));
+    for (DevBuf<double>* b : {&c.mwx, &c.mwy, &c.mwm, &c.mcd, &c.mprn}) HIPCHK(b->alloc(np));
     HIPCHK(c.mrow.alloc((size_t)c.S * kTableChunk * MET_ROW));
     HIPCHK(hipStreamSynchronize(c.stream));
     return 0;
@@ -722,6 +723,25 @@ int upload_mesh(Ctx& c) {
     if (c.prm.ics == 2) {
         HIPCHK(c.sfacq.upload(sfq.data(), sfq.size(), c.stream));
         HIPCHK(c.sfaced.upload(sfe.data(), sfe.size(), c.stream));
+        // slot order, so that each lane reads its own element's column (coalesced) instead of a
+        // row chosen by its local edge number or a neighbour's column
+        std::vector<double> sfs((size_t)2 * 3 * negp * ne, 1.0);
+        for (int e = 0; e < ne; e++)
+            for (int sl = 0; sl < 3; sl++) {
+                const int bits = side_bits(sinfo[e], sl);
+                const int L = bits & 3, role = (bits >> 2) & 1, kind = (bits >> 3) & 3, nled = (bits >> 5) & 3;
+                for (int I = 0; I < negp; I++) {
+                    if (kind == SIDE_INTERNAL) {
+                        const int gown = role ? negp - 1 - I : I, gnb = role ? I : negp - 1 - I;
+                        const int nb = snbr[(size_t)sl * ne + e];
+                        sfs[((size_t)sl * negp + I) * ne + e] = sfe[((size_t)L * negp + gown) * ne + e];
+                        sfs[((size_t)(3 + sl) * negp + I) * ne + e] = sfe[((size_t)nled * negp + gnb) * ne + nb];
+                    } else {
+                        sfs[((size_t)sl * negp + I) * ne + e] = sfe[((size_t)L * negp + I) * ne + e];
+                    }
+                }
+            }
+        HIPCHK(c.sfs.upload(sfs.data(), sfs.size(), c.stream));
         HIPCHK(hipStreamSynchronize(c.stream));
     }
     HIPCHK(c.nmm.alloc((size_t)6 * np));
@@ -1042,8 +1062,7 @@ int ensure_staging(Ctx& c, int nt) {
     }
     const int ns = nt + hmax;
     if (ns > 1023) return fail("stage kernel staging: halo too large for block size " + std::to_string(nt));
-    const int rows = 3 * c.dof + 3 + (c.prm.ics == 2 ? 3 * c.negp : 0);   // stage_kernel's NR
-    const size_t dyn = ((size_t)8 * rows * ns + ns + 7) / 8 * 8;
+    const size_t dyn = ((size_t)8 * (3 * c.dof + 3) * ns + ns + 7) / 8 * 8;
     if (dyn + stage_static_lds(c.dof, c.negp, nt, c.met) > (size_t)c.max_lds)
         return fail("stage kernel staging: block size " + std::to_string(nt) + " needs more LDS than a block may use");
     Ctx::Staging& S = c.stg[nt];
@@ -1066,7 +1085,7 @@ int launch_stage_v(Ctx& c, const StageArgs& a0, bool sph, bool fuse) {
     a.hoff = it->second.hoff.p; a.hidx = it->second.hidx.p; a.snbl = it->second.snbl.p; a.ns = it->second.ns;
     const size_t lds = it->second.lds;
     // every pointer the variant dereferences must be set (a missing one is an error, not a launch)
-    if (MET && (!a.mt.row || !a.mt.nf || !a.fo.ramp))
+    if (MET && (!a.mt.row || !a.mt.cd || !a.mt.wm || !a.mt.wx || !a.mt.wy || !a.mt.prn || !a.fo.ramp))
         return fail("internal: NWS=8 stage launch without its forcing arguments");
     if (!a.uin || !a.uout || !a.wd_in || !a.hoff || !a.hidx || !a.snbl)
         return fail("internal: stage launch with a missing buffer");
@@ -1136,10 +1155,6 @@ int launch_stage_nt(Ctx& c, const StageArgs& a, bool sph, bool fuse, int nt) {
     return launch_stage<P, 128>(c, a, sph, fuse);
 }
 
-// The stage kernel's copy of its output cell means: needed by the limiter when WETDRY follows it
-// (WETDRY rewrites cell means in place).
-double* cmean_out(Ctx& c) { return c.prm.slopeflag != 0 && c.prm.nolifa >= 2 ? c.cmean.p : nullptr; }
-
 StageArgs stage_args(Ctx& c, int irk, const int slot[], bool captured = false) {
     StageArgs a{};
     a.ed = elem_data(c); a.ph = c.phys; a.fo = forcing(c, captured);
@@ -1159,31 +1174,30 @@ StageArgs stage_args(Ctx& c, int irk, const int slot[], bool captured = false) {
     a.irk = irk;
     a.wd_active = c.wd_active;
     a.tip = c.tip ? c.tip2.p + (size_t)irk * c.np : nullptr;
-    a.cmout = cmean_out(c);
+    a.cmout = c.prm.slopeflag != 0 ? c.cmean.p : nullptr;
     return a;
 }
 
-// SLOPELIMITER5 (+ WETDRY) on the stage output u, in place; with WETDRY the stage kernel has left
-// u's cell means in c.cmean.
+// SLOPELIMITER5 (+ WETDRY) on the stage output u, in place; the stage kernel has left u's cell
+// means in c.cmean.
 template <int P>
 int launch_limiter(Ctx& c, double* u, const ElemData& ed, bool capturing) {
     const int ne = c.ne;
     const int ge = (ne + LIM_NT - 1) / LIM_NT;
     if (!c.lim.elq.p || !c.cmean.p) return fail("internal: slope limiter tiles not built");
-    const bool wdon = c.prm.nolifa >= 2;
-    const double* cm = wdon ? c.cmean.p : u;
-    const size_t cms = wdon ? (size_t)ne : (size_t)c.dof * ne;
     LimTiles lt{c.lim.vhoff.p, c.lim.vhidx.p, c.lim.qoff.p, c.lim.qinc_off.p, c.lim.qinc.p, c.lim.elq.p,
                 c.lim.nsl, c.lim.nql};
     const size_t lds = (size_t)8 * (3 * lt.nsl + 6 * lt.nql);
     int ev = prof_begin(c, K_LIMIT, capturing);
     if (c.prm.nolifa >= 2) {
-        hipLaunchKernelGGL((limit_wetdry_kernel<P, true>), dim3(ge), dim3(LIM_NT), lds, c.stream, ed, u, cm, cms,
-                           lt, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p, c.wd_active);
+        hipLaunchKernelGGL((limit_wetdry_kernel<P, true>), dim3(ge), dim3(LIM_NT), lds, c.stream, ed, u,
+                           (const double*)c.cmean.p, lt, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p,
+                           c.wd_active);
         c.wdcur = 1 - c.wdcur;
     } else {
-        hipLaunchKernelGGL((limit_wetdry_kernel<P, false>), dim3(ge), dim3(LIM_NT), lds, c.stream, ed, u, cm, cms,
-                           lt, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p, (int*)nullptr);
+        hipLaunchKernelGGL((limit_wetdry_kernel<P, false>), dim3(ge), dim3(LIM_NT), lds, c.stream, ed, u,
+                           (const double*)c.cmean.p, lt, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p,
+                           (int*)nullptr);
     }
     prof_end(c, ev);
     HIPCHK(hipGetLastError());
@@ -1275,7 +1289,7 @@ int launch_step_explicit(Ctx& c, double* const* st, double* const* wsave = nullp
         a.irk = irk;
         a.wd_active = c.wd_active;
         a.tip = c.tip ? c.tip2.p + (size_t)irk * c.np : nullptr;
-        a.cmout = cmean_out(c);
+        a.cmout = lim ? c.cmean.p : nullptr;
         if (launch_stage_nt<P>(c, a, sph, fuse, c.block)) return 1;
         HIPCHK(hipGetLastError());
         if (fuse) c.wdcur = 1 - c.wdcur;
@@ -1341,7 +1355,7 @@ int launch_adj_nl(Ctx& c, const AdjArgs& a0) {
     AdjArgs a = a0;
     a.hoff = it->second.hoff.p; a.hidx = it->second.hidx.p; a.snbl = it->second.snbl.p; a.ns = it->second.ns;
     const size_t lds = adj_dynamic_lds(c.dof, it->second.ns);
-    if (a.met && (!a.mt.nf || !a.fo.ramp))
+    if (a.met && (!a.mt.cd || !a.mt.wm || !a.mt.wx || !a.mt.wy || !a.mt.prn || !a.fo.ramp))
         return fail("internal: NWS=8 adjoint launch without its forcing arguments");
     if (!a.u || !a.out) return fail("internal: adjoint launch with a missing buffer");
     // NOLIBF=1 (quadratic friction, no hybrid factor) with the nonlinear equations: friction fixed

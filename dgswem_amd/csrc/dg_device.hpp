@@ -72,6 +72,8 @@ struct ElemData {
     const double* __restrict__ cori_el; // [ne] or nullptr
     const double* __restrict__ sfacq;   // [nagp][ne] spherical factor at area points, or nullptr
     const double* __restrict__ sfaced;  // [3][negp][ne] spherical factor at edge points, or nullptr
+    const double* __restrict__ sfs;     // [2][3 slots][negp][ne] the same in slot order: [0] own side
+                                        // at the edge's point I, [1] the internal neighbour's side
     const int* __restrict__ sinfo;      // [ne] packed side descriptors
     const int* __restrict__ snbr;       // [3][ne] neighbour element / boundary-list index per slot
     const int* __restrict__ nm;         // [3][ne] 0-based vertex nodes
@@ -140,13 +142,12 @@ struct Met {
     const double* __restrict__ yc;      // [np] latitude (deg)
     const double* __restrict__ cosy;    // [np] cos(latitude)
     const double* __restrict__ cor;     // [np] 2 omega sin(latitude)
-    double* __restrict__ nf;            // [np][MET_NF] per node: 10-m wind x, y (m/s), |W|, drag
-                                        // coefficient, surface pressure (m of water), unused
+    double* __restrict__ wx;            // [np] 10-m wind (m/s), x
+    double* __restrict__ wy;            // [np] y
+    double* __restrict__ wm;            // [np] |W|
+    double* __restrict__ cd;            // [np] drag coefficient
+    double* __restrict__ prn;           // [np] surface pressure (m of water)
 };
-// The node record of Met::nf: six doubles, so that an element reads each vertex's record with three
-// 16-byte loads.
-constexpr int MET_NF = 6;
-enum { NF_WX = 0, NF_WY = 1, NF_WM = 2, NF_CD = 3, NF_PR = 4 };
 
 // Block-local node bounds of the slope limiter.  Tile t (elements [t*NT, t*NT+NT)) stages the
 // cell means of its own elements (rows 0..NT-1) and of its vertex halo -- the elements outside

@@ -227,11 +227,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
     // edge basis and edge quadrature tables: indexed by each lane's own local edge number, so
     // read from LDS (short latency) rather than as divergent constant-memory loads
     __shared__ double tpe[3][NEGP][DOF], teq[3][NEGP][DOF];
-    // staged states: rows [field*DOF + mode] then the three bathymetry modes, then (spherical runs)
-    // the spherical factors at the edge points [local edge*NEGP + point], ns entries each; then the
-    // wet/dry flags (bytes)
-    constexpr int SFR = 3 * DOF + 3;                   // first spherical-factor row
-    constexpr int NR = SFR + (SPH ? 3 * NEGP : 0);     // staged rows
+    // staged states: rows [field*DOF + mode] then the three bathymetry modes, ns entries each, then
+    // the wet/dry flags (bytes)
     extern __shared__ double stg[];
     const DevTables& T = tab<P>();
     const ElemData& E = a.ed;
@@ -242,7 +239,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
     const int e = tile * NT + tid;
     const double c13 = 1.0 / 3.0, c16 = 1.0 / 6.0;
     const int ns = a.ns;
-    uint8_t* const stwd = reinterpret_cast<uint8_t*>(stg + (size_t)NR * ns);
+    uint8_t* const stwd = reinterpret_cast<uint8_t*>(stg + (size_t)(3 * DOF + 3) * ns);
     for (int j = tid; j < 3 * NEGP * DOF; j += NT) {
         const int l = j / (NEGP * DOF), g = (j / DOF) % NEGP, k = j % DOF;
         tpe[l][g][k] = T.phi_edge[l][g][k];
@@ -273,19 +270,15 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
             const bool ok = g < ne, ok2 = g2 < ne;
             g = ok ? g : 0;
             g2 = ok2 ? g2 : 0;
-            double v[NR], v2[NR];
+            double v[3 * DOF + 3], v2[3 * DOF + 3];
 #pragma unroll
             for (int f = 0; f < 3 * DOF; f++) { v[f] = a.uin[(size_t)f * ne + g]; v2[f] = a.uin[(size_t)f * ne + g2]; }
 #pragma unroll
             for (int m = 0; m < 3; m++) { v[3 * DOF + m] = E.dpv[(size_t)m * ne + g]; v2[3 * DOF + m] = E.dpv[(size_t)m * ne + g2]; }
-#pragma unroll
-            for (int q = SFR; q < NR; q++) { v[q] = E.sfaced[(size_t)(q - SFR) * ne + g]; v2[q] = E.sfaced[(size_t)(q - SFR) * ne + g2]; }
             const uint8_t w = a.wd_in[g], w2 = a.wd_in[g2];
             if (ok) {
 #pragma unroll
                 for (int f = 0; f < 3 * DOF; f++) stg[f * ns + j] = v[f];
-#pragma unroll
-                for (int q = SFR; q < NR; q++) stg[q * ns + j] = v[q];
                 const double d0 = v[3 * DOF], d1 = v[3 * DOF + 1], d2 = v[3 * DOF + 2];
                 stg[(3 * DOF) * ns + j] = c13 * (d0 + d1 + d2);
                 stg[(3 * DOF + 1) * ns + j] = -(c16 * (d0 + d1)) + c13 * d2;
@@ -295,8 +288,6 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
             if (ok2) {
 #pragma unroll
                 for (int f = 0; f < 3 * DOF; f++) stg[f * ns + j2] = v2[f];
-#pragma unroll
-                for (int q = SFR; q < NR; q++) stg[q * ns + j2] = v2[q];
                 const double d0 = v2[3 * DOF], d1 = v2[3 * DOF + 1], d2 = v2[3 * DOF + 2];
                 stg[(3 * DOF) * ns + j2] = c13 * (d0 + d1 + d2);
                 stg[(3 * DOF + 1) * ns + j2] = -(c16 * (d0 + d1)) + c13 * d2;
@@ -351,16 +342,15 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
                 hbx[m] = stg[(3 * DOF + m) * ns + sx];
             }
             const int wd_in_el = role ? wdn : wd, wd_ex_el = role ? wd : wdn;
-            // spherical factors of both sides at every point: each side's own value at its local
-            // edge and point (src/prep_DG.F:1085-1091), staged
+            // spherical factors of both sides at every point, loaded together
             double sfi[NEGP], sfx[NEGP];
 #pragma unroll
             for (int I = 0; I < NEGP; I++) {
                 sfi[I] = 1.0;
                 sfx[I] = 1.0;
                 if (SPH) {
-                    sfi[I] = stg[(SFR + Li * NEGP + I) * ns + si];
-                    sfx[I] = stg[(SFR + Lx * NEGP + NEGP - 1 - I) * ns + sx];
+                    sfi[I] = E.sfs[((size_t)((role ? 3 : 0) + slot) * NEGP + I) * ne + e];
+                    sfx[I] = E.sfs[((size_t)((role ? 0 : 3) + slot) * NEGP + I) * ne + e];
                 }
             }
             // the "in" side subtracts w*F and the "ex" side adds it; -(a*b) == (-a)*b exactly, so
@@ -446,7 +436,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
                 const double* pv = tpe[L][I];
                 const double ze_in = trace<DOF>(U[0], pv), qx_in = trace<DOF>(U[1], pv), qy_in = trace<DOF>(U[2], pv);
                 const double hb_in = trace<3>(hb, pv);
-                const double sf = SPH ? stg[(SFR + L * NEGP + I) * ns + tid] : 1.0;
+                const double sf = SPH ? E.sfs[((size_t)slot * NEGP + I) * ne + e] : 1.0;
                 double f0, f1, f2;
                 if (kind == SIDE_LAND) {
                     const double qn_int = qx_in * nx + qy_in * ny, qt_int = qx_in * tx + qy_in * ty;
@@ -509,8 +499,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
 #pragma unroll
         for (int v = 0; v < 3; v++) {
             const int n = E.nm[(size_t)v * ne + e];
-            const double* r = a.mt.nf + (size_t)n * MET_NF;
-            mcd[v] = r[NF_CD]; mwm[v] = r[NF_WM]; mwx[v] = r[NF_WX]; mwy[v] = r[NF_WY]; mpr[v] = r[NF_PR];
+            mcd[v] = a.mt.cd[n]; mwm[v] = a.mt.wm[n]; mwx[v] = a.mt.wx[n]; mwy[v] = a.mt.wy[n]; mpr[v] = a.mt.prn[n];
         }
     }
     double tipv[3] = {0.0, 0.0, 0.0};
@@ -868,17 +857,15 @@ __global__ void __launch_bounds__(256) node_minmax_kernel(int np, int ne, const 
 // means of its elements and of its vertex halo, each node of the tile takes the min/max over its
 // incident elements in the node->element order (as node_minmax_kernel does, so the bounds are the
 // same bit for bit), and each element then limits its modes 2-3 against the bounds of its nodes.
-// With WETDRY the cell means come from the stage kernel's copy `cmean` [3][ne] (field stride
-// cms = ne): WETDRY rewrites some cell means of u in place, which other tiles' halos must not see.
-// Without it they are read from u itself (cms = DOF*ne), whose cell means the limiter keeps.
-// Blocks of LIM_NT threads; the launch passes 8*(3*nsl + 6*nql) bytes of dynamic LDS.
+// The cell means come from the stage kernel's copy `cmean` [3][ne]: WETDRY rewrites some cell
+// means of u in place, which other tiles' halos must not see.  Blocks of LIM_NT threads; the
+// launch passes 8*(3*nsl + 6*nql) bytes of dynamic LDS.
 // ------------------------------------------------------------------------------------------
 constexpr int LIM_NT = 256;
 
 template <int P, bool WETDRY>
 __global__ void __launch_bounds__(LIM_NT) limit_wetdry_kernel(ElemData E, double* __restrict__ u,
-                                                              const double* __restrict__ cmean, size_t cms,
-                                                              LimTiles lt,
+                                                              const double* __restrict__ cmean, LimTiles lt,
                                                               const uint8_t* __restrict__ wd_in,
                                                               uint8_t* __restrict__ wd_out,
                                                               int* __restrict__ wd_active) {
@@ -922,8 +909,8 @@ __global__ void __launch_bounds__(LIM_NT) limit_wetdry_kernel(ElemData E, double
             const bool ok = g < ne, ok2 = g2 < ne;
             g = ok ? g : 0;
             g2 = ok2 ? g2 : 0;
-            const double z = cmean[g], x = cmean[cms + g], y = cmean[2 * cms + g];
-            const double z2 = cmean[g2], x2 = cmean[cms + g2], y2 = cmean[2 * cms + g2];
+            const double z = cmean[g], x = cmean[(size_t)ne + g], y = cmean[(size_t)2 * ne + g];
+            const double z2 = cmean[g2], x2 = cmean[(size_t)ne + g2], y2 = cmean[(size_t)2 * ne + g2];
             if (ok) { cm[j] = z; cm[nsl + j] = x; cm[2 * nsl + j] = y; }
             if (ok2) { cm[j2] = z2; cm[nsl + j2] = x2; cm[2 * nsl + j2] = y2; }
         }
@@ -1123,8 +1110,7 @@ __global__ void __launch_bounds__(256) met_kernel(Met m, const int* __restrict__
     const double wmag = sqrt(wx * wx + wy * wy);
     double cd = 0.001 * (0.75 + 0.067 * wmag);
     if (cd > 0.0035) cd = 0.0035;
-    double* o = m.nf + (size_t)n * MET_NF;
-    o[NF_WX] = wx; o[NF_WY] = wy; o[NF_WM] = wmag; o[NF_CD] = cd; o[NF_PR] = press; o[5] = 0.0;
+    m.wx[n] = wx; m.wy[n] = wy; m.wm[n] = wmag; m.cd[n] = cd; m.prn[n] = press;
 }
 
 // Device step counter used by the forcing-table lookups of captured steps (a kernel, not a memset,
