@@ -325,6 +325,7 @@ def main() -> int:
     name, tot_ms, launches = stats[gpu.K_STAGE]
     avg_s = tot_ms / 1e3 / max(launches, 1)
     alg_bytes = B_PER_ELEM_STAGE[args.p] * ne
+    traffic = traffic_from_profiles(args.p)
     achieved = alg_bytes / avg_s / 1e9
     step_gbs = alg_bytes * nrk * args.steps / wall / 1e9
     kernels = {}
@@ -383,12 +384,13 @@ def main() -> int:
                    "wetdry": case.params.nolifa >= 2, "seed": args.seed, "state_finite": finite,
                    "block": block},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(args.p),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": name, "avg_launch_us": avg_s * 1e6, "timed_launches": launches,
                      "algorithmic_bytes_per_launch": alg_bytes,
                      # the metric's own definition: B_p per element-stage over the whole timed loop
                      "step_level": {"achieved": step_gbs, "frac": step_gbs / HBM_PEAK_GBS},
-                     "limited_by": "FP64 VALU issue (HBM traffic at the algorithmic minimum)"},
+                     "limited_by": "FP64 VALU issue" + (f" (HBM traffic {traffic / alg_bytes:.2f}x the algorithmic bytes)"
+                                                        if traffic else "")},
         "kernels": kernels,
         "cpu_baseline": cpu,
         "ensemble": ens,

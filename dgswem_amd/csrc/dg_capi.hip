@@ -1155,6 +1155,10 @@ int launch_stage_nt(Ctx& c, const StageArgs& a, bool sph, bool fuse, int nt) {
     return launch_stage<P, 128>(c, a, sph, fuse);
 }
 
+// The stage kernel's copy of its output cell means: needed by the limiter when WETDRY follows it
+// (WETDRY rewrites cell means in place).
+double* cmean_out(Ctx& c) { return c.prm.slopeflag != 0 && c.prm.nolifa >= 2 ? c.cmean.p : nullptr; }
+
 StageArgs stage_args(Ctx& c, int irk, const int slot[], bool captured = false) {
     StageArgs a{};
     a.ed = elem_data(c); a.ph = c.phys; a.fo = forcing(c, captured);
@@ -1174,30 +1178,31 @@ StageArgs stage_args(Ctx& c, int irk, const int slot[], bool captured = false) {
     a.irk = irk;
     a.wd_active = c.wd_active;
     a.tip = c.tip ? c.tip2.p + (size_t)irk * c.np : nullptr;
-    a.cmout = c.prm.slopeflag != 0 ? c.cmean.p : nullptr;
+    a.cmout = cmean_out(c);
     return a;
 }
 
-// SLOPELIMITER5 (+ WETDRY) on the stage output u, in place; the stage kernel has left u's cell
-// means in c.cmean.
+// SLOPELIMITER5 (+ WETDRY) on the stage output u, in place; with WETDRY the stage kernel has left
+// u's cell means in c.cmean.
 template <int P>
 int launch_limiter(Ctx& c, double* u, const ElemData& ed, bool capturing) {
     const int ne = c.ne;
     const int ge = (ne + LIM_NT - 1) / LIM_NT;
     if (!c.lim.elq.p || !c.cmean.p) return fail("internal: slope limiter tiles not built");
+    const bool wdon = c.prm.nolifa >= 2;
+    const double* cm = wdon ? c.cmean.p : u;
+    const size_t cms = wdon ? (size_t)ne : (size_t)c.dof * ne;
     LimTiles lt{c.lim.vhoff.p, c.lim.vhidx.p, c.lim.qoff.p, c.lim.qinc_off.p, c.lim.qinc.p, c.lim.elq.p,
                 c.lim.nsl, c.lim.nql};
     const size_t lds = (size_t)8 * (3 * lt.nsl + 6 * lt.nql);
     int ev = prof_begin(c, K_LIMIT, capturing);
     if (c.prm.nolifa >= 2) {
-        hipLaunchKernelGGL((limit_wetdry_kernel<P, true>), dim3(ge), dim3(LIM_NT), lds, c.stream, ed, u,
-                           (const double*)c.cmean.p, lt, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p,
-                           c.wd_active);
+        hipLaunchKernelGGL((limit_wetdry_kernel<P, true>), dim3(ge), dim3(LIM_NT), lds, c.stream, ed, u, cm, cms,
+                           lt, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p, c.wd_active);
         c.wdcur = 1 - c.wdcur;
     } else {
-        hipLaunchKernelGGL((limit_wetdry_kernel<P, false>), dim3(ge), dim3(LIM_NT), lds, c.stream, ed, u,
-                           (const double*)c.cmean.p, lt, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p,
-                           (int*)nullptr);
+        hipLaunchKernelGGL((limit_wetdry_kernel<P, false>), dim3(ge), dim3(LIM_NT), lds, c.stream, ed, u, cm, cms,
+                           lt, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p, (int*)nullptr);
     }
     prof_end(c, ev);
     HIPCHK(hipGetLastError());
@@ -1289,7 +1294,7 @@ int launch_step_explicit(Ctx& c, double* const* st, double* const* wsave = nullp
         a.irk = irk;
         a.wd_active = c.wd_active;
         a.tip = c.tip ? c.tip2.p + (size_t)irk * c.np : nullptr;
-        a.cmout = lim ? c.cmean.p : nullptr;
+        a.cmout = cmean_out(c);
         if (launch_stage_nt<P>(c, a, sph, fuse, c.block)) return 1;
         HIPCHK(hipGetLastError());
         if (fuse) c.wdcur = 1 - c.wdcur;

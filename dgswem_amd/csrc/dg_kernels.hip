@@ -857,15 +857,17 @@ __global__ void __launch_bounds__(256) node_minmax_kernel(int np, int ne, const 
 // means of its elements and of its vertex halo, each node of the tile takes the min/max over its
 // incident elements in the node->element order (as node_minmax_kernel does, so the bounds are the
 // same bit for bit), and each element then limits its modes 2-3 against the bounds of its nodes.
-// The cell means come from the stage kernel's copy `cmean` [3][ne]: WETDRY rewrites some cell
-// means of u in place, which other tiles' halos must not see.  Blocks of LIM_NT threads; the
-// launch passes 8*(3*nsl + 6*nql) bytes of dynamic LDS.
+// With WETDRY the cell means come from the stage kernel's copy `cmean` [3][ne] (field stride
+// cms = ne): WETDRY rewrites some cell means of u in place, which other tiles' halos must not see.
+// Without it they are read from u itself (cms = DOF*ne), whose cell means the limiter keeps.
+// Blocks of LIM_NT threads; the launch passes 8*(3*nsl + 6*nql) bytes of dynamic LDS.
 // ------------------------------------------------------------------------------------------
 constexpr int LIM_NT = 256;
 
 template <int P, bool WETDRY>
 __global__ void __launch_bounds__(LIM_NT) limit_wetdry_kernel(ElemData E, double* __restrict__ u,
-                                                              const double* __restrict__ cmean, LimTiles lt,
+                                                              const double* __restrict__ cmean, size_t cms,
+                                                              LimTiles lt,
                                                               const uint8_t* __restrict__ wd_in,
                                                               uint8_t* __restrict__ wd_out,
                                                               int* __restrict__ wd_active) {
@@ -909,8 +911,8 @@ __global__ void __launch_bounds__(LIM_NT) limit_wetdry_kernel(ElemData E, double
             const bool ok = g < ne, ok2 = g2 < ne;
             g = ok ? g : 0;
             g2 = ok2 ? g2 : 0;
-            const double z = cmean[g], x = cmean[(size_t)ne + g], y = cmean[(size_t)2 * ne + g];
-            const double z2 = cmean[g2], x2 = cmean[(size_t)ne + g2], y2 = cmean[(size_t)2 * ne + g2];
+            const double z = cmean[g], x = cmean[cms + g], y = cmean[2 * cms + g];
+            const double z2 = cmean[g2], x2 = cmean[cms + g2], y2 = cmean[2 * cms + g2];
             if (ok) { cm[j] = z; cm[nsl + j] = x; cm[2 * nsl + j] = y; }
             if (ok2) { cm[j2] = z2; cm[nsl + j2] = x2; cm[2 * nsl + j2] = y2; }
         }
