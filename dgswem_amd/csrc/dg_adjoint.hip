@@ -8,9 +8,13 @@
 // select: the LLF wave-speed bound differentiates through the largest of its six candidates, the
 // Manning friction floor CF and the 0.9/DT friction cap pass no derivative when active, and
 // Eirik's dry-ocean-edge fix differentiates |h|.  SLOPELIMITER5 is differentiated with its active
-// set frozen (limiter_adj_kernel and the node passes after it); WETDRY only where it is the
-// identity (every element wet and staying wet): the forward sweeps flag any element WETDRY changes,
-// and the gradient call then fails.
+// set frozen (limiter_adj_kernel and the node passes after it).  WETDRY after the limiter is
+// differentiated with its branches frozen too (wetdry_adj_kernel): the wet/dry flags at the start
+// of each stage come from the forward run, a dry element's RHS is zero, and an edge with a dry
+// side takes the reverse of the dry-side flux case the forward values select (dry_side_adj).
+// Without the limiter WETDRY is differentiated only where it is the identity (every element wet
+// and staying wet): the forward sweeps flag any element WETDRY changes, and the gradient call then
+// fails.
 //
 // Kernel layout follows the forward stage kernel: one thread per element, gathers only.  The
 // cotangent of an internal edge flux is the sum of what both neighbours' RHS cotangents ask of it;
@@ -247,6 +251,132 @@ __device__ __forceinline__ void llf_flux_adj_side(const Phys& P, double zi, doub
     hbar = hb;
 }
 
+// The mass flux F0 of an internal edge at one point as the forward stage kernel forms it: its
+// traces in the forward's operation order (no contraction) and the LLF flux, so that the dry-side
+// case (src/internal_edge_hydro.F:212-302) is taken as the forward took it.
+template <int DOF, bool NL>
+__device__ __forceinline__ double edge_f0_exact(const Phys& ph, const double (&Vi)[3][DOF], const double (&Vx)[3][DOF],
+                                                const double (&hbi)[3], const double (&hbx)[3], const double* pi_,
+                                                const double* px_, double sfi, double sfx, double nx, double ny) {
+    const double ze_in = trace1<DOF>(Vi[0], pi_), qx_in = trace1<DOF>(Vi[1], pi_), qy_in = trace1<DOF>(Vi[2], pi_);
+    const double ze_ex = trace1<DOF>(Vx[0], px_), qx_ex = trace1<DOF>(Vx[1], px_), qy_ex = trace1<DOF>(Vx[2], px_);
+    const double hb_in = trace1<3>(hbi, pi_), hb_ex = trace1<3>(hbx, px_);
+    double f0, f1, f2;
+    llf<NL>(ph, ze_in, ze_ex, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_ex, sfi, sfx, nx, ny, ph.g, f0, f1, f2);
+    return f0;
+}
+
+// The P1 bathymetry modes of element g as the forward stage kernel stages them (no contraction).
+__device__ __forceinline__ void elem_hb_exact(const ElemData& E, int ne, int g, double hb[3]) {
+    const double c13 = 1.0 / 3.0, c16 = 1.0 / 6.0;
+    const double d0 = E.dpv[g], d1 = E.dpv[(size_t)ne + g], d2 = E.dpv[2 * (size_t)ne + g];
+    hb[0] = c13 * (d0 + d1 + d2);
+    hb[1] = -(c16 * (d0 + d1)) + c13 * d2;
+    hb[2] = -(0.5 * d0) + 0.5 * d1;
+}
+
+// Side geometry as the forward forms it (no contraction).
+__device__ __forceinline__ void side_geometry_exact(const ElemData& E, int ne, int e, int L, int role, double& xlen,
+                                                    double& nx, double& ny) {
+    int va = (L + 1) % 3, vb = (L + 2) % 3;
+    if (role) { const int t = va; va = vb; vb = t; }
+    const double dy = E.yv[(size_t)vb * ne + e] - E.yv[(size_t)va * ne + e];
+    const double dx = E.xv[(size_t)vb * ne + e] - E.xv[(size_t)va * ne + e];
+    xlen = sqrt(dy * dy + dx * dx);
+    nx = dy / xlen;
+    ny = -dx / xlen;
+}
+
+template <bool NL>
+__device__ void dry_side_adj_cases(const Phys& ph, int cas, int role, double zi, double ze, double xi, double xe,
+                                   double yi, double ye, double hi, double he, double sfi, double sfx, double nx,
+                                   double ny, const double cin[3], const double cex[3], double& zb, double& xb,
+                                   double& yb, double& hbar);
+
+// Reverse of an internal edge point where one side is dry (src/internal_edge_hydro.F:212-302),
+// for this element's own side: the case is taken from the forward's mass flux F0 (recomputed as
+// the forward formed it); cin/cex are d(loss)/d(flux the "in"/"ex" side received).  Cases:
+// 1 reflect into a dry "in" (both sides take the reflected flux's F0, "ex" its momentum: a function
+// of the "ex" traces only), 2 zero-gravity momentum for a dry "in", 3 reflect into a dry "ex",
+// 4 zero-gravity momentum for a dry "ex", 0 the plain flux.  Outputs this side's trace cotangents.
+template <int P, bool NL>
+__device__ void dry_side_adj(const Phys& ph, const ElemData& E, int ne, int e, int nb, int L, int role, int nled,
+                             int gi, int gx, const double (&U)[3][(P + 1) * (P + 2) / 2],
+                             const double (&V)[3][(P + 1) * (P + 2) / 2], int wd, int wdn,
+                             double zi, double ze, double xi, double xe, double yi, double ye, double hi, double he,
+                             double sfi, double sfx, double nx, double ny, const double cin[3], const double cex[3],
+                             double& zb, double& xb, double& yb, double& hbar) {
+    constexpr int DOF = (P + 1) * (P + 2) / 2;
+    const DevTables& T = tab<P>();
+    int cas = 0;
+    {
+        // the forward's F0, operation by operation
+        double hbo[3], hbn[3], xlen_x, nx_x, ny_x;
+        elem_hb_exact(E, ne, e, hbo);
+        elem_hb_exact(E, ne, nb, hbn);
+        side_geometry_exact(E, ne, e, L, role, xlen_x, nx_x, ny_x);
+        const int Li = role ? nled : L, Lx = role ? L : nled;
+        const double f0 = role ? edge_f0_exact<DOF, NL>(ph, V, U, hbn, hbo, T.phi_edge[Li][gi], T.phi_edge[Lx][gx], sfi, sfx, nx_x, ny_x)
+                               : edge_f0_exact<DOF, NL>(ph, U, V, hbo, hbn, T.phi_edge[Li][gi], T.phi_edge[Lx][gx], sfi, sfx, nx_x, ny_x);
+        const int wd_in_el = role ? wdn : wd, wd_ex_el = role ? wd : wdn;
+        if (fabs(f0) > 1.0e-8) {
+            if (wd_in_el == 0) cas = f0 > 0.0 ? 1 : 2;
+            else if (wd_ex_el == 0) cas = f0 < 0.0 ? 3 : 4;
+        }
+    }
+    dry_side_adj_cases<NL>(ph, cas, role, zi, ze, xi, xe, yi, ye, hi, he, sfi, sfx, nx, ny, cin, cex, zb, xb, yb, hbar);
+}
+
+// The derivative part of dry_side_adj, for case `cas`.
+template <bool NL>
+__device__ void dry_side_adj_cases(const Phys& ph, int cas, int role, double zi, double ze, double xi, double xe,
+                                   double yi, double ye, double hi, double he, double sfi, double sfx, double nx,
+                                   double ny, const double cin[3], const double cex[3], double& zb, double& xb,
+                                   double& yb, double& hbar) {
+#pragma clang fp contract(fast)
+    zb = xb = yb = hbar = 0.0;
+    if (cas == 0 || cas == 2 || cas == 4) {
+        double fbn[3] = {cin[0] + cex[0], cin[1] + cex[1], cin[2] + cex[2]};
+        double fbg[3] = {0.0, 0.0, 0.0};
+        if (cas == 2) { fbn[1] = cex[1]; fbn[2] = cex[2]; fbg[1] = cin[1]; fbg[2] = cin[2]; }
+        if (cas == 4) { fbn[1] = cin[1]; fbn[2] = cin[2]; fbg[1] = cex[1]; fbg[2] = cex[2]; }
+        llf_flux_adj_side<NL>(ph, zi, ze, xi, xe, yi, ye, hi, he, sfi, sfx, nx, ny, ph.g, fbn, role, zb, xb, yb, hbar);
+        if (cas != 0) {
+            double z2, x2, y2, h2;
+            llf_flux_adj_side<NL>(ph, zi, ze, xi, xe, yi, ye, hi, he, sfi, sfx, nx, ny, 0.0, fbg, role, z2, x2, y2, h2);
+            zb += z2; xb += x2; yb += y2; hbar += h2;
+        }
+        return;
+    }
+    // reflection: the flux is a function of one side's traces only (the "ex" side in case 1, the
+    // "in" side in case 3); the other side keeps nothing
+    const int src = cas == 1 ? 1 : 0;
+    if (role != src) return;
+    const double tx = -ny, ty = nx, den = nx * ty - ny * tx;
+    const double sx = src ? xe : xi, sy = src ? ye : yi;
+    const double qn = sx * nx + sy * ny, qt = sx * tx + sy * ty;
+    const double mqx = (ty * -qn - ny * qt) / den, mqy = (-tx * -qn + nx * qt) / den;
+    // d(mqx, mqy)/d(sx, sy)
+    const double mxx = (-ty * nx - ny * tx) / den, mxy = (-ty * ny - ny * ty) / den;
+    const double myx = (tx * nx + nx * tx) / den, myy = (tx * ny + nx * ty) / den;
+    double bar[8];
+    if (src) {
+        const double fbr[3] = {cin[0] + cex[0], cex[1], cex[2]};
+        llf_flux_adj<NL>(ph, ze, ze, mqx, xe, mqy, ye, he, he, sfx, sfx, nx, ny, ph.g, fbr, bar);
+        zb = bar[0] + bar[1];
+        hbar = bar[6] + bar[7];
+        xb = bar[3] + mxx * bar[2] + myx * bar[4];
+        yb = bar[5] + mxy * bar[2] + myy * bar[4];
+    } else {
+        const double fbr[3] = {cin[0] + cex[0], cin[1], cin[2]};
+        llf_flux_adj<NL>(ph, zi, zi, xi, mqx, yi, mqy, hi, hi, sfi, sfi, nx, ny, ph.g, fbr, bar);
+        zb = bar[0] + bar[1];
+        hbar = bar[6] + bar[7];
+        xb = bar[2] + mxx * bar[3] + myx * bar[5];
+        yb = bar[4] + mxy * bar[3] + myy * bar[5];
+    }
+}
+
 struct AdjArgs {
     ElemData ed;
     Phys ph;
@@ -262,6 +392,7 @@ struct AdjArgs {
     double* __restrict__ out;            // cotangent of U(IRK)
     double* __restrict__ gdp;            // [3][ne] accumulated cotangent of the vertex depths
     double* __restrict__ gnbar;          // [ne] accumulated cotangent of the element mean Manning n
+    const uint8_t* __restrict__ wd;      // WDFLG during the stage, or nullptr (every element wet)
     // block-local staging, as in the forward stage kernel: tile t stages its elements and its halo
     // hidx[hoff[t]..hoff[t+1]); snbl packs each side's local neighbour index (10 bits each)
     const int* __restrict__ hoff;
@@ -371,6 +502,7 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
     const int info = E.sinfo[e];
     const int nb0 = E.snbr[e], nb1 = E.snbr[(size_t)ne + e], nb2 = E.snbr[2 * (size_t)ne + e];
     const int sl = a.snbl[e];
+    const int wd = a.wd ? (int)a.wd[e] : 1;
 #pragma unroll 1
     for (int slot = 0; slot < 3; slot++) {
         const int bits = side_bits(info, slot);
@@ -385,6 +517,8 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
         const double tx = -ny, ty = nx;
         if (kind == SIDE_INTERNAL) {
             const int ln = (sl >> (10 * slot)) & 1023;
+            const int wdn = a.wd ? (int)a.wd[nb] : 1;
+            if (wd == 0 && wdn == 0) continue;          // the forward adds no flux between dry elements
             double V[3][DOF], Ln[3][DOF];
 #pragma unroll
             for (int f = 0; f < 3; f++)
@@ -413,7 +547,7 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
                 // flux cotangent: the "in" element's RHS took -w_in*F, the "ex" element's +w_ex*F
                 const double* eqo = teq[L][gown];
                 const double* eqn = teq[nled][gnb];
-                double fb[3];
+                double fb[3], cin[3], cex[3];
 #pragma unroll
                 for (int f = 0; f < 3; f++) {
                     double so_ = 0.0, sn_ = 0.0;
@@ -423,12 +557,21 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
                         sn_ = sn_ + edfac_n * eqn[k] * Ln[f][k];
                     }
                     fb[f] = role ? (so_ - sn_) : (sn_ - so_);
+                    cin[f] = role ? -sn_ : -so_;
+                    cex[f] = role ? so_ : sn_;
                 }
                 // canonical (in, ex) order; this element is side `role`
                 double zb, xb, yb, hbar;
-                llf_flux_adj_side<NL>(ph, role ? zn : zo, role ? zo : zn, role ? xn : xo, role ? xo : xn,
-                                      role ? yn : yo, role ? yo : yn, role ? hn : ho, role ? ho : hn,
-                                      role ? sn : so, role ? so : sn, nx, ny, ph.g, fb, role, zb, xb, yb, hbar);
+                if (wd != 0 && wdn != 0) {
+                    llf_flux_adj_side<NL>(ph, role ? zn : zo, role ? zo : zn, role ? xn : xo, role ? xo : xn,
+                                          role ? yn : yo, role ? yo : yn, role ? hn : ho, role ? ho : hn,
+                                          role ? sn : so, role ? so : sn, nx, ny, ph.g, fb, role, zb, xb, yb, hbar);
+                } else {
+                    dry_side_adj<P, NL>(ph, E, ne, e, nb, L, role, nled, gi, gx, U, V, wd, wdn,
+                                        role ? zn : zo, role ? zo : zn, role ? xn : xo, role ? xo : xn,
+                                        role ? yn : yo, role ? yo : yn, role ? hn : ho, role ? ho : hn,
+                                        role ? sn : so, role ? so : sn, nx, ny, cin, cex, zb, xb, yb, hbar);
+                }
 #pragma unroll
                 for (int k = 0; k < DOF; k++) {
                     ub_lds[k][tid] += zb * po[k];
@@ -439,6 +582,7 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
                 for (int k = 0; k < 3; k++) hbb[k] += hbar * po[k];
             }
         } else {
+            if (wd == 0) continue;                      // the forward skips a dry element's boundary edges
 #pragma unroll 1
             for (int I = 0; I < NEGP; I++) {
                 const double* pv = tpe[L][I];
@@ -514,9 +658,9 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
         }
     }
 
-    // ---- volume quadrature (every element wet)
+    // ---- volume quadrature (wet elements only, as in the forward)
     double nbarb = 0.0;
-    {
+    if (wd != 0) {
         double drdx, dsdx, drdy, dsdy;
         {
             const double X0 = E.xv[e], X1 = E.xv[(size_t)ne + e], X2 = E.xv[2 * (size_t)ne + e];
@@ -727,6 +871,173 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
         a.gdp[2 * (size_t)ne + e] += D2b;
     }
 #undef Lm
+}
+
+// ------------------------------------------------------------------------------------------
+// WETDRY (src/wetdry.F:58-548) reversed with its branches frozen: from the state it was applied to
+// (Ul: the limited stage output) and the element's flag before it, each element retakes the
+// forward's decisions operation by operation (no contraction, so that they are the forward's) --
+// which vertices are dry, the branch, the vertices' depth order, the two fmax(H0, .) choices, the
+// vertices dry after the redistribution and the check -- and applies the transpose of the linear
+// map that branch is.  ub (in place): cotangent of WETDRY's output -> of its input; gdp += the
+// cotangents of the vertex depths.
+// ------------------------------------------------------------------------------------------
+template <int P>
+__global__ void __launch_bounds__(256) wetdry_adj_kernel(ElemData E, const double* __restrict__ Ul,
+                                                         const uint8_t* __restrict__ wd_in,
+                                                         double* __restrict__ ub, double* __restrict__ gdp) {
+    constexpr int DOF = (P + 1) * (P + 2) / 2;
+    const DevTables& T = tab<P>();
+    const int ne = E.ne;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    const double ZERO = 1.0e-6, C13 = 1.0 / 3.0, C16 = 1.0 / 6.0;
+    const int sid = E.sid ? (int)E.sid[e] : 0;
+    const double H0 = E.scen[sid].h0;
+    const int wdflg = wd_in[e];
+    double U[3][3];
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) U[f][k] = Ul[(size_t)(f * DOF + k) * ne + e];
+    const double D[3] = {E.dpv[e], E.dpv[(size_t)ne + e], E.dpv[2 * (size_t)ne + e]};
+    const double dpe_min = fmin(fmin(D[0], D[1]), D[2]);
+    // the forward's vertex values, in its operation order
+    int ndry = 0, ht_max_i = 0;
+    double ht_max = ZERO, ze_node[3], ht_node[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        double z = U[0][0];
+#pragma unroll
+        for (int k = 1; k < 3; k++) z = z + T.phi_corner[i][k] * U[0][k];
+        ze_node[i] = z;
+        ht_node[i] = D[i] + z;
+        if (ht_node[i] > ht_max) { ht_max_i = i; ht_max = ht_node[i]; }
+        if (ht_node[i] <= (H0 + ZERO)) ndry++;
+    }
+    if (ndry == 0 && wdflg == 1) return;                       // identity
+    const double dp_avg = C13 * (D[0] + D[1] + D[2]);
+    double G[3][DOF];                                           // output cotangent
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+#pragma unroll
+        for (int k = 0; k < DOF; k++) G[f][k] = ub[(size_t)(f * DOF + k) * ne + e];
+    double I[3][DOF];                                           // input cotangent
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+#pragma unroll
+        for (int k = 0; k < DOF; k++) I[f][k] = 0.0;
+    double Db[3] = {0.0, 0.0, 0.0};
+    if (ndry == 0) {
+        // the check alone: the element keeps its values, its momentum too when the highest
+        // vertex is wet enough
+        const bool keep = ze_node[ht_max_i] > (-dpe_min + H0);
+#pragma unroll
+        for (int k = 0; k < DOF; k++) {
+            I[0][k] = G[0][k];
+            I[1][k] = keep ? G[1][k] : 0.0;
+            I[2][k] = keep ? G[2][k] : 0.0;
+        }
+    } else {
+        const double ht_avg = U[0][0] + dp_avg;
+        if (ndry == 3 || ht_avg <= (H0 + ZERO)) {
+            // flat: ZE = (ht_avg - dp_avg | ZERO - dp_avg, -HB2, -HB3), momentum zero
+            if (ht_avg < ZERO) {
+#pragma unroll
+                for (int i = 0; i < 3; i++) Db[i] -= C13 * G[0][0];
+            } else {
+                I[0][0] = G[0][0];
+            }
+            // -HB2 = C16*(D1+D2) - C13*D3, -HB3 = 0.5*D1 - 0.5*D2
+            Db[0] += C16 * G[0][1] + 0.5 * G[0][2];
+            Db[1] += C16 * G[0][1] - 0.5 * G[0][2];
+            Db[2] += -C13 * G[0][1];
+        } else {
+            // redistribution, as the forward orders the vertices and takes its branches
+            int l2h[3] = {0, 1, 2};
+#pragma unroll
+            for (int k1 = 0; k1 < 3; k1++)
+#pragma unroll
+                for (int k2 = k1; k2 < 3; k2++)
+                    if (ht_node[l2h[k1]] > ht_node[l2h[k2]]) { const int t = l2h[k1]; l2h[k1] = l2h[k2]; l2h[k2] = t; }
+            double dh[3];
+            const bool b0 = H0 > ht_node[l2h[0]];
+            dh[l2h[0]] = fmax(H0, ht_node[l2h[0]]) - ht_node[l2h[0]];
+            const bool b1 = H0 > ht_node[l2h[1]] - 0.5 * dh[l2h[0]];
+            dh[l2h[1]] = fmax(H0, ht_node[l2h[1]] - 0.5 * dh[l2h[0]]) - ht_node[l2h[1]];
+            dh[l2h[2]] = -dh[l2h[0]] - dh[l2h[1]];
+            double ht_hat[3];
+            int nd = 0, dry[3];
+            ht_max = 0;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                ht_hat[i] = ht_node[i] + dh[i];
+                if (ht_hat[i] > ht_max) { ht_max_i = i; ht_max = ht_hat[i]; }
+                dry[i] = ht_hat[i] <= (H0 + 1.0e-10);
+                nd += dry[i];
+            }
+            double ze_hat[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) ze_hat[i] = ht_hat[i] - D[i];
+            const bool keep = ze_hat[ht_max_i] > (-dpe_min + H0);
+            // momentum: QX(1..3) from the vertex values qxh, qxh = qxn + qxs at wet vertices,
+            // qxs = (sum of the dry vertices' qxn)/(3 - nd); zero when nd = 3 or the check fails
+            if (keep && nd < 3) {
+#pragma unroll
+                for (int f = 1; f < 3; f++) {
+                    double hb_[3];
+#pragma unroll
+                    for (int i = 0; i < 3; i++)
+                        hb_[i] = C13 * G[f][0] + (i < 2 ? -C16 : C13) * G[f][1] + (i == 0 ? -0.5 : (i == 1 ? 0.5 : 0.0)) * G[f][2];
+                    double sb = 0.0, nb_[3];
+#pragma unroll
+                    for (int i = 0; i < 3; i++) { nb_[i] = dry[i] ? 0.0 : hb_[i]; if (!dry[i]) sb += hb_[i]; }
+                    const double w = sb / (3.0 - (double)nd);
+#pragma unroll
+                    for (int i = 0; i < 3; i++) if (dry[i]) nb_[i] += w;
+                    // qxn_i = QX1 + PHI(i,2)*QX2 + PHI(i,3)*QX3
+#pragma unroll
+                    for (int i = 0; i < 3; i++) {
+                        I[f][0] += nb_[i];
+                        I[f][1] += T.phi_corner[i][1] * nb_[i];
+                        I[f][2] += T.phi_corner[i][2] * nb_[i];
+                    }
+                }
+            }
+            // elevation: ZE1 kept; ZE(2..3) from ze_hat = ht_hat - D = ze_node + dh
+            I[0][0] += G[0][0];
+            double zb[3], dhb[3], htb[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                zb[i] = (i < 2 ? -C16 : C13) * G[0][1] + (i == 0 ? -0.5 : (i == 1 ? 0.5 : 0.0)) * G[0][2];
+                dhb[i] = zb[i];
+            }
+            // dh(l2) = -dh(l0) - dh(l1)
+            dhb[l2h[0]] -= dhb[l2h[2]];
+            dhb[l2h[1]] -= dhb[l2h[2]];
+            // dh(l1) = H0 - ht(l1), or (ht(l1) - 0.5*dh(l0)) - ht(l1) = -0.5*dh(l0)
+            if (b1) htb[l2h[1]] -= dhb[l2h[1]];
+            else dhb[l2h[0]] += -0.5 * dhb[l2h[1]];
+            // dh(l0) = H0 - ht(l0), or zero
+            if (b0) htb[l2h[0]] -= dhb[l2h[0]];
+            // ht = D + ze_node
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                Db[i] += htb[i];
+                zb[i] += htb[i];
+                I[0][0] += zb[i];
+                I[0][1] += T.phi_corner[i][1] * zb[i];
+                I[0][2] += T.phi_corner[i][2] * zb[i];
+            }
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+#pragma unroll
+        for (int k = 0; k < DOF; k++) ub[(size_t)(f * DOF + k) * ne + e] = I[f][k];
+    gdp[e] += Db[0];
+    gdp[(size_t)ne + e] += Db[1];
+    gdp[2 * (size_t)ne + e] += Db[2];
 }
 
 // Elevation at the gauges: eta_g = sum_K ZE(K, elem_g) * phi_K(gauge), written to out[g].

@@ -201,6 +201,12 @@ struct Ctx {
     // node sums and bound-attaining elements
     DevBuf<double> wseg, lnbc, lgsum;
     DevBuf<int> larg;
+    // active wetting/drying under the limiter: the flags at each checkpoint and at the start of every
+    // stage of a segment, and the limited (pre-WETDRY) output of a stage
+    DevBuf<uint8_t> ckptwd, segwd;
+    DevBuf<double> ltmp;
+    bool adj_wd = false;      // WETDRY after the limiter: flags kept at the checkpoints
+    bool adj_wdrev = false;   // ... and WETDRY changed an element in the forward sweep: reversed
     // graphs of one slot-rotation period, keyed by (ucur, wdcur) at capture
     bool use_graph = false;
     std::map<int, hipGraphExec_t> graphs;
@@ -1209,6 +1215,23 @@ int launch_limiter(Ctx& c, double* u, const ElemData& ed, bool capturing) {
     return 0;
 }
 
+// SLOPELIMITER5 alone (no WETDRY) on u, in place: the limited output the adjoint's frozen-branch
+// WETDRY reverse starts from.
+template <int P>
+int launch_limiter_only(Ctx& c, double* u) {
+    const int ne = c.ne;
+    const int ge = (ne + LIM_NT - 1) / LIM_NT;
+    if (!c.lim.elq.p) return fail("internal: slope limiter tiles not built");
+    LimTiles lt{c.lim.vhoff.p, c.lim.vhidx.p, c.lim.qoff.p, c.lim.qinc_off.p, c.lim.qinc.p, c.lim.elq.p,
+                c.lim.nsl, c.lim.nql};
+    const size_t lds = (size_t)8 * (3 * lt.nsl + 6 * lt.nql);
+    const ElemData ed = elem_data(c);
+    hipLaunchKernelGGL((limit_wetdry_kernel<P, false>), dim3(ge), dim3(LIM_NT), lds, c.stream, ed, u, (const double*)u,
+                       (size_t)c.dof * ne, lt, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p, (int*)nullptr);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 // One time step: NRK stages (stage kernel, then node min/max + limiter/wet-dry when the slope
 // limiter is on) and the device step-counter increment.
 template <int P>
@@ -1263,9 +1286,10 @@ int launch_steps(Ctx& c, int n, bool capturing) {
 
 // One step with explicit state pointers: st[0] input, st[1..NRK-1] the intermediate stage states,
 // st[NRK] output.  With the slope limiter on, each stage's output before limiting is also copied to
-// wsave[irk] (when wsave is given) for the adjoint's frozen-limiter reverse.
+// wsave[irk] (when wsave is given) for the adjoint's frozen-limiter reverse, and the wet/dry flags
+// at the start of each stage to fsave[irk] (when given).
 template <int P>
-int launch_step_explicit(Ctx& c, double* const* st, double* const* wsave = nullptr) {
+int launch_step_explicit(Ctx& c, double* const* st, double* const* wsave = nullptr, uint8_t* const* fsave = nullptr) {
     const int nrk = c.rk.nrk;
     const bool sph = c.prm.ics == 2;
     const bool wdon = c.prm.nolifa >= 2, lim = c.prm.slopeflag != 0, fuse = wdon && !lim;
@@ -1295,6 +1319,7 @@ int launch_step_explicit(Ctx& c, double* const* st, double* const* wsave = nullp
         a.wd_active = c.wd_active;
         a.tip = c.tip ? c.tip2.p + (size_t)irk * c.np : nullptr;
         a.cmout = cmean_out(c);
+        if (fsave) HIPCHK(hipMemcpyAsync(fsave[irk], c.wd[c.wdcur].p, c.ne, hipMemcpyDeviceToDevice, c.stream));
         if (launch_stage_nt<P>(c, a, sph, fuse, c.block)) return 1;
         HIPCHK(hipGetLastError());
         if (fuse) c.wdcur = 1 - c.wdcur;
@@ -1408,9 +1433,13 @@ int adj_forward(Ctx& c, int it0, int nsteps, int interval, bool save) {
         c.hstep = 0;
         for (int s = 0; s < n; s++) {
             const int k = s0 + s;   // steps done before this one
-            if (save && k % interval == 0)
+            if (save && k % interval == 0) {
                 HIPCHK(hipMemcpyAsync(c.ckpt.p + (size_t)(k / interval) * ssz, c.u[c.ucur].p, ssz * sizeof(double),
                                       hipMemcpyDeviceToDevice, c.stream));
+                if (c.adj_wd)
+                    HIPCHK(hipMemcpyAsync(c.ckptwd.p + (size_t)(k / interval) * c.ne, c.wd[c.wdcur].p, c.ne,
+                                          hipMemcpyDeviceToDevice, c.stream));
+            }
             if (launch_step<P>(c, false)) return 1;
             hipLaunchKernelGGL((gauge_kernel<P>), dim3(gg), dim3(B), 0, c.stream, c.ng, c.ne, (const int*)c.gel.p,
                                (const double*)c.gphi.p, (const double*)c.u[c.ucur].p, c.geta.p + (size_t)k * c.ng);
@@ -1442,6 +1471,17 @@ int adj_backward(Ctx& c, int it0, int nsteps, int interval, double weight) {
     const bool lim = c.prm.slopeflag != 0;
     const int np = c.np, B = 256, ge = (ne + B - 1) / B, gn = (np + B - 1) / B;
     const ElemData ed = elem_data(c);
+    // frozen-branch WETDRY reverse of one stage (active wetting/drying): from the stage's limited
+    // output, formed again from its pre-limiter output, and the flags at the start of the stage
+    const bool wdrev_on = c.adj_wdrev;
+    auto wdrev = [&](double* cot, const double* wpre, const uint8_t* flags) -> int {
+        HIPCHK(hipMemcpyAsync(c.ltmp.p, wpre, ssz * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+        if (launch_limiter_only<P>(c, c.ltmp.p)) return 1;
+        hipLaunchKernelGGL((wetdry_adj_kernel<P>), dim3(ge), dim3(B), 0, c.stream, ed, (const double*)c.ltmp.p, flags,
+                           cot, c.gdp.p);
+        HIPCHK(hipGetLastError());
+        return 0;
+    };
     auto limrev = [&](double* cot, const double* wpre) -> int {
         hipLaunchKernelGGL((node_minmax_kernel<P>), dim3(gn), dim3(B), 0, c.stream, np, ne, (const int*)c.n2e_off.p,
                            (const int*)c.n2e.p, wpre, c.nmm.p);
@@ -1465,23 +1505,27 @@ int adj_backward(Ctx& c, int it0, int nsteps, int interval, double weight) {
         auto stage = [&](int k, int i) { return c.seg.p + ((size_t)(k - k0 - 1) * nrk + i) * ssz; };
         // pre-limiter outputs: wstage(k, i) is stage i's output of step k before limiting
         auto wstage = [&](int k, int i) { return c.wseg.p + ((size_t)(k - k0 - 1) * nrk + i) * ssz; };
+        // wet/dry flags at the start of stage i of step k (active wetting/drying)
+        auto fstage = [&](int k, int i) { return c.segwd.p + ((size_t)(k - k0 - 1) * nrk + i) * ne; };
         if (build_stage_tables(c, it0 + k0, n)) return 1;
         HIPCHK(hipMemcpyAsync(stage(k0 + 1, 0), c.ckpt.p + (size_t)sg * ssz, ssz * sizeof(double),
                               hipMemcpyDeviceToDevice, c.stream));
+        if (wdrev_on)
+            HIPCHK(hipMemcpyAsync(c.wd[c.wdcur].p, c.ckptwd.p + (size_t)sg * ne, ne, hipMemcpyDeviceToDevice, c.stream));
         c.hstep = 0;
-        for (int k = k0 + 1; k < k1; k++) {
+        for (int k = k0 + 1; k <= k1; k++) {
+            // the last step runs for its intermediate stages, pre-limiter outputs and flags only
+            if (k == k1 && !(nrk > 1 || lim)) break;
             double* st[KMAXRK + 1];
             double* ws[KMAXRK];
-            for (int i = 0; i < nrk; i++) { st[i] = stage(k, i); ws[i] = lim ? wstage(k, i) : nullptr; }
-            st[nrk] = stage(k + 1, 0);
-            if (launch_step_explicit<P>(c, st, lim ? ws : nullptr)) return 1;
-        }
-        if (nrk > 1 || lim) {   // the last step's intermediate stages (and pre-limiter outputs)
-            double* st[KMAXRK + 1];
-            double* ws[KMAXRK];
-            for (int i = 0; i < nrk; i++) { st[i] = stage(k1, i); ws[i] = lim ? wstage(k1, i) : nullptr; }
-            st[nrk] = c.pout.p;   // output unused
-            if (launch_step_explicit<P>(c, st, lim ? ws : nullptr)) return 1;
+            uint8_t* fs[KMAXRK];
+            for (int i = 0; i < nrk; i++) {
+                st[i] = stage(k, i);
+                ws[i] = lim ? wstage(k, i) : nullptr;
+                fs[i] = wdrev_on ? fstage(k, i) : nullptr;
+            }
+            st[nrk] = k < k1 ? stage(k + 1, 0) : c.pout.p;   // the last step's output is unused
+            if (launch_step_explicit<P>(c, st, lim ? ws : nullptr, wdrev_on ? fs : nullptr)) return 1;
         }
         for (int k = k1; k > k0; k--) {
             c.hstep = k - k0 - 1;
@@ -1492,6 +1536,7 @@ int adj_backward(Ctx& c, int it0, int nsteps, int interval, double weight) {
                 HIPCHK(hipGetLastError());
             }
             ub[nrk] = lam;
+            if (wdrev_on && wdrev(ub[nrk], wstage(k, nrk - 1), fstage(k, nrk - 1))) return 1;
             if (lim && limrev(ub[nrk], wstage(k, nrk - 1))) return 1;
             for (int i = nrk - 1; i >= 0; i--) {
                 AdjArgs a{};
@@ -1509,12 +1554,14 @@ int adj_backward(Ctx& c, int it0, int nsteps, int interval, double weight) {
                 a.out = ub[i];
                 a.gdp = c.gdp.p;
                 a.gnbar = c.phys.load_manning ? c.gnb.p : nullptr;
+                a.wd = wdrev_on ? fstage(k, i) : nullptr;
                 int rc;
                 if constexpr (P == 1) rc = launch_adj<P, 128>(c, a);
                 else rc = launch_adj<P, 64>(c, a);
                 if (rc) return rc;
                 HIPCHK(hipGetLastError());
-                // ub[i] is now the whole cotangent of U(i), the limited output of stage i-1
+                // ub[i] is now the whole cotangent of U(i), the output of stage i-1
+                if (wdrev_on && i >= 1 && wdrev(ub[i], wstage(k, i - 1), fstage(k, i - 1))) return 1;
                 if (lim && i >= 1 && limrev(ub[i], wstage(k, i - 1))) return 1;
             }
             std::swap(lam, ub[0]);   // cotangent of U^(k-1)
@@ -2164,6 +2211,11 @@ int dg_adj_gradient(int h, int it, int nsteps, const double* data, double weight
         HIPCHK(c->lgsum.alloc((size_t)6 * np));
         HIPCHK(c->larg.alloc((size_t)6 * np));
     }
+    // wetting/drying after the slope limiter: differentiated with its branches frozen (flags kept
+    // per checkpoint and per stage)
+    c->adj_wd = c->prm.nolifa >= 2 && c->prm.slopeflag != 0;
+    c->adj_wdrev = false;
+    if (c->adj_wd) HIPCHK(c->ckptwd.alloc((size_t)ncp * ne));
     // the state and wet/dry flags at entry: every exit path puts them back (the sweeps rotate the
     // stage slots and flip the flag buffers)
     HIPCHK(c->wdsave.alloc(ne));
@@ -2175,14 +2227,29 @@ int dg_adj_gradient(int h, int it, int nsteps, const double* data, double weight
         HIPCHK(hipStreamSynchronize(c->stream));
         return 0;
     };
-    // WETDRY is differentiated with its active set frozen at the identity: every element wet and
-    // left unchanged.  The forward sweeps flag any element it changes; such runs are refused.
+    // The forward sweeps flag any element WETDRY changes (wets, dries, redistributes or keeps
+    // flat).  Where WETDRY leaves every element as it is, it is the identity and the reverse skips
+    // it.  Without the slope limiter WETDRY runs fused into the stage kernel and has no reverse:
+    // runs it changes are refused.  With the limiter a run it changes is reversed with WETDRY's
+    // branches frozen (the flags at the start of every stage kept per segment).
     HIPCHK(c->wdact.alloc(1));
     HIPCHK(hipMemsetAsync(c->wdact.p, 0, sizeof(int), c->stream));
     c->wd_active = c->prm.nolifa >= 2 ? c->wdact.p : nullptr;
     const bool prof = c->prof;
     c->prof = false;
     int rc = c->prm.p == 1 ? adj_forward<1>(*c, it, nsteps, interval, true) : adj_forward<2>(*c, it, nsteps, interval, true);
+    if (!rc && c->adj_wd) {
+        int changed = 0;
+        if (hipMemcpyAsync(&changed, c->wdact.p, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = fail("reading the wet/dry activity flag failed");
+        else if (changed) {
+            c->adj_wdrev = true;
+            c->wd_active = nullptr;
+            if (c->segwd.alloc((size_t)interval * nrk * ne) != hipSuccess || c->ltmp.alloc(ssz) != hipSuccess)
+                rc = fail("out of device memory for the wet/dry reverse");
+        }
+    }
     if (!rc) rc = c->prm.p == 1 ? adj_backward<1>(*c, it, nsteps, interval, weight) : adj_backward<2>(*c, it, nsteps, interval, weight);
     c->prof = prof;
     c->wd_active = nullptr;
@@ -2194,10 +2261,10 @@ int dg_adj_gradient(int h, int it, int nsteps, const double* data, double weight
     int wdflag = 0;
     HIPCHK(hipMemcpyAsync(&wdflag, c->wdact.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (wdflag) {
+    if (wdflag && !c->adj_wd) {
         if (restore()) return 1;
-        return fail("wetting/drying changed elements during the run: the adjoint differentiates WETDRY "
-                    "only where it leaves every element as it is");
+        return fail("wetting/drying changed elements during the run: without the slope limiter the adjoint "
+                    "differentiates WETDRY only where it leaves every element as it is");
     }
     // node gradients
     double* gdp_n = c->pout.p;

@@ -236,9 +236,12 @@ int dg_adj_gauges(int handle, int it, int nsteps, double* eta);
  * limiter (SLOPELIMITER5) is differentiated with its active set frozen as the forward run
  * recomputes it (the selected node bounds, the redistribution counts and signs are those of the
  * forward run; the reverse of each branch is exact), so the gradient is that of the piecewise
- * smooth map on the forward run's branch.  Wetting/drying is taken as the identity it is when no
- * element changes state; if any element wets or dries during the run the call fails (nonzero,
- * dg_gpu_last_error() says so) and leaves the state as it was.  NWS=8 forcing and ensembles are supported.  For an ensemble
+ * smooth map on the forward run's branch.  With the slope limiter on, wetting/drying (WETDRY and
+ * the dry-side edge fluxes) is differentiated the same way, with the wet/dry flags and WETDRY's
+ * branches those of the forward run.  Without the limiter it is taken as the identity it is when
+ * no element changes state; if any element wets, dries or is redistributed during such a run the
+ * call fails (nonzero, dg_gpu_last_error() says so) and leaves the state as it was.  NWS=8 forcing
+ * and ensembles are supported.  For an ensemble
  * the members are independent, so the gradient of the summed loss restricted to a member's nodes
  * is that member's own gradient (all members in one batched forward and one batched reverse
  * sweep).  The handle's state is left as it was.  grad_manning / grad_dp: NP values each (either

@@ -145,6 +145,8 @@ typedef struct oracle {
     int *lim_node;   /* [call][np][6]: elements attaining each field's min and max bound, -1 none */
     int *lim_elem;   /* [call][ne][3]: per field, the clamp (2 bits per vertex), pass signs, positive
                         differences and min(redfac, redmax) choices, packed */
+    int wd_pos;
+    int *wd_rec;     /* [call][ne]: WETDRY's branches (wetdry_code), recorded with the limiter's */
 } oracle_t;
 
 #define ZE(s, l, k) o->ze[((size_t)(s) * o->ne + (l)) * o->dofh + (k)]
@@ -1661,24 +1663,26 @@ static void slopelimiter5_frozen(oracle_t *o) {
 
 /* Record the limiter branches of the next ncalls limiter calls (NRK per step). */
 void oracle_limiter_record(oracle_t *o, int ncalls) {
-    free(o->lim_node); free(o->lim_elem);
+    free(o->lim_node); free(o->lim_elem); free(o->wd_rec);
     o->lim_node = (int *)xcalloc((size_t)ncalls * o->np * 6 + 1, sizeof(int));
     o->lim_elem = (int *)xcalloc((size_t)ncalls * o->ne * 3 + 1, sizeof(int));
-    o->lim_cap = ncalls; o->lim_pos = 0; o->lim_mode = 1;
+    o->wd_rec = (int *)xcalloc((size_t)ncalls * o->ne + 1, sizeof(int));
+    o->lim_cap = ncalls; o->lim_pos = 0; o->wd_pos = 0; o->lim_mode = 1;
 }
-/* Copy out / in the records of ncalls calls ([call][np][6] and [call][ne][3]). */
-int oracle_limiter_get(const oracle_t *o, int *node, int *elem) {
+/* Copy out / in the records of ncalls calls ([call][np][6], [call][ne][3] and [call][ne]). */
+int oracle_limiter_get(const oracle_t *o, int *node, int *elem, int *wd) {
     if (!o->lim_node) return 1;
     memcpy(node, o->lim_node, (size_t)o->lim_cap * o->np * 6 * sizeof(int));
     memcpy(elem, o->lim_elem, (size_t)o->lim_cap * o->ne * 3 * sizeof(int));
+    memcpy(wd, o->wd_rec, (size_t)o->lim_cap * o->ne * sizeof(int));
     return 0;
 }
-/* Replay: the next ncalls limiter calls take the given branches; WETDRY is left out (the adjoint
-   takes it as the identity). */
-void oracle_limiter_replay(oracle_t *o, int ncalls, const int *node, const int *elem) {
+/* Replay: the next ncalls limiter and WETDRY calls take the given branches. */
+void oracle_limiter_replay(oracle_t *o, int ncalls, const int *node, const int *elem, const int *wd) {
     oracle_limiter_record(o, ncalls);
     memcpy(o->lim_node, node, (size_t)ncalls * o->np * 6 * sizeof(int));
     memcpy(o->lim_elem, elem, (size_t)ncalls * o->ne * 3 * sizeof(int));
+    memcpy(o->wd_rec, wd, (size_t)ncalls * o->ne * sizeof(int));
     o->lim_mode = 2;
 }
 
@@ -1851,6 +1855,149 @@ static void wetdry(oracle_t *o) {
     }
 }
 
+/* WETDRY with its branches recorded (lim_mode 1) or replayed (lim_mode 2), the frozen-branch map
+   the adjoint differentiates.  The code of one element (wetdry_code):
+     bits 0-1  branch: 0 identity (every vertex wet, element wet), 1 every vertex wet but the element
+               dry (the check alone), 2 flat (every vertex dry, or the mean depth at most H0+ZERO),
+               3 redistribution
+     bit  2    flat: the mean depth floored at ZERO
+     bit  3    flat: the element had a dry vertex but not all three (the check follows)
+     bits 4-6  redistribution: the vertices in increasing depth, as a permutation index
+     bits 7-8  redistribution: the two fmax(H0, ...) took H0
+     bits 9-11 redistribution: the vertices dry after the redistribution
+     bit  12   the check kept the momentum (element wet)
+     bit  13   the element's flag afterwards
+   Replayed at the recording point every branch gives the free WETDRY's values bit for bit. */
+static const int PERM3[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+
+static void wetdry_frozen(oracle_t *o) {
+    int s = o->irk + 1, dofh = o->dofh, call = o->wd_pos++;
+    if (call >= o->lim_cap) { o->series_error = 1; return; }
+    int *rec = o->wd_rec + (size_t)call * o->ne;
+    const int mode = o->lim_mode;
+    const double ZERO = 1.0e-6, H0 = o->h0;
+    const double C13 = 1.0 / 3.0, C16 = 1.0 / 6.0;
+    for (int i = 0; i < o->np; i++) o->nodecode[i] = 0;
+#pragma omp parallel for num_threads(o->nthreads) schedule(static)
+    for (int J = 0; J < o->ne; J++) {
+        int no[3] = {o->nm[3 * J], o->nm[3 * J + 1], o->nm[3 * J + 2]};
+        double *ze = &o->ze[((size_t)s * o->ne + J) * dofh];
+        double *qx = &o->qx[((size_t)s * o->ne + J) * dofh];
+        double *qy = &o->qy[((size_t)s * o->ne + J) * dofh];
+        const double *dpn = &o->dp_node[3 * J];
+        double ze_node[3], ht_node[3], ze_hat[3];
+        int ndry = 0, ht_max_i = 0;
+        double ht_max = ZERO;
+        for (int I = 0; I < 3; I++) {
+            ze_node[I] = ze[0];
+            for (int K = 1; K < 3; K++) ze_node[I] = ze_node[I] + o->phi_corner[I][K] * ze[K];
+            ht_node[I] = dpn[I] + ze_node[I];
+            if (ht_node[I] > ht_max) { ht_max_i = I; ht_max = ht_node[I]; }
+            if (ht_node[I] <= (H0 + ZERO)) ndry++;
+        }
+        double dp_avg = C13 * (dpn[0] + dpn[1] + dpn[2]);
+        int code = 0, branch, check = 0;
+        if (mode == 2) {
+            code = rec[J];
+            branch = code & 3;
+        } else {
+            if (ndry == 0) branch = o->wdflg[J] == 1 ? 0 : 1;
+            else if (ndry == 3) branch = 2;
+            else branch = (ze[0] + dp_avg <= (H0 + ZERO)) ? 2 : 3;
+            code = branch;
+        }
+        if (branch == 0) {
+            code |= 1 << 13;
+        } else if (branch == 1) {
+            for (int c = 0; c < 3; c++) ze_hat[c] = ze_node[c];
+            check = 1;
+        } else if (branch == 2) {
+            double ht_avg = ze[0] + dp_avg;
+            int floor_, partial;
+            if (mode == 2) { floor_ = (code >> 2) & 1; partial = (code >> 3) & 1; }
+            else {
+                floor_ = ht_avg < ZERO; partial = ndry != 3;
+                code |= floor_ << 2 | partial << 3;
+            }
+            for (int k = 0; k < dofh; k++) ze[k] = 0.0;
+            ze[0] = floor_ ? ZERO - dp_avg : ht_avg - dp_avg;
+            ze[1] = -HB(J, 1); ze[2] = -HB(J, 2);
+            for (int k = 0; k < dofh; k++) { qx[k] = 0.0; qy[k] = 0.0; }
+            if (partial) { for (int c = 0; c < 3; c++) ze_hat[c] = ze[0]; check = 1; }
+        } else {
+            int pi, b0, b1, dmask;
+            if (mode == 2) { pi = (code >> 4) & 7; b0 = (code >> 7) & 1; b1 = (code >> 8) & 1; dmask = (code >> 9) & 7; }
+            else {
+                int l2h[3] = {0, 1, 2};
+                for (int k1 = 0; k1 < 3; k1++)
+                    for (int k2 = k1; k2 < 3; k2++)
+                        if (ht_node[l2h[k1]] > ht_node[l2h[k2]]) { int t = l2h[k1]; l2h[k1] = l2h[k2]; l2h[k2] = t; }
+                pi = 0;
+                for (int q = 0; q < 6; q++)
+                    if (PERM3[q][0] == l2h[0] && PERM3[q][1] == l2h[1]) pi = q;
+                b0 = H0 > ht_node[l2h[0]];
+                b1 = 0; dmask = 0;   /* set below */
+            }
+            const int *l2h = PERM3[pi];
+            double dh[3];
+            dh[l2h[0]] = (b0 ? H0 : ht_node[l2h[0]]) - ht_node[l2h[0]];
+            if (mode != 2) b1 = H0 > ht_node[l2h[1]] - 0.5 * dh[l2h[0]];
+            dh[l2h[1]] = (b1 ? H0 : ht_node[l2h[1]] - 0.5 * dh[l2h[0]]) - ht_node[l2h[1]];
+            dh[l2h[2]] = -dh[l2h[0]] - dh[l2h[1]];
+            double ht_hat[3], qxn[3], qyn[3];
+            int nd = 0; double qxs = 0, qys = 0; ht_max = 0;
+            for (int I = 0; I < 3; I++) {
+                ht_hat[I] = ht_node[I] + dh[I];
+                qxn[I] = qx[0]; qyn[I] = qy[0];
+                if (ht_hat[I] > ht_max) { ht_max_i = I; ht_max = ht_hat[I]; }
+                for (int K = 1; K < 3; K++) {
+                    qxn[I] = qxn[I] + o->phi_corner[I][K] * qx[K];
+                    qyn[I] = qyn[I] + o->phi_corner[I][K] * qy[K];
+                }
+                if (mode != 2 && ht_hat[I] <= (H0 + 1.0e-10)) dmask |= 1 << I;
+                if ((dmask >> I) & 1) { nd++; qxs = qxs + qxn[I]; qys = qys + qyn[I]; }
+            }
+            if (mode != 2) code |= pi << 4 | b0 << 7 | b1 << 8 | dmask << 9;
+            if (nd == 3) {
+                for (int k = 0; k < dofh; k++) { qx[k] = 0.0; qy[k] = 0.0; }
+            } else {
+                qxs = qxs / (3.0 - (double)nd);
+                qys = qys / (3.0 - (double)nd);
+                double qxh[3], qyh[3];
+                for (int I = 0; I < 3; I++) {
+                    if ((dmask >> I) & 1) { qxh[I] = 0.0; qyh[I] = 0.0; }
+                    else { qxh[I] = qxn[I] + qxs; qyh[I] = qyn[I] + qys; }
+                }
+                qx[0] = C13 * (qxh[0] + qxh[1] + qxh[2]);
+                qx[1] = -(C16 * (qxh[0] + qxh[1])) + C13 * qxh[2];
+                qx[2] = -(0.5 * qxh[0]) + 0.5 * qxh[1];
+                qy[0] = C13 * (qyh[0] + qyh[1] + qyh[2]);
+                qy[1] = -(C16 * (qyh[0] + qyh[1])) + C13 * qyh[2];
+                qy[2] = -(0.5 * qyh[0]) + 0.5 * qyh[1];
+            }
+            for (int c = 0; c < 3; c++) ze_hat[c] = ht_hat[c] - dpn[c];
+            ze[1] = -(C16 * (ze_hat[0] + ze_hat[1])) + C13 * ze_hat[2];
+            ze[2] = -(0.5 * ze_hat[0]) + 0.5 * ze_hat[1];
+            check = 1;
+        }
+        if (branch >= 2) for (int k = 3; k < dofh; k++) { ze[k] = 0.0; qx[k] = 0.0; qy[k] = 0.0; }
+        if (check) {
+            int keep;
+            if (mode == 2) keep = (code >> 12) & 1;
+            else { keep = ze_hat[ht_max_i] > (-o->dpe_min[J] + H0); code |= keep << 12 | keep << 13; }
+            if (!keep) for (int k = 0; k < dofh; k++) { qx[k] = 0.0; qy[k] = 0.0; }
+        }
+        if (mode != 2) rec[J] = code;
+        o->wdflg[J] = (code >> 13) & 1;
+        if (o->wdflg[J]) {
+            for (int c = 0; c < 3; c++) {
+#pragma omp atomic write
+                o->nodecode[no[c]] = 1;
+            }
+        }
+    }
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* time step                                                                                   */
 /* ------------------------------------------------------------------------------------------ */
@@ -1887,7 +2034,10 @@ void oracle_step(oracle_t *o, int it) {
         oracle_rhs(o, it, irk);
         rk_update(o);
         if (o->slopeflag != 0) slopelimiter5(o);
-        if (o->nolifa >= 2 && o->lim_mode != 2) wetdry(o);
+        if (o->nolifa >= 2) {
+            if (o->lim_mode != 0) wetdry_frozen(o);
+            else wetdry(o);
+        }
     }
     size_t n = (size_t)o->ne * o->dofh;
     memcpy(o->ze, o->ze + (size_t)o->nrk * n, n * sizeof(double));
@@ -2043,7 +2193,7 @@ void oracle_free(oracle_t *o) {
                     o->zemin, o->zemax, o->qxmin, o->qxmax, o->qymin, o->qymax, o->etamax,
                     o->tpk, o->amigt, o->etrf, o->fft, o->facet, o->pert, o->saltamp, o->saltpha,
                     o->tip2, o->tslam, o->tsfea, o->nbd, o->esbin, o->esbin1, o->esbin2, o->eta2bc,
-                    o->lbcodei, o->qnin, o->qnin1, o->qnin2, o->lim_node, o->lim_elem};
+                    o->lbcodei, o->qnin, o->qnin1, o->qnin2, o->lim_node, o->lim_elem, o->wd_rec};
     for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; k++) free(ptrs[k]);
     free(o);
 }

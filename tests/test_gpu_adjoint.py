@@ -176,10 +176,11 @@ def test_p2_gradient_matches_central_differences():
 
 
 def test_adjoint_rejects_runs_whose_wetting_and_drying_acts():
-    """WETDRY is differentiated with its active set frozen at the identity; a run in which it dries
-    or rewets elements is refused after the forward sweep, with the handle's state restored."""
+    """Without the slope limiter WETDRY runs inside the stage kernel and is differentiated only as
+    the identity; a run in which it dries, rewets or redistributes elements is refused after the
+    forward sweep, with the handle's state restored."""
     case = mesh.delaunay_basin(n_elem=1500, seed=33, lx=6.0e4, ly=3.6e4, depth_range=(2.0, 30.0),
-                               nsteps=10)           # limiter and wetting/drying on, dry land
+                               nsteps=10, slopeflag=0)   # wetting/drying on, dry land, no limiter
     s = gpu.GpuSolver(case)
     s.set_gauges(np.array([1, 2], np.int32), np.array([-0.5, -0.5]), np.array([-0.5, -0.5]))
     before = s.state()
@@ -191,12 +192,13 @@ def test_adjoint_rejects_runs_whose_wetting_and_drying_acts():
 
 
 def test_adjoint_refusal_restores_the_wet_dry_flags():
-    """A closed basin whose shoreline moves: WETDRY changes one element's flag at step 52 of the
-    60-step window (the oracle run below shows it).  The refused gradient call must leave both the
-    modal state and the wet/dry flags as they were, so the next step continues from there."""
+    """A closed basin whose shoreline moves, without the slope limiter: WETDRY changes elements'
+    flags during the 60-step window (the oracle run below shows it).  The refused gradient call
+    must leave both the modal state and the wet/dry flags as they were, so the next step continues
+    from there."""
     from oracle.oracle import Oracle
     from test_gpu_edge_cases import closed, wetdry_state
-    case = closed(12, 6, 1, nolifa=2, slopeflag=5)
+    case = closed(12, 6, 1, nolifa=2, slopeflag=0)
     case.dp = 2.0 - 0.0004 * case.x
     st = wetdry_state(case, 2.0)
     o = Oracle(case, threads=4)
@@ -312,5 +314,52 @@ def test_frozen_limiter_gradient_matches_frozen_branch_oracle(p, wetdry):
         fd = fd4(lambda t: loss(case.manning + t * vn, case.dp + t * vd), 1e-5)
         rel = abs(fd - ad) / abs(ad)
         print(f"p={p} wetdry={wetdry}: adjoint {ad:.12e} frozen-branch oracle fd {fd:.12e} rel {rel:.2e}", flush=True)
+        assert rel <= TOL
+    prob.solver.close()
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_active_wetting_and_drying_gradient_matches_frozen_branch_oracle(p):
+    """Wetting and drying that acts, with the slope limiter on: a closed basin whose beach rises
+    from 2 m deep to 2.8 m above the datum under a 2 m hump, so that every stage WETDRY keeps dry
+    elements flat, redistributes the water of partly wet ones and dries or floods some, and the
+    edges between wet and dry elements take the dry-side fluxes (reflection, zero gravity).  The
+    reverse takes WETDRY's branches and the dry-side cases from the forward run; the oracle
+    replays the recorded SLOPELIMITER5 and WETDRY branches (the frozen-branch map).  Its
+    fourth-order differences at step 1e-6, in directions that move the bathymetry as well as
+    Manning's n, match the GPU gradient to 1e-6.  The step is smaller than in the tests above
+    because this map has more kinks (the LLF wave-speed maximum on every wet/dry edge), whose
+    error shrinks in proportion to the step: at p=2, in the second direction (a derivative 100
+    times smaller than the others), the differences at steps 1e-3, 1e-4, 1e-5, 1e-6 and 1e-7
+    differed from the adjoint by 1.3e-3, 7.2e-5, 1.9e-5, 7.6e-7 and 1.2e-7 relative, while at p=1
+    the steps 1e-5 and 1e-6 agree to 6e-8."""
+    from oracle.oracle import Oracle
+    from test_gpu_edge_cases import closed, wetdry_state
+    nst = 60
+    case = closed(12, 6, p, nolifa=2, slopeflag=5)
+    case.dp = 2.0 - 0.0004 * case.x
+    case.manning = 0.02 + 0.01 * np.random.default_rng(4).random(case.np_)
+    st = wetdry_state(case, 2.0)
+    prob = adjoint.twin_problem(case, nsteps=nst, ng=20, seed=8, state0=st, perturb=0.5)
+    J, gn, gdp = prob.gradient(case.manning, case.dp)
+    assert np.abs(gn).max() > 0.0 and np.abs(gdp).max() > 0.0
+    # the gradient call leaves the handle where it was
+    for a, b in zip(prob.solver.state(), st):
+        np.testing.assert_array_equal(a, b)
+    o = Oracle(case, threads=8)
+    o.set_state(*st)
+    o.limiter_record(nst)
+    o.step(nst)
+    records = o.limiter_records()
+    kinds = np.bincount(records[2] & 3, minlength=4)
+    print(f"p={p}: WETDRY branches identity/check/flat/redistribution {kinds.tolist()}", flush=True)
+    assert kinds[2] > 0 and kinds[3] > 0
+    loss = frozen_oracle_loss(case, prob, 20, 8, records)
+    assert loss(case.manning, case.dp) == pytest.approx(J, rel=1e-9)
+    for vn, vd in directions(case, 3, seed=13):
+        ad = float(gn @ vn + gdp @ vd)
+        fd = fd4(lambda t: loss(case.manning + t * vn, case.dp + t * vd), 1e-6)
+        rel = abs(fd - ad) / abs(ad)
+        print(f"p={p}: adjoint {ad:.12e} frozen-branch oracle fd {fd:.12e} rel {rel:.2e}", flush=True)
         assert rel <= TOL
     prob.solver.close()
