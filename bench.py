@@ -195,15 +195,15 @@ def adjoint_leg(gpu, mesh, n_elem: int, steps: int, seed: int):
 
 def batched_adjoint_leg(mesh, n_members: int, n_elem: int, steps: int, seed: int):
     """Per-member gauge-misfit gradients (Manning n and depth) of a p=2 storm ensemble: every member
-    a seeded spherical basin under its own Holland vortex with its own gauges and perturbed-Manning
-    twin data, slope limiter and wetting/drying on with their active sets frozen from the forward
-    run, all members in one batched forward and one batched checkpointed reverse sweep:
-    wall(forward + adjoint) / wall(forward)."""
+    a seeded spherical basin with a dry coast under its own Holland vortex with its own gauges and
+    perturbed-Manning twin data, from the cold start, slope limiter and wetting/drying on with their
+    active sets frozen from the forward run, all members in one batched forward and one batched
+    checkpointed reverse sweep: wall(forward + adjoint) / wall(forward)."""
     import numpy as np
     from dgswem_amd import adjoint
     t0 = time.perf_counter()
     cases = [mesh.storm_basin(n_elem, seed=seed + 300 + i, nsteps=steps, p=2, slopeflag=5, wetdry=True,
-                              shelf=False, cfl=0.15, depth_range=(10.0, 300.0), cf=1e-4)
+                              shelf=True, cfl=0.15, depth_range=(10.0, 300.0), cf=1e-4)
              for i in range(n_members)]
     eprob, probs = adjoint.ensemble_problem(cases, nsteps=steps, ng=20, seed=seed + 400)
     for p in probs:
@@ -226,9 +226,11 @@ def batched_adjoint_leg(mesh, n_members: int, n_elem: int, steps: int, seed: int
     return {"cost_ratio": grad / fwd, "forward_s": fwd, "gradient_s": grad, "members": n_members,
             "elements": int(sum(c.ne for c in cases)), "p": 2, "steps": steps,
             "checkpoint_interval": max(1, int(round(steps ** 0.5))), "loss": J, "gradient_finite": finite,
-            "workload": f"{n_members} spherical storm basins of {n_elem} elements, p=2, limiter 5 and "
-                        "wetting/drying on (active sets frozen), own Holland vortex, 20 gauges and "
-                        "perturbed-Manning twin each; per-member gradients w.r.t. Manning n and depth"}
+            "workload": f"{n_members} spherical storm basins of {n_elem} elements with a shelf rising 2 m "
+                        "above the datum along the coast (dry land, a shoreline WETDRY redistributes), "
+                        "p=2, limiter 5 and wetting/drying on (active sets frozen), from the cold start, "
+                        "own Holland vortex, 20 gauges and perturbed-Manning twin each; per-member "
+                        "gradients w.r.t. Manning n and depth"}
 
 
 def copy_calibration() -> int:

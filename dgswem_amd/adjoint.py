@@ -77,11 +77,16 @@ def twin_problem(case: Case, nsteps: int, ng: int = 50, seed: int = 0, perturb: 
                  weight: float | None = None, interval: int = 0, state0=None) -> MisfitProblem:
     """Gauge data from a twin run whose Manning field is perturbed by a seeded smooth relative
     factor (1 + perturb * field), then a problem set up at the unperturbed parameters.  The runs
-    start from `state0` (default: a Gaussian hump at rest, see hump_state)."""
+    start from `state0` (default: a Gaussian hump at rest, see hump_state; "cold": the solver's
+    cold start, at rest with the initial dry check of src/prep_DG.F:1214-1300)."""
     s = gpu.GpuSolver(case)
     elem, r, sr = seeded_gauges(case, ng, seed)
     s.set_gauges(elem, r, sr)
-    if state0 is None:
+    if isinstance(state0, str):
+        if state0 != "cold":
+            raise ValueError(f"unknown initial state {state0!r}")
+        state0 = s.state()
+    elif state0 is None:
         state0 = hump_state(case)
     s.set_state(*state0)
     rng = np.random.default_rng(seed + 1)
@@ -99,7 +104,8 @@ def twin_problem(case: Case, nsteps: int, ng: int = 50, seed: int = 0, perturb: 
 
 
 def rest_state(case: Case):
-    """All-wet state at rest (zero elevation and flow)."""
+    """All-wet state at rest (zero elevation and flow).  Where the bottom rises above H0 this is
+    not a valid wet/dry state; twin_problem's "cold" start is."""
     z = np.zeros((case.ne, case.dofh))
     return z, z.copy(), z.copy(), np.ones(case.ne, np.int32)
 
@@ -107,14 +113,15 @@ def rest_state(case: Case):
 def ensemble_problem(cases, nsteps: int, ng: int = 20, seed: int = 0, perturb: float = 0.2,
                      state0: str = "rest"):
     """Batched twin problems: each member gets its own gauges and its own perturbed-Manning twin
-    data (as in twin_problem, seeds seed+i), all with the same loss weight 1/(nsteps*ng).  Returns
+    data (as in twin_problem, seeds seed+i), all with the same loss weight 1/(nsteps*ng), from
+    rest (the cold start: elements whose bottom rises above H0 start dry) or from a hump.  Returns
     the ensemble's MisfitProblem (gauges and data concatenated in member order) and the members'
     own problems; the gradient of the summed loss restricted to a member's nodes is that member's
     gradient."""
     w = 1.0 / (nsteps * ng)
     probs = []
     for i, c in enumerate(cases):
-        st = rest_state(c) if state0 == "rest" else hump_state(c)
+        st = "cold" if state0 == "rest" else hump_state(c)
         probs.append(twin_problem(c, nsteps=nsteps, ng=ng, seed=seed + i, perturb=perturb, weight=w,
                                   state0=st))
     ens = gpu.Ensemble([p.solver for p in probs])

@@ -9,7 +9,8 @@
 // Manning friction floor CF and the 0.9/DT friction cap pass no derivative when active, and
 // Eirik's dry-ocean-edge fix differentiates |h|.  SLOPELIMITER5 is differentiated with its active
 // set frozen (limiter_adj_kernel and the node passes after it).  WETDRY after the limiter is
-// differentiated with its branches frozen too (wetdry_adj_kernel): the wet/dry flags at the start
+// differentiated with its branches frozen too (wetdry_adj_element, run by the limiter kernel's
+// adjoint variant on the limited values it forms again): the wet/dry flags at the start
 // of each stage come from the forward run, a dry element's RHS is zero, and an edge with a dry
 // side takes the reverse of the dry-side flux case the forward values select (dry_side_adj).
 // Without the limiter WETDRY is differentiated only where it is the identity (every element wet
@@ -874,8 +875,9 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
 }
 
 // ------------------------------------------------------------------------------------------
-// WETDRY (src/wetdry.F:58-548) reversed with its branches frozen: from the state it was applied to
-// (Ul: the limited stage output) and the element's flag before it, each element retakes the
+// WETDRY (src/wetdry.F:58-548) of element e reversed with its branches frozen: from the state it
+// was applied to (U: the limited stage output's modes 1-3, formed by limit_wetdry_kernel's adjoint
+// variant), the vertex depths D and the element's flag before it, the element retakes the
 // forward's decisions operation by operation (no contraction, so that they are the forward's) --
 // which vertices are dry, the branch, the vertices' depth order, the two fmax(H0, .) choices, the
 // vertices dry after the redistribution and the check -- and applies the transpose of the linear
@@ -883,24 +885,10 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
 // cotangents of the vertex depths.
 // ------------------------------------------------------------------------------------------
 template <int P>
-__global__ void __launch_bounds__(256) wetdry_adj_kernel(ElemData E, const double* __restrict__ Ul,
-                                                         const uint8_t* __restrict__ wd_in,
-                                                         double* __restrict__ ub, double* __restrict__ gdp) {
+__device__ void wetdry_adj_element(const DevTables& T, const double (&U)[3][3], const double (&D)[3], int wdflg,
+                                   double H0, int e, int ne, double* __restrict__ ub, double* __restrict__ gdp) {
     constexpr int DOF = (P + 1) * (P + 2) / 2;
-    const DevTables& T = tab<P>();
-    const int ne = E.ne;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ne) return;
     const double ZERO = 1.0e-6, C13 = 1.0 / 3.0, C16 = 1.0 / 6.0;
-    const int sid = E.sid ? (int)E.sid[e] : 0;
-    const double H0 = E.scen[sid].h0;
-    const int wdflg = wd_in[e];
-    double U[3][3];
-#pragma unroll
-    for (int f = 0; f < 3; f++)
-#pragma unroll
-        for (int k = 0; k < 3; k++) U[f][k] = Ul[(size_t)(f * DOF + k) * ne + e];
-    const double D[3] = {E.dpv[e], E.dpv[(size_t)ne + e], E.dpv[2 * (size_t)ne + e]};
     const double dpe_min = fmin(fmin(D[0], D[1]), D[2]);
     // the forward's vertex values, in its operation order
     int ndry = 0, ht_max_i = 0;

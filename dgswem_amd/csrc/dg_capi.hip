@@ -202,9 +202,8 @@ struct Ctx {
     DevBuf<double> wseg, lnbc, lgsum;
     DevBuf<int> larg;
     // active wetting/drying under the limiter: the flags at each checkpoint and at the start of every
-    // stage of a segment, and the limited (pre-WETDRY) output of a stage
+    // stage of a segment
     DevBuf<uint8_t> ckptwd, segwd;
-    DevBuf<double> ltmp;
     bool adj_wd = false;      // WETDRY after the limiter: flags kept at the checkpoints
     bool adj_wdrev = false;   // ... and WETDRY changed an element in the forward sweep: reversed
     // graphs of one slot-rotation period, keyed by (ucur, wdcur) at capture
@@ -1203,22 +1202,26 @@ int launch_limiter(Ctx& c, double* u, const ElemData& ed, bool capturing) {
     const size_t lds = (size_t)8 * (3 * lt.nsl + 6 * lt.nql);
     int ev = prof_begin(c, K_LIMIT, capturing);
     if (c.prm.nolifa >= 2) {
-        hipLaunchKernelGGL((limit_wetdry_kernel<P, true>), dim3(ge), dim3(LIM_NT), lds, c.stream, ed, u, cm, cms,
-                           lt, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p, c.wd_active);
+        hipLaunchKernelGGL((limit_wetdry_kernel<P, true, false>), dim3(ge), dim3(LIM_NT), lds, c.stream, ed, u, cm,
+                           cms, lt, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p, c.wd_active,
+                           (double*)nullptr, (double*)nullptr);
         c.wdcur = 1 - c.wdcur;
     } else {
-        hipLaunchKernelGGL((limit_wetdry_kernel<P, false>), dim3(ge), dim3(LIM_NT), lds, c.stream, ed, u, cm, cms,
-                           lt, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p, (int*)nullptr);
+        hipLaunchKernelGGL((limit_wetdry_kernel<P, false, false>), dim3(ge), dim3(LIM_NT), lds, c.stream, ed, u, cm,
+                           cms, lt, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p, (int*)nullptr,
+                           (double*)nullptr, (double*)nullptr);
     }
     prof_end(c, ev);
     HIPCHK(hipGetLastError());
     return 0;
 }
 
-// SLOPELIMITER5 alone (no WETDRY) on u, in place: the limited output the adjoint's frozen-branch
-// WETDRY reverse starts from.
+// Frozen-branch WETDRY reverse of one stage: SLOPELIMITER5 formed again from the stage's
+// pre-limiter output wpre (not written), each element's limited modes going straight to its
+// WETDRY reverse with the stage's flags `flags`: cot (in place) becomes the cotangent of the
+// limiter's output, gdp gains the vertex-depth terms.
 template <int P>
-int launch_limiter_only(Ctx& c, double* u) {
+int launch_wetdry_rev(Ctx& c, const double* wpre, const uint8_t* flags, double* cot) {
     const int ne = c.ne;
     const int ge = (ne + LIM_NT - 1) / LIM_NT;
     if (!c.lim.elq.p) return fail("internal: slope limiter tiles not built");
@@ -1226,8 +1229,9 @@ int launch_limiter_only(Ctx& c, double* u) {
                 c.lim.nsl, c.lim.nql};
     const size_t lds = (size_t)8 * (3 * lt.nsl + 6 * lt.nql);
     const ElemData ed = elem_data(c);
-    hipLaunchKernelGGL((limit_wetdry_kernel<P, false>), dim3(ge), dim3(LIM_NT), lds, c.stream, ed, u, (const double*)u,
-                       (size_t)c.dof * ne, lt, (const uint8_t*)c.wd[c.wdcur].p, c.wd[1 - c.wdcur].p, (int*)nullptr);
+    hipLaunchKernelGGL((limit_wetdry_kernel<P, false, true>), dim3(ge), dim3(LIM_NT), lds, c.stream, ed,
+                       const_cast<double*>(wpre), wpre, (size_t)c.dof * ne, lt, flags, (uint8_t*)nullptr,
+                       (int*)nullptr, cot, c.gdp.p);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1475,12 +1479,7 @@ int adj_backward(Ctx& c, int it0, int nsteps, int interval, double weight) {
     // output, formed again from its pre-limiter output, and the flags at the start of the stage
     const bool wdrev_on = c.adj_wdrev;
     auto wdrev = [&](double* cot, const double* wpre, const uint8_t* flags) -> int {
-        HIPCHK(hipMemcpyAsync(c.ltmp.p, wpre, ssz * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-        if (launch_limiter_only<P>(c, c.ltmp.p)) return 1;
-        hipLaunchKernelGGL((wetdry_adj_kernel<P>), dim3(ge), dim3(B), 0, c.stream, ed, (const double*)c.ltmp.p, flags,
-                           cot, c.gdp.p);
-        HIPCHK(hipGetLastError());
-        return 0;
+        return launch_wetdry_rev<P>(c, wpre, flags, cot);
     };
     auto limrev = [&](double* cot, const double* wpre) -> int {
         hipLaunchKernelGGL((node_minmax_kernel<P>), dim3(gn), dim3(B), 0, c.stream, np, ne, (const int*)c.n2e_off.p,
@@ -2246,7 +2245,7 @@ int dg_adj_gradient(int h, int it, int nsteps, const double* data, double weight
         else if (changed) {
             c->adj_wdrev = true;
             c->wd_active = nullptr;
-            if (c->segwd.alloc((size_t)interval * nrk * ne) != hipSuccess || c->ltmp.alloc(ssz) != hipSuccess)
+            if (c->segwd.alloc((size_t)interval * nrk * ne) != hipSuccess)
                 rc = fail("out of device memory for the wet/dry reverse");
         }
     }

@@ -862,15 +862,25 @@ __global__ void __launch_bounds__(256) node_minmax_kernel(int np, int ne, const 
 // Without it they are read from u itself (cms = DOF*ne), whose cell means the limiter keeps.
 // Blocks of LIM_NT threads; the launch passes 8*(3*nsl + 6*nql) bytes of dynamic LDS.
 // ------------------------------------------------------------------------------------------
+// With WDADJ (the adjoint's reverse sweep; WETDRY false) the kernel writes nothing to u: each
+// element's limited modes, formed in registers exactly as the forward formed them, go to the
+// frozen-branch WETDRY reverse of that element (wetdry_adj_element), which turns the cotangent ub
+// of WETDRY's output into that of its input and adds the vertex-depth cotangents to gdp.
 constexpr int LIM_NT = 256;
 
-template <int P, bool WETDRY>
+template <int P>
+__device__ void wetdry_adj_element(const DevTables& T, const double (&U)[3][3], const double (&D)[3], int wdflg,
+                                   double H0, int e, int ne, double* __restrict__ ub, double* __restrict__ gdp);
+
+template <int P, bool WETDRY, bool WDADJ>
 __global__ void __launch_bounds__(LIM_NT) limit_wetdry_kernel(ElemData E, double* __restrict__ u,
                                                               const double* __restrict__ cmean, size_t cms,
                                                               LimTiles lt,
                                                               const uint8_t* __restrict__ wd_in,
                                                               uint8_t* __restrict__ wd_out,
-                                                              int* __restrict__ wd_active) {
+                                                              int* __restrict__ wd_active,
+                                                              double* __restrict__ ub, double* __restrict__ gdp) {
+    static_assert(!(WETDRY && WDADJ), "the WETDRY reverse starts from the limiter's output alone");
     constexpr int DOF = (P + 1) * (P + 2) / 2;
     const DevTables& T = tab<P>();
     const int ne = E.ne;
@@ -892,7 +902,7 @@ __global__ void __launch_bounds__(LIM_NT) limit_wetdry_kernel(ElemData E, double
     const int lq = lt.elq[ec];
     double D[3] = {0.0, 0.0, 0.0}, h0 = 0.0;
     int wdi = 0;
-    if (WETDRY) {
+    if (WETDRY || WDADJ) {
 #pragma unroll
         for (int v = 0; v < 3; v++) D[v] = E.dpv[(size_t)v * ne + ec];
         const int sid = E.sid ? (int)E.sid[ec] : 0;
@@ -1003,6 +1013,11 @@ __global__ void __launch_bounds__(LIM_NT) limit_wetdry_kernel(ElemData E, double
             U[iv][1] = -(1.0 / 6.0 * (v[0] + v[1])) + 1.0 / 3.0 * v[2];
             U[iv][2] = -(0.5 * v[0]) + 0.5 * v[1];
         }
+    }
+    if (WDADJ) {
+        const double U3[3][3] = {{U[0][0], U[0][1], U[0][2]}, {U[1][0], U[1][1], U[1][2]}, {U[2][0], U[2][1], U[2][2]}};
+        wetdry_adj_element<P>(T, U3, D, wdi, h0, e, ne, ub, gdp);
+        return;
     }
     if (WETDRY) {
         const double c13 = 1.0 / 3.0, c16 = 1.0 / 6.0;

@@ -139,14 +139,19 @@ typedef struct oracle {
     int fser, fser_nrec, fser_next, nvel, nfluxnodes, *lbcodei;
     double ftiminc, qtime1, qtime2, *qnin, *qnin1, *qnin2;
     int series_error;
-    /* frozen-branch SLOPELIMITER5 (a check of the adjoint): mode 1 records the branches every
-       limiter call takes, mode 2 replays recorded branches instead of taking them */
+    /* frozen-branch SLOPELIMITER5, WETDRY and internal-edge dry-side treatment (a check of the
+       adjoint): mode 1 records the branches every stage takes, mode 2 replays recorded branches
+       instead of taking them */
     int lim_mode, lim_cap, lim_pos;
     int *lim_node;   /* [call][np][6]: elements attaining each field's min and max bound, -1 none */
     int *lim_elem;   /* [call][ne][3]: per field, the clamp (2 bits per vertex), pass signs, positive
                         differences and min(redfac, redmax) choices, packed */
     int wd_pos;
     int *wd_rec;     /* [call][ne]: WETDRY's branches (wetdry_code), recorded with the limiter's */
+    int ie_pos;
+    unsigned char *ie_rec;  /* [call][nieds][negp]: each internal edge point's dry-side treatment
+                               (0 none; a dry "in": 1 reflection, 2 zero-gravity momentum flux; a
+                               dry "ex": 3 reflection, 4 zero-gravity momentum flux) */
 } oracle_t;
 
 #define ZE(s, l, k) o->ze[((size_t)(s) * o->ne + (l)) * o->dofh + (k)]
@@ -1072,6 +1077,13 @@ static void flow_edges(oracle_t *o) {
 
 static void internal_edges(oracle_t *o) {
     int irk = o->irk, dofh = o->dofh, negp = o->negp;
+    /* frozen-branch runs: this stage's dry-side records (one internal_edges call per stage, as
+       one limiter call) */
+    unsigned char *ie_calls = NULL;
+    if (o->lim_mode != 0 && o->ie_rec) {
+        int call = o->ie_pos++;
+        if (call < o->lim_cap) ie_calls = o->ie_rec + (size_t)call * o->nieds * negp;
+    }
 #pragma omp parallel for num_threads(o->nthreads) schedule(static)
     for (int L = 0; L < o->nieds; L++) {
         double rzi[MAXDOF], rxi[MAXDOF], ryi[MAXDOF], rze[MAXDOF], rxe[MAXDOF], rye[MAXDOF];
@@ -1083,6 +1095,7 @@ static void internal_edges(oracle_t *o) {
         double nx = o->cosnx[ged], ny = o->sinnx[ged], tx = -ny, ty = nx;
         double edfac_in = o->xlen[ged] / o->areas[el_in];
         double edfac_ex = o->xlen[ged] / o->areas[el_ex];
+        unsigned char *side_rec = ie_calls ? ie_calls + (size_t)L * negp : NULL;
         for (int I = 0; I < negp; I++) {
             int gi = I, gx = negp - I - 1;
             double ze_in = 0.0, qx_in = 0.0, qy_in = 0.0, ze_ex = 0.0, qx_ex = 0.0, qy_ex = 0.0;
@@ -1100,9 +1113,19 @@ static void internal_edges(oracle_t *o) {
             double f[3];
             llf(o, ze_in, ze_ex, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_ex, sfac_in, sfac_ex, nx, ny, o->g, f);
             double f_o = f[0], g_in = f[1], h_in = f[2], g_ex = f[1], h_ex = f[2];
-            if (fabs(f[0]) > 1.0e-8) {
-                if (o->wdflg[el_in] == 0) {
-                    if (f[0] > 0.0) {
+            /* the dry-side treatment (src/internal_edge_hydro.F:212-302), taken from f[0] and the
+               flags, or replayed */
+            int side = 0;
+            if (side_rec && o->lim_mode == 2) {
+                side = side_rec[I];
+            } else if (fabs(f[0]) > 1.0e-8) {
+                if (o->wdflg[el_in] == 0) side = f[0] > 0.0 ? 1 : 2;
+                else if (o->wdflg[el_ex] == 0) side = f[0] < 0.0 ? 3 : 4;
+            }
+            if (side_rec && o->lim_mode == 1) side_rec[I] = (unsigned char)side;
+            if (side != 0) {
+                if (side <= 2) {
+                    if (side == 1) {
                         double qn_ext = qx_ex * nx + qy_ex * ny, qt_ext = qx_ex * tx + qy_ex * ty;
                         double qn_int = -qn_ext, qt_int = qt_ext;
                         double den = (nx * ty - ny * tx);
@@ -1116,8 +1139,8 @@ static void internal_edges(oracle_t *o) {
                         llf(o, ze_in, ze_ex, qx_in, qx_ex, qy_in, qy_ex, hb_in, hb_ex, sfac_in, sfac_ex, nx, ny, 0.0, f);
                         g_in = f[1]; h_in = f[2];
                     }
-                } else if (o->wdflg[el_ex] == 0) {
-                    if (f[0] < 0.0) {
+                } else {
+                    if (side == 3) {
                         double qn_int = qx_in * nx + qy_in * ny, qt_int = qx_in * tx + qy_in * ty;
                         double qn_ext = -qn_int, qt_ext = qt_int;
                         double den = (nx * ty - ny * tx);
@@ -1663,28 +1686,36 @@ static void slopelimiter5_frozen(oracle_t *o) {
 
 /* Record the limiter branches of the next ncalls limiter calls (NRK per step). */
 void oracle_limiter_record(oracle_t *o, int ncalls) {
-    free(o->lim_node); free(o->lim_elem); free(o->wd_rec);
+    free(o->lim_node); free(o->lim_elem); free(o->wd_rec); free(o->ie_rec);
     o->lim_node = (int *)xcalloc((size_t)ncalls * o->np * 6 + 1, sizeof(int));
     o->lim_elem = (int *)xcalloc((size_t)ncalls * o->ne * 3 + 1, sizeof(int));
     o->wd_rec = (int *)xcalloc((size_t)ncalls * o->ne + 1, sizeof(int));
-    o->lim_cap = ncalls; o->lim_pos = 0; o->wd_pos = 0; o->lim_mode = 1;
+    o->ie_rec = (unsigned char *)xcalloc((size_t)ncalls * o->nieds * o->negp + 1, 1);
+    o->lim_cap = ncalls; o->lim_pos = 0; o->wd_pos = 0; o->ie_pos = 0; o->lim_mode = 1;
 }
-/* Copy out / in the records of ncalls calls ([call][np][6], [call][ne][3] and [call][ne]). */
-int oracle_limiter_get(const oracle_t *o, int *node, int *elem, int *wd) {
+/* Copy out / in the records of ncalls calls ([call][np][6], [call][ne][3], [call][ne] and
+   [call][nieds][negp]). */
+int oracle_limiter_get(const oracle_t *o, int *node, int *elem, int *wd, unsigned char *ie) {
     if (!o->lim_node) return 1;
     memcpy(node, o->lim_node, (size_t)o->lim_cap * o->np * 6 * sizeof(int));
     memcpy(elem, o->lim_elem, (size_t)o->lim_cap * o->ne * 3 * sizeof(int));
     memcpy(wd, o->wd_rec, (size_t)o->lim_cap * o->ne * sizeof(int));
+    memcpy(ie, o->ie_rec, (size_t)o->lim_cap * o->nieds * o->negp);
     return 0;
 }
-/* Replay: the next ncalls limiter and WETDRY calls take the given branches. */
-void oracle_limiter_replay(oracle_t *o, int ncalls, const int *node, const int *elem, const int *wd) {
+/* Replay: the next ncalls limiter, WETDRY and internal-edge dry-side decisions take the given
+   branches. */
+void oracle_limiter_replay(oracle_t *o, int ncalls, const int *node, const int *elem, const int *wd,
+                           const unsigned char *ie) {
     oracle_limiter_record(o, ncalls);
     memcpy(o->lim_node, node, (size_t)ncalls * o->np * 6 * sizeof(int));
     memcpy(o->lim_elem, elem, (size_t)ncalls * o->ne * 3 * sizeof(int));
     memcpy(o->wd_rec, wd, (size_t)ncalls * o->ne * sizeof(int));
+    memcpy(o->ie_rec, ie, (size_t)ncalls * o->nieds * o->negp);
     o->lim_mode = 2;
 }
+/* Internal edges (the size of a dry-side record's [nieds] dimension). */
+int oracle_num_internal_edges(const oracle_t *o) { return o->nieds; }
 
 static void slopelimiter5(oracle_t *o) {
     if (o->lim_mode != 0) { slopelimiter5_frozen(o); return; }
@@ -2193,7 +2224,7 @@ void oracle_free(oracle_t *o) {
                     o->zemin, o->zemax, o->qxmin, o->qxmax, o->qymin, o->qymax, o->etamax,
                     o->tpk, o->amigt, o->etrf, o->fft, o->facet, o->pert, o->saltamp, o->saltpha,
                     o->tip2, o->tslam, o->tsfea, o->nbd, o->esbin, o->esbin1, o->esbin2, o->eta2bc,
-                    o->lbcodei, o->qnin, o->qnin1, o->qnin2, o->lim_node, o->lim_elem, o->wd_rec};
+                    o->lbcodei, o->qnin, o->qnin1, o->qnin2, o->lim_node, o->lim_elem, o->wd_rec, o->ie_rec};
     for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; k++) free(ptrs[k]);
     free(o);
 }

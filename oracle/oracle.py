@@ -15,6 +15,7 @@ _LIB = None
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 
 
 def lib():
@@ -53,8 +54,10 @@ def lib():
         L.oracle_series_error.argtypes = [C.c_void_p]
         L.oracle_get_tip2.argtypes = [C.c_void_p, _dp]
         L.oracle_limiter_record.argtypes = [C.c_void_p, C.c_int]
-        L.oracle_limiter_get.argtypes = [C.c_void_p, _ip, _ip, _ip]
-        L.oracle_limiter_replay.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip]
+        L.oracle_limiter_get.argtypes = [C.c_void_p, _ip, _ip, _ip, _u8p]
+        L.oracle_limiter_replay.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _u8p]
+        L.oracle_num_internal_edges.argtypes = [C.c_void_p]
+        L.oracle_num_internal_edges.restype = C.c_int
         L.oracle_set_etamax.argtypes = [C.c_void_p, _dp]
         L.oracle_num_edges.argtypes = [C.c_void_p, _ip]
         L.oracle_num_edges.restype = C.c_int
@@ -211,26 +214,35 @@ class Oracle:
         return e, u, v, nc
 
     def limiter_record(self, nsteps: int):
-        """Record the branches SLOPELIMITER5 and WETDRY take over the next `nsteps` steps."""
+        """Record the branches SLOPELIMITER5, WETDRY and the internal edges' dry-side treatment
+        take over the next `nsteps` steps."""
         self._lim_calls = nsteps * self.nrk
         lib().oracle_limiter_record(self.h, self._lim_calls)
 
     def limiter_records(self):
-        """(node bounds [call][np][6], limiter elements [call][ne][3], WETDRY elements [call][ne])."""
+        """(node bounds [call][np][6], limiter elements [call][ne][3], WETDRY elements [call][ne],
+        internal-edge dry-side treatments [call][internal edge][edge point])."""
+        nie = lib().oracle_num_internal_edges(self.h)
+        negp = self.case.dg.ph + 2
         node = np.empty(self._lim_calls * self.np_ * 6, np.int32)
         elem = np.empty(self._lim_calls * self.ne * 3, np.int32)
         wd = np.empty(self._lim_calls * self.ne, np.int32)
-        if lib().oracle_limiter_get(self.h, node, elem, wd) != 0:
+        ie = np.empty(self._lim_calls * nie * negp, np.uint8)
+        if lib().oracle_limiter_get(self.h, node, elem, wd, ie) != 0:
             raise RuntimeError("no limiter record")
-        return node, elem, wd
+        return node, elem, wd, ie
 
     def limiter_replay(self, records):
         """The next steps take the recorded SLOPELIMITER5 and WETDRY branches: the frozen-branch
         map the adjoint differentiates."""
-        node, elem, wd = records
+        node, elem, wd, ie = records
         n = node.size // (self.np_ * 6)
+        nie = lib().oracle_num_internal_edges(self.h)
+        if ie.size != n * nie * (self.case.dg.ph + 2):
+            raise ValueError("records of another mesh or order")
         lib().oracle_limiter_replay(self.h, n, np.ascontiguousarray(node, np.int32),
-                                    np.ascontiguousarray(elem, np.int32), np.ascontiguousarray(wd, np.int32))
+                                    np.ascontiguousarray(elem, np.int32), np.ascontiguousarray(wd, np.int32),
+                                    np.ascontiguousarray(ie, np.uint8))
 
     def etamax(self):
         """ETAMAX as modal2nodal has kept it so far (-99999 before the first call)."""

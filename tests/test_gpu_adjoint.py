@@ -56,7 +56,7 @@ def test_loss_matches_forward(p1):
     case, prob, J, gn, gdp = p1
     print(f"J = {J:.6e}", flush=True)
     assert J > 0.0
-    assert J == pytest.approx(prob.loss(case.manning, case.dp), rel=1e-13)
+    assert J == pytest.approx(prob.loss(case.manning, case.dp), rel=1e-13, abs=0.0)
     assert np.all(np.isfinite(gn)) and np.all(np.isfinite(gdp))
     assert np.abs(gn).max() > 0.0 and np.abs(gdp).max() > 0.0
 
@@ -117,7 +117,7 @@ def test_torch_autograd_front_end(p1):
     prob.loss = lambda *a, **k: (calls.append("loss"), orig_l(*a, **k))[1]
     try:
         loss = adjoint.gauge_misfit(m, d, prob)
-        assert float(loss.detach()) == pytest.approx(J, rel=1e-13)
+        assert float(loss.detach()) == pytest.approx(J, rel=1e-13, abs=0.0)
         (2.0 * loss).backward()
         assert calls == ["gradient"]
         with torch.no_grad():
@@ -125,7 +125,7 @@ def test_torch_autograd_front_end(p1):
         assert calls == ["gradient", "loss"]
     finally:
         prob.gradient, prob.loss = orig_g, orig_l
-    assert float(plain) == pytest.approx(J, rel=1e-13)
+    assert float(plain) == pytest.approx(J, rel=1e-13, abs=0.0)
     np.testing.assert_array_equal(m.grad.numpy(), 2.0 * gn)
     np.testing.assert_array_equal(d.grad.numpy(), 2.0 * gdp)
 
@@ -155,7 +155,7 @@ def test_gradient_matches_cpu_oracle_differences(p1):
         return prob.weight * J
 
     j0 = oracle_loss(case.manning, case.dp)
-    assert j0 == pytest.approx(J, rel=1e-9)
+    assert j0 == pytest.approx(J, rel=1e-9, abs=0.0)
     for vn, vd in directions(case, 3, seed=11):
         ad = float(gn @ vn + gdp @ vd)
         fd = fd4(lambda t: oracle_loss(case.manning + t * vn, case.dp + t * vd))
@@ -308,7 +308,7 @@ def test_frozen_limiter_gradient_matches_frozen_branch_oracle(p, wetdry):
     records = o.limiter_records()
     assert np.mean((records[1] & 0x3F) != 0) > 0.01        # the limiter clamps on this run
     loss = frozen_oracle_loss(case, prob, 40, 8, records)
-    assert loss(case.manning, case.dp) == pytest.approx(J, rel=1e-9)
+    assert loss(case.manning, case.dp) == pytest.approx(J, rel=1e-9, abs=0.0)
     for vn, vd in directions(case, 3, seed=13):
         ad = float(gn @ vn + gdp @ vd)
         fd = fd4(lambda t: loss(case.manning + t * vn, case.dp + t * vd), 1e-5)
@@ -355,7 +355,7 @@ def test_active_wetting_and_drying_gradient_matches_frozen_branch_oracle(p):
     print(f"p={p}: WETDRY branches identity/check/flat/redistribution {kinds.tolist()}", flush=True)
     assert kinds[2] > 0 and kinds[3] > 0
     loss = frozen_oracle_loss(case, prob, 20, 8, records)
-    assert loss(case.manning, case.dp) == pytest.approx(J, rel=1e-9)
+    assert loss(case.manning, case.dp) == pytest.approx(J, rel=1e-9, abs=0.0)
     for vn, vd in directions(case, 3, seed=13):
         ad = float(gn @ vn + gdp @ vd)
         fd = fd4(lambda t: loss(case.manning + t * vn, case.dp + t * vd), 1e-6)

@@ -61,7 +61,7 @@ def test_members_equal_single_runs_bitwise(batch):
         assert np.abs(gni).max() > 0.0
         np.testing.assert_array_equal(gn[noff[i]:noff[i + 1]], gni)
         np.testing.assert_array_equal(gdp[noff[i]:noff[i + 1]], gdpi)
-    assert J == pytest.approx(Js, rel=1e-12)
+    assert J == pytest.approx(Js, rel=1e-12, abs=0.0)
 
 
 def test_batched_gradient_matches_central_differences(batch):
@@ -99,7 +99,7 @@ def test_member_gradient_matches_cpu_oracle_differences(batch):
         return p0.weight * Jo
 
     j0 = oracle_loss(c0.manning)
-    assert j0 == pytest.approx(p0.loss(c0.manning, c0.dp), rel=1e-8)
+    assert j0 == pytest.approx(p0.loss(c0.manning, c0.dp), rel=1e-8, abs=0.0)
     rng = np.random.default_rng(10)
     vn = rng.standard_normal(c0.np_) * c0.manning
     ad = float(gn[:c0.np_] @ vn)

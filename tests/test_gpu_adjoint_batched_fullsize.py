@@ -1,19 +1,23 @@
 """The batched adjoint at the size the benchmark measures it: 32 spherical storm basins of 100k
-elements at p=2 (3.17M elements), slope limiter 5 and wetting/drying on (all wet) with their
-active sets frozen from the forward run, each member under its own Holland vortex with 20 gauges
-and a perturbed-Manning twin, 1000 steps with checkpoint spacing 32: the benchmark's
-batched-adjoint workload itself.
+elements at p=2 (3.17M elements), each with a shelf that rises 2 m above the datum along its coast
+(dry land, a shoreline that WETDRY redistributes every stage), slope limiter 5 and wetting/drying
+on with their active sets frozen from the forward run, each member under its own Holland vortex
+with 20 gauges and a perturbed-Manning twin, from the cold start (at rest, the coast dried by the
+initial dry check), 1000 steps with checkpoint spacing 32: the benchmark's batched-adjoint
+workload itself.
 
+* The map the adjoint differentiates, over the whole workload: member 0's slice of the batched
+  gradient (its loss the mean square elevation at its gauges) against the oracle (FP64, CPU) with
+  every SLOPELIMITER5, WETDRY and dry-side edge branch replayed from the unperturbed run,
+  fourth-order differences at step 1e-6 in a direction that moves Manning's n and the bathymetry:
+  relative 1e-6 (measured 3.8e-8).
 * The batched gradient against fourth-order central differences of the batched GPU forward model
-  in three seeded Manning directions, relative 1e-5 at difference step 1e-5.  The forward model
-  differenced here is the real map, whose slope-limiter branches (and the LLF wave-speed maximum)
-  switch under the perturbation while the adjoint keeps the forward run's branches: on this
-  workload the two agree to 1.4e-6 .. 5.1e-6 at step 1e-5; a step of 1e-4 crosses more switches
-  (3e-6 .. 1e-2 with a moving hump), 1e-6 is dominated by rounding (2e-5).
-* The map the adjoint differentiates, at this workload's mesh size and order: one member's
-  single-run gradient over 100 steps from a Gaussian hump, against the oracle (FP64, CPU) with the
-  slope limiter's branches replayed from the unperturbed run, fourth-order differences at step
-  1e-5 in a direction that moves Manning's n and the bathymetry: relative 1e-6.
+  in three seeded Manning directions.  That forward model is the real map, which is not even
+  continuous here: from the cold start the fluxes across the shoreline are exactly zero, any
+  perturbation lifts some above the dry-side threshold, and thousands of edge points jump to the
+  reflected or zero-gravity fluxes.  The differences at step 1e-5 agreed to 2.4e-4 .. 2.7e-4, and
+  the error does not shrink with the step (in direction 0: 0.23 at 1e-4, 2.4e-4 at 1e-5, 1.4e-4 at
+  1e-6), so the bar for this check is 1e-3; on all-wet basins the same check agreed to 5e-6.
 """
 import copy
 import os
@@ -28,9 +32,10 @@ pytestmark = pytest.mark.gpu
 
 STEPS = 1000
 MEMBERS = 32
-TOL_REAL_MAP = 1e-5
+TOL_REAL_MAP = 1e-3
 TOL_FROZEN = 1e-6
 EPS = 1e-5
+EPS_FROZEN = 1e-6
 THREADS = min(16, os.cpu_count() or 1)
 
 
@@ -40,7 +45,7 @@ def fd4(loss, eps=EPS):
 
 def member_case(i, nsteps=STEPS):
     return mesh.storm_basin(100_000, seed=300 + i, nsteps=nsteps, p=2, slopeflag=5, wetdry=True,
-                            shelf=False, cfl=0.15, depth_range=(10.0, 300.0), cf=1e-4)
+                            shelf=True, cfl=0.15, depth_range=(10.0, 300.0), cf=1e-4)
 
 
 @pytest.fixture(scope="module")
@@ -56,6 +61,10 @@ def batch():
     eprob.interval = 32
     J, gn, gdp = eprob.gradient(mn, dp)
     print(f"J {J:.6e}", flush=True)
+    eprob.cases = cases
+    eprob.reset()
+    eprob.eta = eprob.solver.gauges(STEPS)         # at the same parameters
+    eprob.reset()
     yield eprob, mn, dp, J, gn, gdp
     eprob.solver.close()
 
@@ -77,42 +86,65 @@ def test_full_size_gradient_matches_central_differences(batch, d):
     assert rel <= TOL_REAL_MAP
 
 
-def test_member_gradient_matches_frozen_branch_oracle():
-    nsteps = 100
-    c0 = member_case(0, nsteps)
-    st0 = adjoint.hump_state(c0)
-    p0 = adjoint.twin_problem(c0, nsteps=nsteps, ng=20, seed=400, perturb=0.5, state0=st0)
-    J0, gn, gdp = p0.gradient(c0.manning, c0.dp)
-    p0.solver.close()
+def test_member_gradient_matches_frozen_branch_oracle(batch):
+    """Member 0 over all 1000 steps, its gauges and state the first of the batch's.  Its loss here
+    is its gauges' mean square elevation (data zero), a second batched gradient call: against the
+    twin data member 0's misfits are only about 7e-12 m (the storm barely reaches its gauges), and
+    the oracle's and the GPU's elevations differ by about 1e-15 m after 1000 steps (the Holland
+    terms use the device's transcendental functions), which moves that gradient by about 1e-4
+    (measured 2.6e-4); with zero data the same differences enter at 1e-14."""
+    import time
+    eprob, mn, dp, J, gn, gdp = batch
+    c0 = eprob.cases[0]
+    n0, ne0 = c0.np_, c0.ne
+    st0 = tuple(a[:ne0] for a in eprob.state0)
+    assert (st0[3] == 0).any()                      # the cold start dried the coast
+    data = eprob.data.copy()
+    data[:, :20] = 0.0
+    eprob.reset()
+    J2, gn2, gdp2 = eprob.solver.gradient(data, eprob.weight, eprob.interval)
+    eprob.reset()
+    eta0 = eprob.solver.gauges(STEPS)[:, :20]
+    eprob.reset()
+    J0 = eprob.weight * float(np.sum(eta0 ** 2))
     elem, r, s = adjoint.seeded_gauges(c0, 20, 400)
     phi = np.stack([basis_at(c0.dg.pl, r[g], s[g]) for g in range(len(elem))])
-    o = Oracle(c0, threads=THREADS)
-    o.set_state(*st0)
-    o.limiter_record(nsteps)
-    o.step(nsteps)
-    records = o.limiter_records()
-    assert np.mean((records[1] & 0x3F) != 0) > 0.001
 
-    def frozen_loss(m, dpt):
+    def run(m, dpt, records=None):
         c = copy.copy(c0)
         c.manning = m
         c.dp = dpt
         oc = Oracle(c, threads=THREADS)
         oc.set_state(*st0)
-        oc.limiter_replay(records)
+        if records is None:
+            oc.limiter_record(STEPS)
+        else:
+            oc.limiter_replay(records)
         Jo = 0.0
-        for k in range(nsteps):
+        for k in range(STEPS):
             oc.step(1)
             eta = np.sum(oc.state()[0][elem - 1] * phi, axis=1)
-            Jo += np.sum((eta - p0.data[k]) ** 2)
-        return p0.weight * Jo
+            Jo += np.sum(eta ** 2)
+        return eprob.weight * Jo, oc
 
-    assert frozen_loss(c0.manning, c0.dp) == pytest.approx(J0, rel=1e-10)
+    t0 = time.time()
+    Jr, o = run(c0.manning, c0.dp)
+    records = o.limiter_records()
+    del o
+    kinds = np.bincount(records[2] & 3, minlength=4)
+    print(f"oracle run {time.time() - t0:.0f} s: loss {Jr:.12e} (GPU {J0:.12e}); limiter clamps "
+          f"{np.mean((records[1] & 0x3F) != 0):.4f}; WETDRY identity/check/flat/redistribution "
+          f"{kinds.tolist()}; dry-side cases {np.bincount(records[3], minlength=5).tolist()}", flush=True)
+    assert kinds[3] > 0
+    # member 0's gauge elevations are about 1.6e-5 m (rms), the oracle's and the GPU's differ by
+    # about 1e-15 m: the losses agree to a few 1e-10 (measured 3.8e-10)
+    assert Jr == pytest.approx(J0, rel=1e-8, abs=0.0)
     rng = np.random.default_rng(12)
-    vn = rng.standard_normal(c0.np_) * c0.manning
-    vd = rng.standard_normal(c0.np_) * c0.dp
-    ad = float(gn @ vn + gdp @ vd)
-    fd = fd4(lambda t: frozen_loss(c0.manning + t * vn, c0.dp + t * vd))
+    vn = rng.standard_normal(n0) * c0.manning
+    vd = rng.standard_normal(n0) * c0.dp
+    ad = float(gn2[:n0] @ vn + gdp2[:n0] @ vd)
+    fd = fd4(lambda t: run(c0.manning + t * vn, c0.dp + t * vd, records)[0], EPS_FROZEN)
     rel = abs(fd - ad) / abs(ad)
-    print(f"frozen-branch oracle, 100k p=2: fd {fd:.12e} adjoint {ad:.12e} rel {rel:.2e}", flush=True)
+    print(f"member 0, 1000 steps, frozen-branch oracle: fd {fd:.12e} adjoint {ad:.12e} rel {rel:.2e} "
+          f"({time.time() - t0:.0f} s in all)", flush=True)
     assert rel <= TOL_FROZEN

@@ -1,5 +1,6 @@
-"""The oracle's frozen-branch map (the map the discrete adjoint differentiates): SLOPELIMITER5 and
-WETDRY with every branch recorded from a run and replayed.
+"""The oracle's frozen-branch map (the map the discrete adjoint differentiates): SLOPELIMITER5,
+WETDRY and the internal edges' dry-side treatment with every branch recorded from a run and
+replayed.
 
 * Replayed at the recording point, the frozen map gives the free run's states and wet/dry flags
   bit for bit, on a basin where elements dry and flood during the run, at p=1 and p=2.
@@ -8,8 +9,13 @@ WETDRY with every branch recorded from a run and replayed.
 * Replayed at perturbed parameters the frozen map moves smoothly: its second difference in a
   Manning-and-depth direction scales as the step squared at steps of 1e-5 (a switched branch would
   make it scale linearly).  The frozen map still has the kinks the adjoint linearises without
-  recording them (the LLF bound, the dry-side flux cases, the friction cap), which steps of 1e-4
-  and more cross on this basin.
+  recording them (the LLF bound, the friction cap), which steps of 1e-4 and more cross on this
+  basin.
+* From the cold start of a basin with a shelf (dry land along the coast, the water at rest), the
+  fluxes across the shoreline are zero, so no edge takes a dry-side case (reflection or
+  zero-gravity flux); perturbed parameters make them nonzero and switch thousands of edge points
+  to those cases, a jump of order g*H0^2/2 in the momentum flux.  With the cases replayed the
+  frozen map stays smooth there.
 """
 import copy
 
@@ -104,6 +110,43 @@ def test_frozen_map_is_smooth_at_perturbed_parameters():
     second = []
     for h in (2e-5, 1e-5):
         second.append(np.linalg.norm(run(h) - 2.0 * u0 + run(-h)))
+    ratio = second[0] / second[1]
+    print(f"second differences {second}, ratio {ratio:.2f} (4 for a smooth map)", flush=True)
+    assert 3.0 < ratio < 5.0
+
+
+def test_frozen_map_is_smooth_from_a_cold_start():
+    c = mesh.delaunay_basin(n_elem=3000, seed=5, lx=6.0e4, ly=3.6e4, depth_range=(2.0, 20.0), shelf=True,
+                            p=1, slopeflag=5, wetdry=True, nsteps=60, cf=1e-4)
+    st = Oracle(c, threads=4).state()
+    assert (st[3] == 0).any()                       # the cold start's dry check dried the coast
+    rng = np.random.default_rng(3)
+    vn = rng.standard_normal(c.np_) * c.manning
+    vd = rng.standard_normal(c.np_) * c.dp
+
+    def run(t, records=None):
+        cc = copy.copy(c)
+        cc.manning = c.manning + t * vn
+        cc.dp = c.dp + t * vd
+        oc = Oracle(cc, threads=4)
+        oc.set_state(*st)
+        if records is None:
+            oc.limiter_record(60)
+        else:
+            oc.limiter_replay(records)
+        oc.step(60)
+        return oc
+
+    o = run(0.0)
+    records = o.limiter_records()
+    assert not records[3].any()                     # no dry-side case at the recording point
+    sides = np.bincount(run(1e-5).limiter_records()[3], minlength=5)
+    print(f"dry-side cases of the free run at step 1e-5: {sides.tolist()}", flush=True)
+    assert sides[1:].sum() > 1000                   # ... but many once the parameters move
+    u0 = run(0.0, records).state()[0]
+    np.testing.assert_array_equal(u0, o.state()[0])
+    second = [np.linalg.norm(run(h, records).state()[0] - 2.0 * u0 + run(-h, records).state()[0])
+              for h in (2e-5, 1e-5)]
     ratio = second[0] / second[1]
     print(f"second differences {second}, ratio {ratio:.2f} (4 for a smooth map)", flush=True)
     assert 3.0 < ratio < 5.0
