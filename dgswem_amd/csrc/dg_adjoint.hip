@@ -436,7 +436,10 @@ __device__ __forceinline__ void elem_hb_areas(const ElemData& E, int ne, int g, 
 }
 
 // QF: quadratic bottom friction without the hybrid factor (NOLIBF=1), fixed at compile time.
-template <int P, int NT, bool SPH, bool NL, bool QF>
+// WD: the stage's wet/dry flags are read (a.wd) and dry elements and dry-side edge cases are
+// reversed; without it every element is wet and that code is compiled out (it would otherwise
+// raise the kernel's register use past what the all-wet runs need).
+template <int P, int NT, bool SPH, bool NL, bool QF, bool WD>
 __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a) {
 #pragma clang fp contract(fast)
     constexpr int DOF = (P + 1) * (P + 2) / 2;
@@ -503,7 +506,7 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
     const int info = E.sinfo[e];
     const int nb0 = E.snbr[e], nb1 = E.snbr[(size_t)ne + e], nb2 = E.snbr[2 * (size_t)ne + e];
     const int sl = a.snbl[e];
-    const int wd = a.wd ? (int)a.wd[e] : 1;
+    const int wd = WD ? (int)a.wd[e] : 1;
 #pragma unroll 1
     for (int slot = 0; slot < 3; slot++) {
         const int bits = side_bits(info, slot);
@@ -518,7 +521,7 @@ __global__ void __launch_bounds__(NT, P == 1 ? 2 : 1) adj_stage_kernel(AdjArgs a
         const double tx = -ny, ty = nx;
         if (kind == SIDE_INTERNAL) {
             const int ln = (sl >> (10 * slot)) & 1023;
-            const int wdn = a.wd ? (int)a.wd[nb] : 1;
+            const int wdn = WD ? (int)a.wd[nb] : 1;
             if (wd == 0 && wdn == 0) continue;          // the forward adds no flux between dry elements
             double V[3][DOF], Ln[3][DOF];
 #pragma unroll

@@ -130,7 +130,7 @@ struct Ctx {
     // mesh
     DevBuf<uint16_t> sid;
     DevBuf<Scen> scen;
-    DevBuf<double> xv, yv, dpv, nbar, cori_el, sfacq, sfaced, sfs;
+    DevBuf<double> xv, yv, dpv, nbar, cori_el, sfacq, sfaced, sfs, geo;
     DevBuf<int> sinfo, snbr, nm, n2e_off, n2e;
     std::vector<int> h_sinfo, h_snbr;            // host copies (staging lists)
     bool has_cori_el = false;
@@ -298,6 +298,7 @@ ElemData elem_data(Ctx& c) {
     E.sfacq = c.prm.ics == 2 ? c.sfacq.p : nullptr;
     E.sfaced = c.prm.ics == 2 ? c.sfaced.p : nullptr;
     E.sfs = c.prm.ics == 2 ? c.sfs.p : nullptr;
+    E.geo = c.geo.p;
     E.sinfo = c.sinfo.p; E.snbr = c.snbr.p; E.nm = c.nm.p;
     return E;
 }
@@ -718,6 +719,10 @@ int upload_mesh(Ctx& c) {
     HIPCHK(c.nm.upload(nm.data(), nm.size(), c.stream));
     HIPCHK(c.sinfo.upload(sinfo.data(), sinfo.size(), c.stream));
     HIPCHK(c.snbr.upload(snbr.data(), snbr.size(), c.stream));
+    HIPCHK(c.geo.alloc((size_t)13 * ne));
+    hipLaunchKernelGGL(geom_kernel, dim3((ne + 255) / 256), dim3(256), 0, c.stream, ne, (const double*)c.xv.p,
+                       (const double*)c.yv.p, (const int*)c.sinfo.p, c.geo.p);
+    HIPCHK(hipGetLastError());
     c.stg.clear();
     HIPCHK(c.n2e_off.upload(off.data(), off.size(), c.stream));
     HIPCHK(c.n2e.upload(n2e.data(), n2e.size(), c.stream));
@@ -1381,6 +1386,18 @@ int adj_check(Ctx& c) {
     return 0;
 }
 
+template <int P, int NT, bool NL, bool WD>
+int launch_adj_v(Ctx& c, const AdjArgs& a, int grid, size_t lds, bool qf) {
+    if (c.prm.ics == 2) {
+        if (qf) hipLaunchKernelGGL((adj_stage_kernel<P, NT, true, NL, NL, WD>), dim3(grid), dim3(NT), lds, c.stream, a);
+        else hipLaunchKernelGGL((adj_stage_kernel<P, NT, true, NL, false, WD>), dim3(grid), dim3(NT), lds, c.stream, a);
+    } else {
+        if (qf) hipLaunchKernelGGL((adj_stage_kernel<P, NT, false, NL, NL, WD>), dim3(grid), dim3(NT), lds, c.stream, a);
+        else hipLaunchKernelGGL((adj_stage_kernel<P, NT, false, NL, false, WD>), dim3(grid), dim3(NT), lds, c.stream, a);
+    }
+    return 0;
+}
+
 template <int P, int NT, bool NL>
 int launch_adj_nl(Ctx& c, const AdjArgs& a0) {
     const int grid = (c.ne + NT - 1) / NT;
@@ -1395,14 +1412,8 @@ int launch_adj_nl(Ctx& c, const AdjArgs& a0) {
     // NOLIBF=1 (quadratic friction, no hybrid factor) with the nonlinear equations: friction fixed
     // at compile time
     const bool qf = NL && c.phys.iflinbf == 0 && c.phys.ifhybf == 0;
-    if (c.prm.ics == 2) {
-        if (qf) hipLaunchKernelGGL((adj_stage_kernel<P, NT, true, NL, NL>), dim3(grid), dim3(NT), lds, c.stream, a);
-        else hipLaunchKernelGGL((adj_stage_kernel<P, NT, true, NL, false>), dim3(grid), dim3(NT), lds, c.stream, a);
-    } else {
-        if (qf) hipLaunchKernelGGL((adj_stage_kernel<P, NT, false, NL, NL>), dim3(grid), dim3(NT), lds, c.stream, a);
-        else hipLaunchKernelGGL((adj_stage_kernel<P, NT, false, NL, false>), dim3(grid), dim3(NT), lds, c.stream, a);
-    }
-    return 0;
+    if (a.wd) return launch_adj_v<P, NT, NL, true>(c, a, grid, lds, qf);
+    return launch_adj_v<P, NT, NL, false>(c, a, grid, lds, qf);
 }
 
 template <int P, int NT>

@@ -146,10 +146,11 @@ DG_CRM_FN double cr_cos(double x) {
 // Correctly rounded x**y for y = 1/3 rounded to a double (0x1.5555555555555p-2), for positive x
 // in [2^-1000, 2^1000]: x**y = x**(1/3) * x**(-d) with d = 1/3 - y = 2^-54/3.  The cube root is
 // refined to double-double by one Newton step from the library's cbrt, and the second factor is
-// 1 - d*ln(x) to 2^-110.  The correction terms need only modest relative accuracy (the Newton
-// quotient 2^-45, ln(x) 2^-20: their errors enter at 2^-97 and 2^-75 of the result), so the device
-// forms them in single precision.  Outside that range as pow: +0 for +-0, NaN for negative x; the
-// library's cube root otherwise.
+// 1 - d*ln(x) to 2^-110.  The correction terms need only modest accuracy (the Newton quotient
+// 2^-45 relative, ln(x) 2^-20 absolute: their errors enter at 2^-97 and 2^-75 of the result), so
+// they are formed in single precision: on the device from the hardware's approximate reciprocal
+// and base-2 logarithm (about 1 ulp each), ln(x) as ln(2) * (exponent + log2(significand)).
+// Outside that range as pow: +0 for +-0, NaN for negative x; the library's cube root otherwise.
 DG_CRM_FN double cr_pow_third(double x) {
     const double c0 = std::cbrt(x);
     if (!(x >= 0x1p-1000 && x <= 0x1p+1000)) return x < 0.0 ? (x - x) / (x - x) : (x == 0.0 ? 0.0 : c0);
@@ -158,14 +159,16 @@ DG_CRM_FN double cr_pow_third(double x) {
     const double e = (t.hi - x) + (t.lo + c0 * s.lo);     // c0^3 - x (t.hi - x is exact)
     const double a = 3.0 * s.hi;
 #if defined(__HIP_DEVICE_COMPILE__)
-    const float ra = __frcp_rn((float)a), lnf = __logf((float)x);
+    const float ra = __builtin_amdgcn_rcpf((float)a);
+    const float l2 = (float)__builtin_amdgcn_frexp_exp(x) + __builtin_amdgcn_logf((float)__builtin_amdgcn_frexp_mant(x));
+    const double eps = (double)l2 * (0.6931471805599453 * 0x1.5555555555555p-56);   // ln(x) * 2^-54 / 3
 #else
-    const float ra = 1.0f / (float)a, lnf = std::log((float)x);
+    const float ra = 1.0f / (float)a;
+    const double eps = (double)std::log((float)x) * 0x1.5555555555555p-56;         // ln(x) * 2^-54 / 3
 #endif
     double r = (double)ra;
     r = r * (2.0 - a * r);                                 // 1/a to 2^-46
     const double d = e * r;                                // Newton: cbrt(x) = c0 - d
-    const double eps = (double)lnf * 0x1.5555555555555p-56; // ln(x) * 2^-54 / 3
     return c0 + (-d - c0 * eps);
 }
 

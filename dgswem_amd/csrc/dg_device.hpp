@@ -77,6 +77,8 @@ struct ElemData {
     const int* __restrict__ sinfo;      // [ne] packed side descriptors
     const int* __restrict__ snbr;       // [3][ne] neighbour element / boundary-list index per slot
     const int* __restrict__ nm;         // [3][ne] 0-based vertex nodes
+    const double* __restrict__ geo;     // [13][ne] per slot s: rows 3s..3s+2 the side's unit normal
+                                        // (NX, NY) and XLEN/AREAS; rows 9..12 DRDX, DSDX, DRDY, DSDY
 };
 
 // Boundary forcing.  Per-stage scalars come from host-built tables [scenario][row][constituent];

@@ -188,6 +188,56 @@ __device__ __forceinline__ bool wetdry_keeps(const DevTables& T, const double (&
 }
 
 // ------------------------------------------------------------------------------------------
+// Element geometry, once per mesh (src/calc_normal.F:28-49, src/prep_DG.F:959-1163): per slot the
+// side's unit normal in the reference's node order (the counter-clockwise order of the edge's
+// NEDEL(1) element, so both elements of an internal edge form the same normal from the same two
+// vertices) and XLEN/AREAS, and the Jacobian terms DRDX, DSDX, DRDY, DSDY.  The stage kernel
+// reads them instead of forming them every stage; the expressions are the ones it would evaluate,
+// so the values are the same bits.
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) geom_kernel(int ne, const double* __restrict__ xv, const double* __restrict__ yv,
+                                                   const int* __restrict__ sinfo, double* __restrict__ geo) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    const double X[3] = {xv[e], xv[(size_t)ne + e], xv[2 * (size_t)ne + e]};
+    const double Y[3] = {yv[e], yv[(size_t)ne + e], yv[2 * (size_t)ne + e]};
+    const double AREAS = (X[0] - X[2]) * (Y[1] - Y[2]) + (X[2] - X[1]) * (Y[0] - Y[2]);
+    const int info = sinfo[e];
+    for (int slot = 0; slot < 3; slot++) {
+        const int bits = side_bits(info, slot);
+        const int L = bits & 3, role = (bits >> 2) & 1;
+        int va = (L + 1) % 3, vb = (L + 2) % 3;
+        if (role) { const int t = va; va = vb; vb = t; }
+        const double dy = Y[vb] - Y[va];
+        const double dx = X[vb] - X[va];
+        const double xlen = sqrt(dy * dy + dx * dx);
+        geo[(size_t)(3 * slot) * ne + e] = dy / xlen;
+        geo[(size_t)(3 * slot + 1) * ne + e] = -dx / xlen;
+        geo[(size_t)(3 * slot + 2) * ne + e] = xlen / AREAS;
+    }
+    double AREA = (X[0] - X[2]) * (Y[1] - Y[2]) + (X[2] - X[1]) * (Y[0] - Y[2]);
+    AREA = AREA * 0.5;
+    geo[(size_t)9 * ne + e] = 1.0 / AREA * (Y[2] - Y[0]);
+    geo[(size_t)10 * ne + e] = 1.0 / AREA * (Y[0] - Y[1]);
+    geo[(size_t)11 * ne + e] = 1.0 / AREA * (X[0] - X[2]);
+    geo[(size_t)12 * ne + e] = 1.0 / AREA * (X[1] - X[0]);
+}
+
+// XLEN of a side and AREAS (twice the element's area), formed from the vertex coordinates as the
+// geometry kernel forms them: the boundary-edge terms need them apart.
+__device__ __forceinline__ void side_xlen_areas(const ElemData& E, int ne, int e, int L, int role, double& xlen,
+                                                double& AREAS) {
+    const double X[3] = {E.xv[e], E.xv[(size_t)ne + e], E.xv[2 * (size_t)ne + e]};
+    const double Y[3] = {E.yv[e], E.yv[(size_t)ne + e], E.yv[2 * (size_t)ne + e]};
+    AREAS = (X[0] - X[2]) * (Y[1] - Y[2]) + (X[2] - X[1]) * (Y[0] - Y[2]);
+    int va = (L + 1) % 3, vb = (L + 2) % 3;
+    if (role) { const int t = va; va = vb; vb = t; }
+    const double dy = Y[vb] - Y[va];
+    const double dx = X[vb] - X[va];
+    xlen = sqrt(dy * dy + dx * dx);
+}
+
+// ------------------------------------------------------------------------------------------
 // Stage kernel.  Reference statements restated:
 //   OCEAN_EDGE_HYDRO   src/ocean_edge_hydro.F:19-355
 //   LAND_EDGE_HYDRO    src/land_edge_hydro.F:26-311
@@ -249,8 +299,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
     // that the latencies overlap (lanes past the last element read element ne-1 and exit after
     // the barrier)
     const int ec = e < ne ? e : ne - 1;
-    const double X0 = E.xv[ec], X1 = E.xv[(size_t)ne + ec], X2 = E.xv[2 * (size_t)ne + ec];
-    const double Y0 = E.yv[ec], Y1 = E.yv[(size_t)ne + ec], Y2 = E.yv[2 * (size_t)ne + ec];
+    double G[9];                         // per slot NX, NY, XLEN/AREAS
+#pragma unroll
+    for (int j = 0; j < 9; j++) G[j] = E.geo[(size_t)j * ne + ec];
     const int info = E.sinfo[ec];
     const int nb0 = E.snbr[ec], nb1 = E.snbr[(size_t)ne + ec], nb2 = E.snbr[2 * (size_t)ne + ec];
     const int sl = a.snbl[ec];
@@ -301,7 +352,6 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
     const int row = table_step(a.fo) * a.fo.nrk + a.irk;
 
     const int wd = stwd[tid];
-    const double AREAS = (X0 - X2) * (Y1 - Y2) + (X2 - X1) * (Y0 - Y2);
 #pragma unroll
     for (int j = 0; j < 3 * DOF; j++) { acc_lds[0][j][tid] = 0.0; acc_lds[1][j][tid] = 0.0; }
 
@@ -311,14 +361,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
         const int L = bits & 3, role = (bits >> 2) & 1, kind = (bits >> 3) & 3, nled = (bits >> 5) & 3;
         const int nb = slot == 0 ? nb0 : (slot == 1 ? nb1 : nb2);
         // edge geometry in the reference's node order (the counter-clockwise order of NEDEL(1))
-        int va = (L + 1) % 3, vb = (L + 2) % 3;
-        if (role) { const int t = va; va = vb; vb = t; }
-        const double xa = va == 0 ? X0 : (va == 1 ? X1 : X2), ya = va == 0 ? Y0 : (va == 1 ? Y1 : Y2);
-        const double xb = vb == 0 ? X0 : (vb == 1 ? X1 : X2), yb = vb == 0 ? Y0 : (vb == 1 ? Y1 : Y2);
-        const double dy = yb - ya;
-        const double dx = xb - xa;
-        const double xlen = sqrt(dy * dy + dx * dx);
-        const double nx = dy / xlen, ny = -dx / xlen;
+        const double nx = slot == 0 ? G[0] : (slot == 1 ? G[3] : G[6]);
+        const double ny = slot == 0 ? G[1] : (slot == 1 ? G[4] : G[7]);
+        const double edfac = slot == 0 ? G[2] : (slot == 1 ? G[5] : G[8]);
         const double tx = -ny, ty = nx;
         if (kind == SIDE_INTERNAL) {
             const int ln = (sl >> (10 * slot)) & 1023;
@@ -355,7 +400,6 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
             }
             // the "in" side subtracts w*F and the "ex" side adds it; -(a*b) == (-a)*b exactly, so
             // the sign goes on the edge factor once and one update serves both roles
-            const double edfac = xlen / AREAS;
             const double edfs = role ? edfac : -edfac;
             double acc[3][DOF];
 #pragma unroll
@@ -424,6 +468,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
                 for (int k = 0; k < DOF; k++) acc_lds[1][f * DOF + k][tid] = acc_lds[1][f * DOF + k][tid] + acc[f][k];
         } else {
             if (wd == 0) continue;
+            double xlen, AREAS;
+            side_xlen_areas(E, ne, e, L, role, xlen, AREAS);
             double U[3][DOF], hb[3];
 #pragma unroll
             for (int f = 0; f < 3; f++)
@@ -510,6 +556,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
     double sfqv[NAGP];
 #pragma unroll
     for (int I = 0; I < NAGP; I++) sfqv[I] = SPH ? E.sfacq[(size_t)I * ne + e] : 1.0;
+    double jac[4];                       // DRDX, DSDX, DRDY, DSDY
+#pragma unroll
+    for (int j = 0; j < 4; j++) jac[j] = E.geo[(size_t)(9 + j) * ne + e];
     const bool pre0 = a.nprev > 0 && a.aprev[0] != 0.0;
     double B0[3][DOF];
     if (pre0) {
@@ -528,15 +577,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
             for (int k = 0; k < DOF; k++) R[0][k] = R[0][k] + T.srfac[I][k] * sc.rain;
     }
     if (wd != 0) {
-        double drdx, dsdx, drdy, dsdy;
-        {
-            double AREA = (X0 - X2) * (Y1 - Y2) + (X2 - X1) * (Y0 - Y2);
-            AREA = AREA * 0.5;
-            drdx = 1.0 / AREA * (Y2 - Y0);
-            dsdx = 1.0 / AREA * (Y0 - Y1);
-            drdy = 1.0 / AREA * (X0 - X2);
-            dsdy = 1.0 / AREA * (X1 - X0);
-        }
+        const double drdx = jac[0], dsdx = jac[1], drdy = jac[2], dsdy = jac[3];
         // wind stress at the element's vertices and the pressure gradient sums (MET)
         double mw[8];
         if (MET) {
