@@ -7,7 +7,8 @@ the states stay within round-off (about 3e-14 after 1000 steps at p=1).
 * the 997,306-element tidal basin (limiter 5, wetting/drying, five constituents): 1000 steps at
   p=1, 200 steps at p=2;
 * eight members of the 64-member storm ensemble (the benchmark's first eight sizes), batched with
-  graph replay: 200 steps.
+  graph replay: 200 steps, or all 1000 of the ensemble configuration with DGSWEM_FULL_CHECKS=1
+  (about two minutes of oracle time more).
 Bar: relative L2 <= 1e-10 in elevation and velocity and identical wet/dry masks at every check.
 """
 import os
@@ -51,7 +52,7 @@ def test_full_size_long_horizon(p, steps, chunk):
 def test_storm_members_long_horizon():
     rng = np.random.default_rng(0)
     sizes = np.exp(rng.uniform(np.log(20_000), np.log(300_000), 64)).astype(int)[:8]
-    steps, chunk = 200, 50
+    steps, chunk = (1000, 250) if os.environ.get("DGSWEM_FULL_CHECKS") == "1" else (200, 50)
     cases = [mesh.storm_basin(int(n), seed=100 + i, nsteps=steps) for i, n in enumerate(sizes)]
     print(f"storm members: {len(cases)}, {sum(c.ne for c in cases)} elements, {steps} steps", flush=True)
     solvers = [gpu.GpuSolver(c) for c in cases]
