@@ -633,7 +633,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
             dhbx = dhbx + hb[k] * dpxk;
             dhby = dhby + hb[k] * dpyk;
         }
-#pragma unroll 1
+#pragma unroll
         for (int I = 0; I < NAGP; I++) {
             const double* pa = T.phi_area[I];
             double ze = U[0][0], qx = U[1][0], qy = U[2][0];
