@@ -633,7 +633,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(P == 1 
             dhbx = dhbx + hb[k] * dpxk;
             dhby = dhby + hb[k] * dpyk;
         }
-#pragma unroll
+        // p=1: the three area points unrolled, so that their long dependent chains overlap (at p=2
+        // the six points unrolled run slower)
+#pragma unroll(P == 1 ? NAGP : 1)
         for (int I = 0; I < NAGP; I++) {
             const double* pa = T.phi_area[I];
             double ze = U[0][0], qx = U[1][0], qy = U[2][0];
