@@ -363,3 +363,23 @@ def test_active_wetting_and_drying_gradient_matches_frozen_branch_oracle(p):
         print(f"p={p}: adjoint {ad:.12e} frozen-branch oracle fd {fd:.12e} rel {rel:.2e}", flush=True)
         assert rel <= TOL
     prob.solver.close()
+
+
+def test_active_wetting_and_drying_gradient_independent_of_checkpoint_spacing():
+    """The wet/dry flags the reverse sweep reads are kept per checkpoint and per stage of each
+    recomputed segment; the gradient must not depend on where the segments start (spacings that
+    divide the 60 steps and ones that do not), bit for bit."""
+    from test_gpu_edge_cases import closed, wetdry_state
+    case = closed(12, 6, 1, nolifa=2, slopeflag=5)
+    case.dp = 2.0 - 0.0004 * case.x
+    case.manning = 0.02 + 0.01 * np.random.default_rng(4).random(case.np_)
+    st = wetdry_state(case, 2.0)
+    prob = adjoint.twin_problem(case, nsteps=60, ng=20, seed=8, state0=st, perturb=0.5)
+    J, gn, gdp = prob.gradient(case.manning, case.dp)
+    for interval in (1, 7, 13, 60):
+        prob.interval = interval
+        J2, gn2, gdp2 = prob.gradient(case.manning, case.dp)
+        assert J2 == J
+        np.testing.assert_array_equal(gn2, gn)
+        np.testing.assert_array_equal(gdp2, gdp)
+    prob.solver.close()
