@@ -383,3 +383,46 @@ def test_active_wetting_and_drying_gradient_independent_of_checkpoint_spacing():
         np.testing.assert_array_equal(gn2, gn)
         np.testing.assert_array_equal(gdp2, gdp)
     prob.solver.close()
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_storm_with_a_dry_coast_gradient_matches_frozen_branch_oracle(p):
+    """A spherical storm basin (NWS=8 Holland forcing) whose coast the cold start dries, with a
+    hump added on the wet elements: the wet/dry variant of the spherical Holland adjoint kernel,
+    against the frozen-branch oracle (SLOPELIMITER5, WETDRY and the dry-side edge cases replayed).
+    The loss is the gauges' mean square elevation (data zero), so that last-bit differences of
+    the Holland terms between the GPU and the oracle stay far below the misfit."""
+    from oracle.oracle import Oracle
+    nst = 60
+    case = mesh.storm_basin(2000, seed=66, nsteps=nst, p=p, slopeflag=5, wetdry=True, shelf=True,
+                            cfl=0.15 if p == 2 else 0.3, lx=3.0e5, ly=2.0e5, depth_range=(3.0, 20.0), cf=1e-4)
+    g = gpu.GpuSolver(case)
+    cold = g.state()
+    g.close()
+    assert (cold[3] == 0).any()
+    hump = adjoint.hump_state(case, amplitude=0.5)
+    ze = cold[0].copy()
+    wet = cold[3] == 1
+    ze[wet, 0] += hump[0][wet, 0]
+    st = (ze, cold[1], cold[2], cold[3])
+    prob = adjoint.twin_problem(case, nsteps=nst, ng=20, seed=9, state0=st, perturb=0.5)
+    prob.data = np.zeros_like(prob.data)
+    J, gn, gdp = prob.gradient(case.manning, case.dp)
+    o = Oracle(case, threads=8)
+    o.set_state(*st)
+    o.limiter_record(nst)
+    o.step(nst)
+    records = o.limiter_records()
+    kinds = np.bincount(records[2] & 3, minlength=4)
+    sides = np.bincount(records[3], minlength=5)
+    print(f"p={p}: WETDRY branches {kinds.tolist()}, dry-side cases {sides.tolist()}", flush=True)
+    assert kinds[2] + kinds[3] > 0
+    loss = frozen_oracle_loss(case, prob, 20, 9, records)
+    assert loss(case.manning, case.dp) == pytest.approx(J, rel=1e-9, abs=0.0)
+    for vn, vd in directions(case, 3, seed=14):
+        ad = float(gn @ vn + gdp @ vd)
+        fd = fd4(lambda t: loss(case.manning + t * vn, case.dp + t * vd), 1e-6)
+        rel = abs(fd - ad) / abs(ad)
+        print(f"p={p}: adjoint {ad:.12e} frozen-branch oracle fd {fd:.12e} rel {rel:.2e}", flush=True)
+        assert rel <= TOL
+    prob.solver.close()
